@@ -1,0 +1,21 @@
+"""MI355X-native tiled Gaussian-splatting frame pipeline.
+
+The package name starts with a digit, so import it with
+``importlib.import_module("3dgs_tile_webgpu_amd")``.
+
+Public API (mirrors the reference's pass API for the frame path):
+  gaussian_pass / GaussianPass   -- one camera's pass over the C ABI
+  PackedScene, workload_scene    -- packed buffers and synthetic workloads
+  PerspectiveCamera              -- Three.js camera conventions
+"""
+from ._native import GsError, GsRangeError, GsTypeError, lib  # noqa: F401
+from .camera import PerspectiveCamera, elements, orbit_cameras  # noqa: F401
+from .gaussian_pass import (  # noqa: F401
+    GaussianPass,
+    IntersectionCapacityPolicy,
+    frame_for,
+    gaussian_pass,
+    object_frames_for,
+    validate_options,
+)
+from .scene import WORKLOADS, PackedScene, Workload, generate, spec_for, workload_scene  # noqa: F401

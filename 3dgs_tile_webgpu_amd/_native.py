@@ -1,0 +1,209 @@
+"""ctypes declarations of the C ABI in include/gs.h and include/gs_producer.h.
+
+The library is the in-tree ``_lib/libgs.so`` built by ``build.py``. Loading
+fails loudly when it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libgs.so"
+
+GS_OK = 0
+GS_E_RANGE = -1
+GS_E_TYPE = -2
+GS_E_OOM = -3
+GS_E_DEVICE = -4
+GS_E_STATE = -5
+
+GS_DEPTH_FLOAT32 = 0
+GS_DEPTH_PACKED16 = 1
+GS_AA_COMPENSATED = 0
+GS_AA_CLASSIC = 1
+GS_BOUNDARY = {"clip": 0, "repeat-x": 1, "repeat-y": 2}
+GS_MEMORY_HOST = 0
+GS_MEMORY_DEVICE = 1
+
+u32 = C.c_uint32
+f32 = C.c_float
+P = C.c_void_p
+
+
+class GsConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("depth_sort_mode", u32),
+        ("antialias_mode", u32),
+        ("subpixel_sample_culling", u32),
+        ("raster_chunk_size", u32),
+        ("intersection_capacity", u32),
+        ("max_intersection_capacity", u32),
+        ("output_depth", u32),
+        ("transmittance_threshold", f32),
+        ("depth_alpha_threshold", f32),
+        ("invalid_depth_value", f32),
+        ("background", f32 * 4),
+        ("raster_stats", u32),
+    ]
+
+
+class GsDataDesc(C.Structure):
+    _fields_ = [
+        ("count", u32),
+        ("active_count", u32),
+        ("geometry_format", u32),
+        ("sh_format", u32),
+        ("sh_degree", u32),
+        ("memory", u32),
+        ("means", P),
+        ("scales_opacity", P),
+        ("rotations", P),
+        ("sh", P),
+        ("active_slots", P),
+    ]
+
+
+class GsFrame(C.Structure):
+    _fields_ = [
+        ("projection", f32 * 16),
+        ("width", u32),
+        ("height", u32),
+        ("near_plane", f32),
+        ("far_plane", f32),
+        ("filter_variance", f32),
+        ("boundary_axis", u32),
+        ("attempt", u32),
+        ("reset_valid", u32),
+    ]
+
+
+class GsImage(C.Structure):
+    _fields_ = [("color", P), ("depth", P), ("memory", u32)]
+
+
+class GsState(C.Structure):
+    _fields_ = [
+        ("attempt", u32),
+        ("version", u32),
+        ("requested", u32),
+        ("overflow", u32),
+        ("visible", u32),
+        ("intersections", u32),
+        ("capacity", u32),
+        ("tiles_x", u32),
+        ("tiles_y", u32),
+        ("raster_checked", C.c_uint64),
+        ("raster_blended", C.c_uint64),
+    ]
+
+
+class GsSceneSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("count", u32),
+        ("clusters", u32),
+        ("extent", f32),
+        ("cluster_sigma", f32),
+        ("log_scale_mean", f32),
+        ("log_scale_sigma", f32),
+        ("log_scale_min", f32),
+        ("log_scale_max", f32),
+        ("anisotropic", u32),
+        ("major_mean", f32),
+        ("major_sigma", f32),
+        ("minor_mean", f32),
+        ("minor_sigma", f32),
+        ("opacity_sigma", f32),
+        ("sh_degree", u32),
+        ("dc_mean", f32),
+        ("dc_sigma", f32),
+        ("band_sigma", f32),
+        ("near_fraction", f32),
+        ("near_clusters", u32),
+        ("near_depth_min", f32),
+        ("near_depth_max", f32),
+        ("near_sigma", f32),
+        ("camera_position", C.c_double * 3),
+        ("camera_target", C.c_double * 3),
+        ("camera_fov_deg", C.c_double),
+        ("camera_aspect", C.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/*.h.
+SIGNATURES = {
+    "gs_config_default": (None, [C.POINTER(GsConfig)]),
+    "gs_create": (C.c_int, [C.POINTER(GsConfig), C.POINTER(P)]),
+    "gs_destroy": (None, [P]),
+    "gs_last_error": (C.c_char_p, [P]),
+    "gs_set_data": (C.c_int, [P, C.POINTER(GsDataDesc)]),
+    "gs_update_active": (C.c_int, [P, u32, u32, P, u32]),
+    "gs_set_objects": (C.c_int, [P, u32, P]),
+    "gs_object_frame": (None, [P, P, C.c_int, P]),
+    "gs_render": (C.c_int, [P, C.POINTER(GsFrame), C.POINTER(GsImage)]),
+    "gs_render_batch": (C.c_int, [P, u32, C.POINTER(GsFrame), C.POINTER(GsImage)]),
+    "gs_read_state": (C.c_int, [P, C.POINTER(GsState)]),
+    "gs_resize_intersections": (C.c_int, [P, u32]),
+    "gs_read_image": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t]),
+    "gs_read_debug": (C.c_int, [P, C.c_char_p, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "gs_stream": (P, [P]),
+    "gs_device_image": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
+    "gs_capacity_next": (u32, [u32, u32, u32, C.c_double, C.c_double]),
+    "gs_encode_compact": (None, [C.c_size_t, P, P, P, P, P]),
+    "gs_encode_quat_oct101012": (u32, [P]),
+    "gs_pack_sh_rgb8e8": (u32, [P]),
+    "gs_encode_sh_rgb8e8": (None, [C.c_size_t, P, P]),
+    "gs_f32_to_f16": (C.c_uint16, [f32]),
+    "gs_scene_spec_default": (None, [C.POINTER(GsSceneSpec)]),
+    "gs_generate_scene": (C.c_int, [C.POINTER(GsSceneSpec), P, P, P, P, P, P, P]),
+}
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """The loaded native library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(or python 3dgs_tile_webgpu_amd/build.py); there is no fallback path"
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+class GsError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class GsRangeError(GsError, ValueError):
+    """RangeError of the reference: a value outside its domain."""
+
+
+class GsTypeError(GsError, TypeError):
+    """TypeError of the reference: an unsupported format or variant."""
+
+
+def check(code: int, ctx=None) -> None:
+    if code == GS_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        raw = lib().gs_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    if code == GS_E_RANGE:
+        raise GsRangeError(code, msg or "range error")
+    if code == GS_E_TYPE:
+        raise GsTypeError(code, msg or "type error")
+    names = {GS_E_OOM: "allocation-failed", GS_E_DEVICE: "device error", GS_E_STATE: "invalid state"}
+    raise GsError(code, msg or names.get(code, "error"))

@@ -1,0 +1,67 @@
+"""Build recipe for the native library (libgs.so) of this package.
+
+Compiles the gfx950 kernels, the C ABI and the host-side producer with
+hipcc into ``_lib/libgs.so`` next to this file. Floating-point contraction
+is disabled everywhere (the kernels must round exactly like the CPU
+restatement they are checked against), and packed FP32 instructions are
+disabled for the device code.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+LIB_DIR = HERE / "_lib"
+LIB_PATH = LIB_DIR / "libgs.so"
+ROOT = HERE.parent
+
+SOURCES = ["gs_kernels.hip", "gs_api.hip", "gs_producer.cpp"]
+HEADERS = ["gs_internal.h", "gs_project.h", "gs_math.h"]
+
+ARCH = os.environ.get("GS_OFFLOAD_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build libgs")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source into _lib/libgs.so (always recompiles)."""
+    LIB_DIR.mkdir(exist_ok=True)
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({proc.returncode}):\n{proc.stderr[-8000:]}")
+    warnings = [l for l in proc.stderr.splitlines() if "warning" in l and "packed-fp32" not in l]
+    if verbose and warnings:
+        print("\n".join(warnings), file=sys.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

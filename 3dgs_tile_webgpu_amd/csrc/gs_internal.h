@@ -1,0 +1,117 @@
+// gs_internal.h - types shared by the gfx950 kernels and the host driver.
+#ifndef GS_INTERNAL_H
+#define GS_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gs {
+
+constexpr int kBlock = 256;          // threads per block (4 wave64)
+constexpr int kTile = 16;            // tile edge in pixels (constants.ts:1)
+constexpr int kItems = 16;           // keys per thread in scans and sorts
+constexpr int kPartition = kBlock * kItems;  // 4096 keys per block
+constexpr int kRadixBits = 8;        // digit width of the LSD sorts
+constexpr int kRadix = 1 << kRadixBits;
+
+// Per-frame uniforms, uploaded once per frame (FrameUniforms.ts:41-58 plus
+// the stage constants the reference keeps in uniforms).
+struct FrameParams {
+  float projection[16];
+  float viewport[4];  // width, height, near, far
+  float filter_variance;
+  float transmittance_threshold;
+  float background[4];
+  float depth_alpha_threshold;
+  float invalid_depth_value;
+  uint32_t boundary_axis;
+  uint32_t active_count;
+  uint32_t tiles_x, tiles_y, tile_count;
+  uint32_t width, height;
+  uint32_t capacity;
+  uint32_t attempt, reset_valid;
+  uint32_t object_count;
+  uint32_t sh_degree;
+  uint32_t depth_mode;             // 0 float32 keys, 1 packed16
+  uint32_t antialias_compensated;
+  uint32_t subpixel_culling;
+  uint32_t chunk_size;             // 0 = no chunking
+  uint32_t has_active_slots;
+  uint32_t output_depth;
+  uint32_t raster_stats;
+};
+
+// Device-side counters of one frame (dispatch state + outputState).
+struct DeviceState {
+  uint32_t visible;   // V
+  uint32_t total;     // requested K (saturating)
+  uint32_t count;     // K published (0 on overflow)
+  uint32_t overflow;
+  uint32_t out_attempt, out_version, out_demand, out_overflow;
+  unsigned long long checked, blended;
+};
+
+struct SceneView {
+  const float4* means;
+  const uint2* scales_opacity;
+  const uint32_t* rotations;
+  const uint32_t* sh;
+  const uint32_t* active_slots;
+  const float* objects;
+};
+
+struct ProjectedView {
+  float4* mean;
+  float4* conic;
+  float4* color;
+  uint32_t* tile_counts;
+};
+
+// Length of a device-sized sequence: (*ptr rounded up to a multiple of
+// 2^log2_div, divided by it) * mul, or `constant` when ptr is null.
+struct LenSpec {
+  const uint32_t* ptr;
+  uint32_t log2_div;  // 0 = no division
+  uint32_t mul;
+  uint32_t constant;
+};
+
+enum ScanMode : uint32_t {
+  kScanPlain = 0,
+  kScanVisible = 1,        // total -> state.visible
+  kScanIntersections = 2,  // total -> state.total, count, overflow, output
+};
+
+struct FrameBuffers {
+  FrameParams params;
+  DeviceState* state;
+  SceneView scene;
+  ProjectedView proj;
+  uint32_t* visible_offsets;
+  uint2* depth_a;
+  uint2* depth_b;
+  uint32_t* ordered_counts;
+  uint32_t* intersection_offsets;
+  uint2* records_a;
+  uint2* records_b;
+  uint32_t* tile_offsets;
+  uint32_t* hist;       // kRadix * ceil(max(N, capacity) / kPartition)
+  uint32_t* partials;   // scan partials
+  uint16_t* color_out;  // W*H*4 binary16
+  float* depth_out;     // W*H f32 or null
+  uint32_t max_n;       // slot capacity of the per-Gaussian arrays
+  uint32_t max_k;       // intersection capacity
+  uint32_t max_tiles;
+  uint32_t depth_passes;
+  uint32_t tile_passes;
+};
+
+// Enqueue one frame on `stream` (all kernels; sizes read on the device).
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream);
+// Location of the sorted depth / tile records after launch_frame.
+const uint2* sorted_depth_records(const FrameBuffers& b);
+const uint2* sorted_tile_records(const FrameBuffers& b);
+
+}  // namespace gs
+
+#endif

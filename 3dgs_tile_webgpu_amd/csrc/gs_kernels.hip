@@ -1,0 +1,762 @@
+// gs_kernels.hip - gfx950 kernels of the tiled Gaussian-splatting frame.
+//
+// Stage map (reference file:line -> kernel here):
+//   projection         ProjectionStage.ts:385-649         -> k_project
+//   visible scan+compact scan.ts:78-83, visibility.ts:37-61 -> k_scan_reduce /
+//                      k_scan_partials / k_scan_downsweep (fused compaction)
+//   depth radix sort   RadixSorter.ts:215-313, radix.ts    -> k_radix_hist /
+//                      plain scan / k_radix_scatter (8-bit digits)
+//   ordered counts + intersection scan visibility.ts:63-76, scan.ts
+//                      -> the same scan kernels with a gathering load
+//   prepare dispatch / output state intersections.ts:10-41, 114-134
+//                      -> folded into k_scan_partials (kScanIntersections)
+//   emission           intersections.ts:43-111             -> k_emit
+//   tile radix sort    TiledGaussianPipeline.ts:442-446    -> radix kernels
+//   tile offsets       tileOffsets.ts:4-128                -> k_tile_offsets
+//   raster + chunks + composite TileRasterizer.ts:477-964  -> k_raster
+//
+// Every size that the reference reads through indirect dispatch (V, K) is
+// read from DeviceState on the device, so the frame can be replayed as a
+// captured graph; grids are sized by capacity and idle blocks exit at once.
+#include "gs_internal.h"
+#include "gs_project.h"
+
+namespace gs {
+
+// ------------------------------------------------------------ utilities ----
+
+GS_DEV uint32_t sat_add(uint32_t a, uint32_t b) {
+  uint32_t s = a + b;
+  return s < a ? 0xffffffffu : s;
+}
+
+GS_DEV uint32_t lane_id() { return __lane_id(); }
+
+GS_DEV uint32_t eval_len(const LenSpec& l) {
+  if (!l.ptr) return l.constant;
+  uint32_t v = *l.ptr;
+  if (l.log2_div) v = (uint32_t)(((uint64_t)v + (1u << l.log2_div) - 1u) >> l.log2_div);
+  return v * l.mul;
+}
+
+// Inclusive saturating scan across one wave64.
+GS_DEV uint32_t wave_inclusive_scan(uint32_t v) {
+  uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v = sat_add(v, o);
+  }
+  return v;
+}
+
+// Exclusive saturating scan across the 256-thread block; *total gets the sum.
+GS_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds4) {
+  uint32_t incl = wave_inclusive_scan(v);
+  uint32_t prev = __shfl_up(incl, 1, 64);
+  uint32_t excl = lane_id() == 0 ? 0u : prev;
+  uint32_t wave = threadIdx.x >> 6;
+  if (lane_id() == 63) lds4[wave] = incl;
+  __syncthreads();
+  uint32_t w0 = lds4[0], w1 = lds4[1], w2 = lds4[2], w3 = lds4[3];
+  uint32_t before = 0;
+  if (wave > 0) before = w0;
+  if (wave > 1) before = sat_add(before, w1);
+  if (wave > 2) before = sat_add(before, w2);
+  *total = sat_add(sat_add(w0, w1), sat_add(w2, w3));
+  __syncthreads();
+  return sat_add(before, excl);
+}
+
+// -------------------------------------------------------------- project ----
+
+__global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
+                                                    DeviceState* __restrict__ state,
+                                                    SceneView sc, ProjectedView out) {
+  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  if (gid == 0) {
+    state->checked = 0ull;
+    state->blended = 0ull;
+  }
+  if (gid >= fp.active_count) return;
+  out.tile_counts[gid] = 0u;
+  out.mean[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t slot = fp.has_active_slots ? sc.active_slots[gid] : gid;
+  float4 mean = sc.means[slot];
+  if (mean.w < 0.0f) return;
+  uint32_t object = gs_f2u(mean.w);
+  if (object >= fp.object_count) return;
+  const float* frame = sc.objects + (size_t)object * 40;
+  if (!(frame[36] > 0.0f)) return;
+  uint2 so = sc.scales_opacity[slot];
+  float scale[3] = {gs_expf(gs_half_to_float(so.x & 0xffffu)),
+                    gs_expf(gs_half_to_float(so.x >> 16)),
+                    gs_expf(gs_half_to_float(so.y & 0xffffu))};
+  float base_opacity = gs_half_to_float(so.y >> 16);
+  float rot[4];
+  decode_rotation(sc.rotations[slot], rot);
+  float mv[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) mv[c][r] = frame[16 + c * 4 + r];
+  float cam[3] = {frame[32], frame[33], frame[34]};
+  float local[3] = {mean.x, mean.y, mean.z};
+  float dir[3] = {local[0] - cam[0], local[1] - cam[1], local[2] - cam[2]};
+  {
+    float len = gs_sqrtf(dot3(dir, dir));
+    dir[0] = dir[0] / len;
+    dir[1] = dir[1] / len;
+    dir[2] = dir[2] / len;
+  }
+  float view[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float s = mv[0][i] * local[0];
+    s = s + mv[1][i] * local[1];
+    s = s + mv[2][i] * local[2];
+    s = s + mv[3][i] * 1.0f;
+    view[i] = s;
+  }
+  float cv[3][3];
+  view_covariance(scale, rot, mv, cv);
+  Projected p;
+  perspective_projection(view, cv, fp, p);
+  if (!p.valid) return;
+  float opacity = gs_clampf(base_opacity, 0.0f, 1.0f);
+  if (fp.antialias_compensated)
+    opacity = opacity * gs_sqrtf(gs_clampf(p.original_det / p.det, 0.0f, 1.0f));
+  if (opacity < (float)(1.0 / 255.0) || opacity != opacity) return;
+  float thr = gs_logf(opacity * 255.0f);
+  float ext_x = gs_sqrtf(thr * 2.0f * gs_clampf(p.cov[0], 1e-12f, 1e4f));
+  float ext_y = gs_sqrtf(thr * 2.0f * gs_clampf(p.cov[2], 1e-12f, 1e4f));
+  float rx = gs_ceilf(ext_x), ry = gs_ceilf(ext_y);
+  if (rx <= 0.0f || ry <= 0.0f) return;
+  float bminx = p.cx - rx, bminy = p.cy - ry, bmaxx = p.cx + rx, bmaxy = p.cy + ry;
+  float vw = fp.viewport[0], vh = fp.viewport[1];
+  uint32_t axis = fp.boundary_axis;
+  if ((axis != 1 && (bmaxx < 0.0f || bminx >= vw)) ||
+      (axis != 2 && (bmaxy < 0.0f || bminy >= vh)))
+    return;
+  if (fp.subpixel_culling &&
+      !subpixel_has_sample(p.cx, p.cy, p.conic, thr, ext_x, ext_y, gs_f2u(vw), gs_f2u(vh),
+                           axis)) {
+    out.mean[gid] = make_float4(p.cx, p.cy, p.depth, -1.0f);
+    return;
+  }
+  float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
+  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx / kTileF), 0.0f, max_tx));
+  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy / kTileF), 0.0f, max_ty));
+  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx / kTileF), 0.0f, max_tx));
+  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy / kTileF), 0.0f, max_ty));
+  if (axis == 1) {
+    tmin_x = 0;
+    tmax_x = (int)fp.tiles_x - 1;
+  }
+  if (axis == 2) {
+    tmin_y = 0;
+    tmax_y = (int)fp.tiles_y - 1;
+  }
+  uint32_t count = 0;
+  for (int ty = tmin_y; ty <= tmax_y; ty++)
+    for (int tx = tmin_x; tx <= tmax_x; tx++)
+      if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis)) count++;
+  if (count == 0) return;
+  // SH colour (projectionHelpers.ts:118-176), direction = normalized
+  // object-space offset from the camera.
+  uint32_t degree = fp.sh_degree;
+  uint32_t ncoef = (degree + 1) * (degree + 1);
+  const uint32_t* shc = sc.sh + (size_t)slot * ncoef;
+  float x = dir[0], y = dir[1], z = dir[2];
+  float k[3];
+  decode_sh(shc[0], k);
+  float w0 = 0.28209479177387814f;
+  float col[3] = {w0 * k[0], w0 * k[1], w0 * k[2]};
+#define GS_SH_ADD(wexpr, idx)                       \
+  {                                                 \
+    float w_ = (wexpr);                             \
+    decode_sh(shc[idx], k);                         \
+    col[0] = col[0] + w_ * k[0];                    \
+    col[1] = col[1] + w_ * k[1];                    \
+    col[2] = col[2] + w_ * k[2];                    \
+  }
+  if (degree >= 1) {
+    GS_SH_ADD(-0.4886025119029199f * y, 1);
+    GS_SH_ADD(0.4886025119029199f * z, 2);
+    GS_SH_ADD(-0.4886025119029199f * x, 3);
+  }
+  if (degree >= 2) {
+    float xx = x * x, yy = y * y, zz = z * z;
+    GS_SH_ADD(1.0925484305920792f * x * y, 4);
+    GS_SH_ADD(-1.0925484305920792f * y * z, 5);
+    GS_SH_ADD(0.31539156525252005f * (2.0f * zz - xx - yy), 6);
+    GS_SH_ADD(-1.0925484305920792f * x * z, 7);
+    GS_SH_ADD(0.5462742152960396f * (xx - yy), 8);
+  }
+  if (degree >= 3) {
+    float xx = x * x, yy = y * y, zz = z * z;
+    GS_SH_ADD(-0.5900435899266435f * y * (3.0f * xx - yy), 9);
+    GS_SH_ADD(2.890611442640554f * x * y * z, 10);
+    GS_SH_ADD(-0.4570457994644658f * y * (4.0f * zz - xx - yy), 11);
+    GS_SH_ADD(0.3731763325901154f * z * (2.0f * zz - 3.0f * xx - 3.0f * yy), 12);
+    GS_SH_ADD(-0.4570457994644658f * x * (4.0f * zz - xx - yy), 13);
+    GS_SH_ADD(1.445305721320277f * z * (xx - yy), 14);
+    GS_SH_ADD(-0.5900435899266435f * x * (xx - 3.0f * yy), 15);
+  }
+#undef GS_SH_ADD
+  float r = gs_clampf(col[0] + 0.5f, 0.0f, 1.0f);
+  float g = gs_clampf(col[1] + 0.5f, 0.0f, 1.0f);
+  float b = gs_clampf(col[2] + 0.5f, 0.0f, 1.0f);
+  out.mean[gid] = make_float4(p.cx, p.cy, p.depth, opacity);
+  out.conic[gid] = make_float4(p.conic[0], p.conic[1], p.conic[2], rx + ry * 512.0f);
+  out.color[gid] = make_float4(r, g, b, gs_b2f(slot ^ 0x3f800000u));
+  out.tile_counts[gid] = count;
+}
+
+// ----------------------------------------------------------------- scans ----
+//
+// Reduce-then-scan over partitions of kPartition elements: k_scan_reduce
+// writes one saturating sum per partition, k_scan_partials scans those in a
+// single block, k_scan_downsweep rescans each partition with its offset and
+// hands (index, exclusive prefix, value) to a store functor. Saturating
+// addition is associative, so the result equals the reference's serial
+// Blelloch scan (scan.ts:27-62) bit for bit.
+
+struct LoadVisible {  // scan.ts:78-83: 1 if projectedMean.w > 0
+  const float4* mean;
+  GS_DEV uint32_t operator()(uint32_t i) const { return mean[i].w > 0.0f ? 1u : 0u; }
+};
+struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
+  const uint2* sorted;
+  const uint32_t* counts;
+  GS_DEV uint32_t operator()(uint32_t r) const { return counts[sorted[r].y]; }
+};
+struct LoadPlain {
+  const uint32_t* v;
+  GS_DEV uint32_t operator()(uint32_t i) const { return v[i]; }
+};
+
+struct StoreCompact {  // visibility.ts:37-61 (+ visibleOffsets)
+  const float4* mean;
+  uint32_t depth_mode;
+  float near, far;
+  uint32_t* offsets;
+  uint2* records;
+  GS_DEV void operator()(uint32_t i, uint32_t prefix, uint32_t flag) const {
+    offsets[i] = prefix;
+    if (!flag) return;
+    float depth = mean[i].z;
+    uint32_t key;
+    if (depth_mode == 0) {
+      key = gs_f2b(depth);
+    } else {
+      float q = gs_clampf((depth - near) / (far - near), 0.0f, 1.0f);
+      key = gs_f2u(__builtin_rintf(q * 65535.0f));
+    }
+    records[prefix] = make_uint2(key, i);
+  }
+};
+struct StoreOrdered {
+  uint32_t* ordered;
+  uint32_t* offsets;
+  GS_DEV void operator()(uint32_t r, uint32_t prefix, uint32_t value) const {
+    ordered[r] = value;
+    offsets[r] = prefix;
+  }
+};
+struct StorePlain {
+  uint32_t* v;
+  GS_DEV void operator()(uint32_t i, uint32_t prefix, uint32_t) const { v[i] = prefix; }
+};
+
+template <class Load>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(Load load, LenSpec len_spec,
+                                                        uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds4[4];
+  uint32_t len = eval_len(len_spec);
+  uint32_t base = blockIdx.x * (uint32_t)kPartition;
+  if (base >= len) return;
+  uint32_t first = base + threadIdx.x * kItems;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++)
+    if (first + k < len) sum = sat_add(sum, load(first + k));
+  uint32_t total;
+  block_exclusive_scan(sum, &total, lds4);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// One block of 1024 threads scans ceil(len / kPartition) partition sums in
+// chunks of 1024 with a running carry. Mode selects what the total feeds.
+__global__ __launch_bounds__(1024) void k_scan_partials(LenSpec len_spec,
+                                                        uint32_t* __restrict__ partials,
+                                                        ScanMode mode, uint32_t capacity,
+                                                        uint32_t attempt, uint32_t reset_valid,
+                                                        DeviceState* __restrict__ state) {
+  __shared__ uint32_t wave_sums[16];
+  __shared__ uint32_t carry_s;
+  uint32_t len = eval_len(len_spec);
+  uint32_t nparts = (uint32_t)(((uint64_t)len + kPartition - 1) / kPartition);
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nparts; base += 1024) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < nparts ? partials[i] : 0u;
+    uint32_t incl = wave_inclusive_scan(v);
+    uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t before = carry_s;
+    for (uint32_t w = 0; w < wave; w++) before = sat_add(before, wave_sums[w]);
+    uint32_t prev = __shfl_up(incl, 1, 64);
+    uint32_t excl = lane_id() == 0 ? 0u : prev;
+    if (i < nparts) partials[i] = sat_add(before, excl);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t c = carry_s;
+      for (uint32_t w = 0; w < 16; w++) c = sat_add(c, wave_sums[w]);
+      carry_s = c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  uint32_t total = carry_s;
+  if (mode == kScanVisible) {
+    state->visible = total;
+  } else if (mode == kScanIntersections) {
+    // intersections.ts:10-41 and :114-134: never publish a truncated set.
+    uint32_t overflow = total > capacity ? 1u : 0u;
+    state->total = total;
+    state->count = overflow ? 0u : total;
+    state->overflow = overflow;
+    uint32_t last_version = reset_valid ? 0u : state->out_version;
+    state->out_attempt = attempt;
+    state->out_version = overflow ? last_version : attempt;
+    state->out_demand = total;
+    state->out_overflow = overflow;
+  }
+}
+
+template <class Load, class Store>
+__global__ __launch_bounds__(kBlock) void k_scan_downsweep(Load load, Store store,
+                                                           LenSpec len_spec,
+                                                           const uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds4[4];
+  uint32_t len = eval_len(len_spec);
+  uint32_t base = blockIdx.x * (uint32_t)kPartition;
+  if (base >= len) return;
+  uint32_t first = base + threadIdx.x * kItems;
+  uint32_t vals[kItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    vals[k] = first + k < len ? load(first + k) : 0u;
+    sum = sat_add(sum, vals[k]);
+  }
+  uint32_t total;
+  uint32_t prefix = block_exclusive_scan(sum, &total, lds4);
+  prefix = sat_add(partials[blockIdx.x], prefix);
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    if (first + k < len) store(first + k, prefix, vals[k]);
+    prefix = sat_add(prefix, vals[k]);
+  }
+}
+
+// ----------------------------------------------------------- radix sort ----
+//
+// Stable LSD sort of (key, value) pairs with 8-bit digits. A pass is a
+// per-partition digit histogram laid out [digit][partition], its exclusive
+// scan (the plain scan kernels), then a scatter that ranks keys stably
+// inside the partition: 16 rounds of 256 consecutive keys, each wave finds
+// its same-digit peers with 9 ballots, waves are ordered through LDS.
+
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint2* __restrict__ in,
+                                                       const uint32_t* __restrict__ count_ptr,
+                                                       uint32_t shift,
+                                                       uint32_t* __restrict__ hist) {
+  __shared__ uint32_t bins[kRadix];
+  uint32_t count = *count_ptr;
+  uint32_t nparts = (count + kPartition - 1) / kPartition;
+  if (blockIdx.x >= nparts) return;
+  bins[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t base = blockIdx.x * (uint32_t)kPartition;
+#pragma unroll 4
+  for (int r = 0; r < kItems; r++) {
+    uint32_t i = base + r * kBlock + threadIdx.x;
+    if (i < count) atomicAdd(&bins[(in[i].x >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nparts + blockIdx.x] = bins[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint2* __restrict__ in,
+                                                          uint2* __restrict__ out,
+                                                          const uint32_t* __restrict__ count_ptr,
+                                                          uint32_t shift,
+                                                          const uint32_t* __restrict__ hist) {
+  __shared__ uint32_t running[kRadix];
+  __shared__ uint32_t wave_cnt[4][kRadix];
+  uint32_t count = *count_ptr;
+  uint32_t nparts = (count + kPartition - 1) / kPartition;
+  if (blockIdx.x >= nparts) return;
+  uint32_t t = threadIdx.x;
+  running[t] = hist[t * nparts + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+  __syncthreads();
+  uint32_t wave = t >> 6, lane = lane_id();
+  uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint32_t base = blockIdx.x * (uint32_t)kPartition;
+  for (int r = 0; r < kItems; r++) {
+    uint32_t i = base + r * kBlock + t;
+    bool valid = i < count;
+    uint2 rec = valid ? in[i] : make_uint2(0u, 0u);
+    uint32_t d = valid ? ((rec.x >> shift) & (kRadix - 1)) : (uint32_t)kRadix;
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int bit = 0; bit <= kRadixBits; bit++) {
+      uint64_t bal = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? bal : ~bal;
+    }
+    uint32_t rank = __popcll(peers & lt_mask);
+    bool leader = (peers >> lane) == 1ull;  // highest lane among its peers
+    if (valid && leader) wave_cnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank;
+      for (uint32_t w = 0; w < wave; w++) pos += wave_cnt[w][d];
+      out[pos] = rec;
+    }
+    __syncthreads();
+    running[t] += wave_cnt[0][t] + wave_cnt[1][t] + wave_cnt[2][t] + wave_cnt[3][t];
+#pragma unroll
+    for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- emission ----
+
+// intersections.ts:43-111: per depth rank, row-major bbox tiles, the same
+// exact tile test as the projection; sentinel (T, id) pads reserved slots.
+__global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
+                                                 const DeviceState* __restrict__ state,
+                                                 const uint2* __restrict__ sorted_depth,
+                                                 const float4* __restrict__ pmean,
+                                                 const float4* __restrict__ pconic,
+                                                 const uint32_t* __restrict__ ordered,
+                                                 const uint32_t* __restrict__ offsets,
+                                                 uint2* __restrict__ records) {
+  uint32_t rank = blockIdx.x * kBlock + threadIdx.x;
+  if (state->overflow || rank >= state->visible) return;
+  uint32_t capacity = fp.capacity;
+  uint32_t gid = sorted_depth[rank].y;
+  float4 mean = pmean[gid];
+  float4 conic4 = pconic[gid];
+  uint32_t packed = gs_f2u(conic4.w);
+  float rx = (float)(packed % 512u), ry = (float)(packed / 512u);
+  float conic[3] = {conic4.x, conic4.y, conic4.z};
+  float thr = gs_logf(mean.w * 255.0f);
+  int max_tx = (int)fp.tiles_x - 1, max_ty = (int)fp.tiles_y - 1;
+  int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) / kTileF)), 0), max_tx);
+  int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) / kTileF)), 0), max_ty);
+  int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) / kTileF)), 0), max_tx);
+  int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) / kTileF)), 0), max_ty);
+  uint32_t axis = fp.boundary_axis;
+  if (axis == 1) {
+    tmin_x = 0;
+    tmax_x = max_tx;
+  }
+  if (axis == 2) {
+    tmin_y = 0;
+    tmax_y = max_ty;
+  }
+  uint32_t reserved = ordered[rank];
+  uint32_t base = offsets[rank];
+  uint32_t local = 0;
+  float vw = fp.viewport[0], vh = fp.viewport[1];
+  for (int ty = tmin_y; ty <= tmax_y; ty++)
+    for (int tx = tmin_x; tx <= tmax_x; tx++)
+      if (tile_hit(mean.x, mean.y, conic, thr, rx, ry, tx, ty, vw, vh, axis)) {
+        if (local < reserved && base + local < capacity)
+          records[base + local] = make_uint2((uint32_t)ty * fp.tiles_x + (uint32_t)tx, gid);
+        local++;
+      }
+  for (uint32_t pad = local; pad < reserved; pad++)
+    if (base + pad < capacity) records[base + pad] = make_uint2(fp.tile_count, gid);
+}
+
+// ------------------------------------------------------------ tile ranges ----
+
+// tileOffsets.ts:4-128 in one pass: record i owns the tiles in
+// (tile[i-1], tile[i]], so every tile gets the index of the first record of
+// the first non-empty tile at or after it -- the suffix minimum -- and
+// off[T] ends the last range (K, or the first sentinel).
+__global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
+                                                         const DeviceState* __restrict__ state,
+                                                         const uint2* __restrict__ sorted,
+                                                         uint32_t* __restrict__ off) {
+  uint32_t count = state->count;
+  uint32_t tiles = fp.tile_count;
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i > count) return;
+  uint32_t lo = i == 0 ? 0u : sorted[i - 1].x + 1u;
+  uint32_t hi = i == count ? tiles : sorted[i].x;
+  if (hi > tiles) hi = tiles;
+  for (uint32_t t = lo; t <= hi; t++) off[t] = i;
+}
+
+// ---------------------------------------------------------------- raster ----
+//
+// One 256-thread block per 16x16 tile; each wave owns an 8x8 quadrant.
+// Gaussians are staged 256 at a time in LDS (structure of arrays); each
+// pixel blends front to back and stops when T < threshold; the block leaves
+// a batch loop once every pixel has stopped. Tiles with more records than
+// chunk_size are blended as consecutive independent chunks (each from
+// T = 1) and composited in order, exactly as the reference's chunk tasks
+// and composite pass (TileRasterizer.ts:331-412, 807-902).
+
+__global__ __launch_bounds__(kBlock) void k_raster(const FrameParams fp,
+                                                   DeviceState* __restrict__ state,
+                                                   const uint2* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ off,
+                                                   const float4* __restrict__ pmean,
+                                                   const float4* __restrict__ pconic,
+                                                   const float4* __restrict__ pcolor,
+                                                   uint16_t* __restrict__ color_out,
+                                                   float* __restrict__ depth_out) {
+  __shared__ float s_cx[kBlock], s_cy[kBlock], s_depth[kBlock], s_op[kBlock];
+  __shared__ float s_c0[kBlock], s_c1[kBlock], s_c2[kBlock], s_thr[kBlock];
+  __shared__ float s_r[kBlock], s_g[kBlock], s_b[kBlock];
+  if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
+  uint32_t tile = blockIdx.x;
+  if (tile >= fp.tile_count) return;
+  uint32_t t = threadIdx.x;
+  uint32_t wave = t >> 6, lane = t & 63u;
+  uint32_t lx = (wave & 1u) * 8u + (lane & 7u);
+  uint32_t ly = (wave >> 1) * 8u + (lane >> 3);
+  uint32_t tx = tile % fp.tiles_x, ty = tile / fp.tiles_x;
+  uint32_t x = tx * kTile + lx, y = ty * kTile + ly;
+  bool active = x < fp.width && y < fp.height;
+  float pxc = (float)x + 0.5f, pyc = (float)y + 0.5f;
+  float vw = fp.viewport[0], vh = fp.viewport[1];
+  uint32_t axis = fp.boundary_axis;
+  float threshold = fp.transmittance_threshold;
+  uint32_t begin = off[tile], end = off[tile + 1];
+  uint32_t count = end - begin;
+  uint32_t cs = fp.chunk_size;
+  bool chunked = cs != 0 && count > cs;
+  uint32_t nchunks = chunked ? (count + cs - 1) / cs : 1u;
+  float acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f, acc_d = 0.0f, trans = 1.0f;
+  uint32_t checked = 0, blended = 0;
+  for (uint32_t chunk = 0; chunk < nchunks; chunk++) {
+    uint32_t cb = chunked ? begin + chunk * cs : begin;
+    uint32_t ce = chunked ? min(cb + cs, end) : end;
+    float lr = 0.0f, lg = 0.0f, lb = 0.0f, ld = 0.0f, lt = 1.0f;
+    bool done = !active || (chunked && trans < threshold);
+    for (uint32_t batch = cb; batch < ce; batch += kBlock) {
+      if (__syncthreads_and(done)) break;
+      uint32_t idx = batch + t;
+      if (idx < ce) {
+        uint32_t gid = sorted[idx].y;
+        float4 m = pmean[gid];
+        float4 c = pconic[gid];
+        float4 k = pcolor[gid];
+        s_cx[t] = m.x;
+        s_cy[t] = m.y;
+        s_depth[t] = m.z;
+        s_op[t] = m.w;
+        s_thr[t] = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
+        s_c0[t] = c.x;
+        s_c1[t] = c.y;
+        s_c2[t] = c.z;
+        s_r[t] = k.x;
+        s_g[t] = k.y;
+        s_b[t] = k.z;
+      }
+      __syncthreads();
+      uint32_t n = min((uint32_t)kBlock, ce - batch);
+      if (!done) {
+        for (uint32_t j = 0; j < n; j++) {
+          checked++;
+          float c0 = s_c0[j], c1 = s_c1[j], c2 = s_c2[j];
+          float dx, dy;
+          if (axis == 0) {
+            dx = pxc - s_cx[j];
+            dy = pyc - s_cy[j];
+          } else {
+            boundary_delta(pxc, pyc, s_cx[j], s_cy[j], c0, c1, c2, vw, vh, axis, dx, dy);
+          }
+          float power = (c0 * (dx * dx) + c1 * 2.0f * dx * dy + c2 * (dy * dy)) * -0.5f;
+          if (power > 0.0f || power < -s_thr[j]) continue;
+          float alpha = gs_clampf(s_op[j] * gs_expf(power), 0.0f, 0.99f);
+          if (alpha < (float)(1.0 / 255.0)) continue;
+          float contribution = lt * alpha;
+          lr = lr + s_r[j] * contribution;
+          lg = lg + s_g[j] * contribution;
+          lb = lb + s_b[j] * contribution;
+          ld = ld + s_depth[j] * contribution;
+          blended++;
+          lt = lt * (1.0f - alpha);
+          if (lt < threshold) {
+            done = true;
+            break;
+          }
+        }
+      }
+    }
+    if (!chunked) {
+      acc_r = lr;
+      acc_g = lg;
+      acc_b = lb;
+      acc_d = ld;
+      trans = lt;
+    } else if (active && !(trans < threshold)) {
+      acc_r = acc_r + lr * trans;
+      acc_g = acc_g + lg * trans;
+      acc_b = acc_b + lb * trans;
+      acc_d = acc_d + ld * trans;
+      trans = trans * lt;
+    }
+  }
+  if (fp.raster_stats) {
+    // One atomic per wave keeps the counters cheap.
+    unsigned long long c64 = checked, b64 = blended;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      c64 += __shfl_xor(c64, d, 64);
+      b64 += __shfl_xor(b64, d, 64);
+    }
+    if (lane == 0) {
+      atomicAdd(&state->checked, c64);
+      atomicAdd(&state->blended, b64);
+    }
+  }
+  if (!active) return;
+  // storeFinalPixel (TileRasterizer.ts:930-964)
+  float bga = gs_clampf(fp.background[3], 0.0f, 1.0f);
+  float o0 = acc_r + fp.background[0] * trans * bga;
+  float o1 = acc_g + fp.background[1] * trans * bga;
+  float o2 = acc_b + fp.background[2] * trans * bga;
+  float o3 = 1.0f - trans * (1.0f - bga);
+  size_t p = (size_t)y * fp.width + x;
+  uint2 packed = make_uint2(gs_float_to_half(o0) | (gs_float_to_half(o1) << 16),
+                            gs_float_to_half(o2) | (gs_float_to_half(o3) << 16));
+  reinterpret_cast<uint2*>(color_out)[p] = packed;
+  if (depth_out) {
+    // finalizeGaussianDepth.ts:18-68 with the default depth node
+    // (GaussianContextNodes.ts:181-189).
+    float galpha = 1.0f - trans;
+    float od = 0.0f;
+    if (galpha > 0.0f) od = acc_d / galpha;
+    auto finite = [](float v) { return v == v && gs_fabsf(v) <= kMaxF32; };
+    bool scalar_valid = galpha > 0.0f && finite(acc_d) && finite(od) && od > 0.0f;
+    bool valid = scalar_valid && galpha > fp.depth_alpha_threshold;
+    float dev = fp.invalid_depth_value;
+    if (valid) {
+      float near = fp.viewport[2], far = fp.viewport[3];
+      float vz = -od;
+      float enc = gs_clampf((near + vz) * far / ((far - near) * vz), 0.0f, 1.0f);
+      if (finite(enc)) dev = enc;
+    }
+    depth_out[p] = dev;
+  }
+}
+
+// --------------------------------------------------------------- driver ----
+
+namespace {
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+hipError_t plain_scan(uint32_t* data, LenSpec len, uint32_t max_len, uint32_t* partials,
+                      DeviceState* state, hipStream_t s) {
+  uint32_t blocks = div_up(max_len, kPartition);
+  if (blocks == 0) blocks = 1;
+  k_scan_reduce<LoadPlain><<<blocks, kBlock, 0, s>>>(LoadPlain{data}, len, partials);
+  k_scan_partials<<<1, 1024, 0, s>>>(len, partials, kScanPlain, 0u, 0u, 0u, state);
+  k_scan_downsweep<LoadPlain, StorePlain>
+      <<<blocks, kBlock, 0, s>>>(LoadPlain{data}, StorePlain{data}, len, partials);
+  return hipGetLastError();
+}
+
+// Sort pairs in a <-> b for `passes` 8-bit digits; the result lands in a
+// for an even pass count, in b otherwise.
+void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
+                uint32_t passes, uint32_t* hist, uint32_t* partials, DeviceState* state,
+                hipStream_t s) {
+  uint32_t blocks = div_up(max_count, kPartition);
+  if (blocks == 0) blocks = 1;
+  LenSpec hist_len = {count_ptr, 12u /* log2 kPartition */, (uint32_t)kRadix, 0u};
+  for (uint32_t p = 0; p < passes; p++) {
+    const uint2* in = (p % 2 == 0) ? a : b;
+    uint2* out = (p % 2 == 0) ? b : a;
+    uint32_t shift = p * kRadixBits;
+    k_radix_hist<<<blocks, kBlock, 0, s>>>(in, count_ptr, shift, hist);
+    (void)plain_scan(hist, hist_len, blocks * kRadix, partials, state, s);
+    k_radix_scatter<<<blocks, kBlock, 0, s>>>(in, out, count_ptr, shift, hist);
+  }
+}
+
+}  // namespace
+
+static_assert(kPartition == 4096, "hist LenSpec uses log2(kPartition) = 12");
+
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t s) {
+  const FrameParams& fp = b.params;
+  DeviceState* st = b.state;
+  uint32_t n_blocks = div_up(b.max_n, kBlock);
+  if (n_blocks == 0) n_blocks = 1;
+  k_project<<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj);
+
+  // Visible flags -> exclusive offsets -> compacted (depth key, ordinal).
+  LenSpec active_len = {nullptr, 0u, 1u, fp.active_count};
+  uint32_t part_blocks = div_up(b.max_n, kPartition);
+  if (part_blocks == 0) part_blocks = 1;
+  LoadVisible lv{b.proj.mean};
+  k_scan_reduce<LoadVisible><<<part_blocks, kBlock, 0, s>>>(lv, active_len, b.partials);
+  k_scan_partials<<<1, 1024, 0, s>>>(active_len, b.partials, kScanVisible, 0u, 0u, 0u, st);
+  StoreCompact sc{b.proj.mean, fp.depth_mode, fp.viewport[2], fp.viewport[3],
+                  b.visible_offsets, b.depth_a};
+  k_scan_downsweep<LoadVisible, StoreCompact>
+      <<<part_blocks, kBlock, 0, s>>>(lv, sc, active_len, b.partials);
+
+  // Depth sort over V.
+  radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.hist,
+             b.partials, st, s);
+  const uint2* sorted_depth = sorted_depth_records(b);
+
+  // Depth-ordered tile counts -> write offsets; K, overflow, output state.
+  LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
+  LoadOrdered lo{sorted_depth, b.proj.tile_counts};
+  k_scan_reduce<LoadOrdered><<<part_blocks, kBlock, 0, s>>>(lo, vis_len, b.partials);
+  k_scan_partials<<<1, 1024, 0, s>>>(vis_len, b.partials, kScanIntersections, fp.capacity,
+                                      fp.attempt, fp.reset_valid, st);
+  k_scan_downsweep<LoadOrdered, StoreOrdered><<<part_blocks, kBlock, 0, s>>>(
+      lo, StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, b.partials);
+
+  k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.mean, b.proj.conic,
+                                     b.ordered_counts, b.intersection_offsets, b.records_a);
+
+  radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.hist,
+             b.partials, st, s);
+  const uint2* sorted = sorted_tile_records(b);
+
+  k_tile_offsets<<<div_up((uint64_t)b.max_k + 1, kBlock), kBlock, 0, s>>>(fp, st, sorted,
+                                                                          b.tile_offsets);
+  k_raster<<<b.max_tiles, kBlock, 0, s>>>(fp, st, sorted, b.tile_offsets, b.proj.mean,
+                                          b.proj.conic, b.proj.color, b.color_out,
+                                          b.depth_out);
+  return hipGetLastError();
+}
+
+const uint2* sorted_depth_records(const FrameBuffers& b) {
+  return (b.depth_passes % 2 == 0) ? b.depth_a : b.depth_b;
+}
+const uint2* sorted_tile_records(const FrameBuffers& b) {
+  return (b.tile_passes % 2 == 0) ? b.records_a : b.records_b;
+}
+
+}  // namespace gs

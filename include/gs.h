@@ -1,0 +1,201 @@
+/* gs.h - C ABI of the MI355X tiled Gaussian-splatting frame pipeline.
+ *
+ * This is the drop-in boundary for the reference's per-frame render path
+ * (vinneyto/3dgs_tile_webgpu). Each entry point cites the reference
+ * interface it replaces. All pointers are plain host (or, where stated,
+ * device) pointers; no PyTorch or HIP types cross the boundary.
+ *
+ * Conventions (reference GaussianPass.ts:161-260, 1010-1017;
+ * docs/dynamic-intersections.md:25-54):
+ *  - every call returns an int status: GS_OK or a negative GS_E_* code;
+ *  - configuration errors return GS_E_RANGE / GS_E_TYPE and leave the
+ *    context unchanged (the reference throws RangeError / TypeError);
+ *  - intersection overflow is NOT an error: the frame publishes nothing,
+ *    keeps the previous image and reports the demand via gs_read_state;
+ *  - gs_last_error returns a per-context message for the last failure.
+ *
+ * Threading: a context is single-threaded and owns one HIP stream. Render
+ * calls are asynchronous on that stream; gs_read_state, gs_read_image and
+ * gs_read_debug synchronise. Independent contexts may run concurrently.
+ */
+#ifndef GS_H
+#define GS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_OK 0
+#define GS_E_RANGE (-1)  /* RangeError: a value outside its domain */
+#define GS_E_TYPE (-2)   /* TypeError: an unsupported format or variant */
+#define GS_E_OOM (-3)    /* allocation-failed */
+#define GS_E_DEVICE (-4) /* HIP runtime failure */
+#define GS_E_STATE (-5)  /* call order violated (e.g. render before data) */
+
+/* pipeline/types.ts:12-17 */
+#define GS_DEPTH_FLOAT32 0u
+#define GS_DEPTH_PACKED16 1u
+#define GS_AA_COMPENSATED 0u
+#define GS_AA_CLASSIC 1u
+#define GS_BOUNDARY_CLIP 0u
+#define GS_BOUNDARY_REPEAT_X 1u
+#define GS_BOUNDARY_REPEAT_Y 2u
+
+/* GaussianData.ts:10-33 formats */
+#define GS_GEOMETRY_COMPACT 0u
+#define GS_SH_RGB8E8 0u
+
+#define GS_MEMORY_HOST 0u
+#define GS_MEMORY_DEVICE 1u
+
+typedef struct gs_ctx gs_ctx;
+
+/* Pass options (pipeline/types.ts:27-82, GaussianPassOptions) that shape
+ * the GPU pipeline. Defaults from gs_config_default match GaussianPass.ts
+ * :157-260 and IntersectionCapacityPolicy.ts:71-81. */
+typedef struct gs_config {
+  int32_t device;                    /* HIP device ordinal */
+  uint32_t depth_sort_mode;          /* GS_DEPTH_*, default FLOAT32 */
+  uint32_t antialias_mode;           /* GS_AA_*, default COMPENSATED */
+  uint32_t subpixel_sample_culling;  /* default 1 */
+  uint32_t raster_chunk_size;        /* 0 = null (no chunks), default 8192 */
+  uint32_t intersection_capacity;    /* initial record capacity, default 262144 */
+  uint32_t max_intersection_capacity;/* hard limit, default 1 << 28 */
+  uint32_t output_depth;             /* also produce the r32f scalar target */
+  float transmittance_threshold;     /* (0, 1), default 1e-4 */
+  float depth_alpha_threshold;       /* default 0.95 */
+  float invalid_depth_value;         /* default 1 */
+  float background[4];               /* default (0, 0, 0, 0) */
+  uint32_t raster_stats;             /* count checked/blended pairs */
+} gs_config;
+
+/* Packed Gaussian buffers (GaussianData.ts:10-33 / GaussianBuffers), copied
+ * during the call (host memory) or device-to-device (device memory). No
+ * pointer is retained after return. */
+typedef struct gs_data_desc {
+  uint32_t count;           /* slots in every buffer */
+  uint32_t active_count;    /* active ordinals (<= count) */
+  uint32_t geometry_format; /* GS_GEOMETRY_COMPACT */
+  uint32_t sh_format;       /* GS_SH_RGB8E8 */
+  uint32_t sh_degree;       /* 0..3 */
+  uint32_t memory;          /* GS_MEMORY_HOST or GS_MEMORY_DEVICE */
+  const float* means;              /* count * 4 f32: xyz, objectId (<0 hole) */
+  const uint32_t* scales_opacity;  /* count * 2 u32 (compact) */
+  const uint32_t* rotations;       /* count u32 (oct101012) */
+  const uint32_t* sh;              /* count * (deg+1)^2 u32 (rgb8e8) */
+  const uint32_t* active_slots;    /* active_count u32 source slots, or NULL
+                                      for the identity list */
+} gs_data_desc;
+
+/* Per-frame camera and domain (FrameUniforms.ts:41-58 / prepareFrame). */
+typedef struct gs_frame {
+  float projection[16];  /* camera.projectionMatrix, column-major */
+  uint32_t width, height;
+  float near_plane, far_plane; /* depthRange (default camera near/far) */
+  float filter_variance;       /* default 0.3 */
+  uint32_t boundary_axis;      /* GS_BOUNDARY_* */
+  uint32_t attempt;            /* render(attempt, resetValid) */
+  uint32_t reset_valid;
+} gs_frame;
+
+/* Output targets. With memory == GS_MEMORY_HOST the call synchronises and
+ * copies; with GS_MEMORY_DEVICE the kernels write the caller's device
+ * buffers directly (asynchronous). NULL targets keep the library's own
+ * images (see gs_read_image). color: W*H*4 binary16 (rgba16float);
+ * depth: W*H f32 (only with output_depth). */
+typedef struct gs_image {
+  void* color;
+  void* depth;
+  uint32_t memory;
+} gs_image;
+
+/* outputState + dispatch state (intersections.ts:10-41, 114-134;
+ * GaussianPassStats, types.ts:94-103). */
+typedef struct gs_state {
+  uint32_t attempt;
+  uint32_t version;            /* last attempt whose image is complete */
+  uint32_t requested;          /* demand K (saturates at 0xffffffff) */
+  uint32_t overflow;           /* 1 if demand > capacity */
+  uint32_t visible;            /* V */
+  uint32_t intersections;      /* K published this frame (0 on overflow) */
+  uint32_t capacity;
+  uint32_t tiles_x, tiles_y;
+  uint64_t raster_checked;     /* with raster_stats */
+  uint64_t raster_blended;
+} gs_state;
+
+void gs_config_default(gs_config* config);
+
+/* GaussianPass constructor + TiledGaussianPipeline constructor
+ * (GaussianPass.ts:144-307, 852-877; TiledGaussianPipeline.ts:65-163). */
+int gs_create(const gs_config* config, gs_ctx** out);
+void gs_destroy(gs_ctx* ctx);
+const char* gs_last_error(const gs_ctx* ctx);
+
+/* TiledGaussianPipeline.updateData (TiledGaussianPipeline.ts:203-218,
+ * ProjectionStage.ts:172-212, 298-329). */
+int gs_set_data(gs_ctx* ctx, const gs_data_desc* data);
+
+/* Active-slot update for [start, start+n) (GaussianData activation,
+ * GaussianData.ts:170-232; ProjectionStage.updateActiveSlots :298-329).
+ * new_active_count may shrink or grow the active list (<= count). */
+int gs_update_active(gs_ctx* ctx, uint32_t start, uint32_t n,
+                     const uint32_t* slots, uint32_t new_active_count);
+
+/* Object frame table: 10 vec4 (40 f32) per object id: matrixWorld,
+ * modelView, camera position in object space, visibility flag
+ * (ObjectFrameState.ts:97-120). */
+int gs_set_objects(gs_ctx* ctx, uint32_t object_count, const float* frames);
+
+/* Fill one object's 40-float frame from f64 matrices the way
+ * ObjectFrameState.writeCloud does (ObjectFrameState.ts:97-120):
+ * camera_world and object_world are column-major 4x4 f64. */
+void gs_object_frame(const double camera_world[16], const double object_world[16],
+                     int visible, float out[40]);
+
+/* prepareFrame + render (TiledGaussianPipeline.ts:286-341). */
+int gs_render(gs_ctx* ctx, const gs_frame* frame, const gs_image* out);
+
+/* Many cameras of uneven size in one captured launch (no reference
+ * counterpart; the reference renders one camera per pass and shares data
+ * through dataMode "external", docs/render-extensions.md:5-47). */
+int gs_render_batch(gs_ctx* ctx, uint32_t n, const gs_frame* frames,
+                    const gs_image* outs);
+
+/* readStats / getOutputStatus (TiledGaussianPipeline.ts:353-366;
+ * IntersectionStage.ts:197-213; GaussianPass.ts:989-1008). */
+int gs_read_state(gs_ctx* ctx, gs_state* out);
+
+/* resizeIntersections (TiledGaussianPipeline.ts:220-231). */
+int gs_resize_intersections(gs_ctx* ctx, uint32_t capacity);
+
+/* Copy the last frame's color (W*H*4 binary16) / depth (W*H f32) to host. */
+int gs_read_image(gs_ctx* ctx, void* color, size_t color_bytes, void* depth,
+                  size_t depth_bytes);
+
+/* getResources as debug copies (TiledGaussianPipeline.ts:391-406): names
+ * projected_mean, projected_conic, projected_color, tile_counts,
+ * visible_offsets, depth_records, ordered_counts, intersection_offsets,
+ * sorted_records, tile_offsets. *size receives the byte size; with
+ * dst == NULL only the size is returned. */
+int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t bytes,
+                  size_t* size);
+
+/* The HIP stream the context renders on (a hipStream_t), and its device
+ * output images, for callers that chain their own kernels. */
+void* gs_stream(gs_ctx* ctx);
+int gs_device_image(gs_ctx* ctx, void** color, void** depth);
+
+/* Capacity growth rule (IntersectionCapacityPolicy.ts:90-101): returns the
+ * next capacity, or 0 when demand exceeds maximum. */
+uint32_t gs_capacity_next(uint32_t current, uint32_t demand, uint32_t maximum,
+                          double growth, double headroom);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GS_H */

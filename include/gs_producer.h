@@ -1,0 +1,79 @@
+/* gs_producer.h - packed-buffer producer: the encoder that feeds the
+ * GaussianStore contract, and a seeded synthetic scene generator.
+ *
+ * The encoder restates the reference's Rust producer so synthetic and real
+ * scenes reach the frame pipeline in exactly the layout the store publishes:
+ *   gs_encode_compact   <- rust/gaussian-backend/src/engine.rs:1801-1832
+ *                          (+ model.rs:225-228 render_scale_opacity,
+ *                           spark-lib splat_encode.rs:127-136, 234-252)
+ *   gs_encode_sh_rgb8e8 <- engine.rs:1857-1872, 1940-1949 (pack_sh)
+ * The generator has no reference counterpart; it produces the synthetic
+ * workloads (clustered means, log-normal scales, uniform rotations,
+ * sigmoid-normal opacity, rgb8e8 SH) that tests and benchmarks render.
+ */
+#ifndef GS_PRODUCER_H
+#define GS_PRODUCER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compact shape + rotation for n Gaussians. scales: n*3 linear scales;
+ * opacity: n; quats: n*4 (x, y, z, w), normalised by the encoder.
+ * out_shape: n*2 u32; out_rotation: n u32. */
+void gs_encode_compact(size_t n, const float* scales, const float* opacity,
+                       const float* quats, uint32_t* out_shape, uint32_t* out_rotation);
+
+/* Quaternion -> oct101012 (splat_encode.rs:234-252). */
+uint32_t gs_encode_quat_oct101012(const float quat_xyzw[4]);
+
+/* One signed RGB SH coefficient -> rgb8e8 (engine.rs:1940-1949). */
+uint32_t gs_pack_sh_rgb8e8(const float rgb[3]);
+/* n coefficients (n*3 floats) -> n packed words. */
+void gs_encode_sh_rgb8e8(size_t n, const float* rgb, uint32_t* out);
+
+/* IEEE binary16 round-to-nearest-even (half::f16::from_f32). */
+uint16_t gs_f32_to_f16(float v);
+
+typedef struct gs_scene_spec {
+  uint64_t seed;
+  uint32_t count;
+  uint32_t clusters;         /* cluster centres, uniform in [-extent, extent]^3 */
+  float extent;
+  float cluster_sigma;       /* isotropic spread of each cluster */
+  float log_scale_mean, log_scale_sigma, log_scale_min, log_scale_max;
+  uint32_t anisotropic;      /* one random axis major, the other two minor */
+  float major_mean, major_sigma, minor_mean, minor_sigma;
+  float opacity_sigma;       /* opacity = sigmoid(N(0, sigma)) */
+  uint32_t sh_degree;
+  float dc_mean, dc_sigma, band_sigma;
+  /* Near-plane clusters: a fraction of points around centres placed at view
+   * depth [near_depth_min, near_depth_max] inside the camera frustum. */
+  float near_fraction;
+  uint32_t near_clusters;
+  float near_depth_min, near_depth_max;
+  float near_sigma;
+  double camera_position[3];
+  double camera_target[3];
+  double camera_fov_deg;     /* vertical */
+  double camera_aspect;
+} gs_scene_spec;
+
+void gs_scene_spec_default(gs_scene_spec* spec);
+
+/* Generate count Gaussians (object id 0). means: count*4 f32; shape:
+ * count*2; rotation: count; sh: count*(deg+1)^2. The optional float
+ * outputs (may be NULL) receive the pre-encoding values: scales count*3,
+ * opacity count, quats count*4. Returns 0, or -1 on invalid spec. */
+int gs_generate_scene(const gs_scene_spec* spec, float* means, uint32_t* shape,
+                      uint32_t* rotation, uint32_t* sh, float* scales_out,
+                      float* opacity_out, float* quats_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
