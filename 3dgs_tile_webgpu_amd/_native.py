@@ -6,9 +6,11 @@ fails loudly when it is missing: there is no fallback implementation.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libgs.so"
+# GS_LIBRARY overrides the in-tree build (e.g. an instrumented copy).
+LIB_PATH = Path(os.environ.get("GS_LIBRARY") or Path(__file__).resolve().parent / "_lib" / "libgs.so")
 
 GS_OK = 0
 GS_E_RANGE = -1
@@ -45,6 +47,7 @@ class GsConfig(C.Structure):
         ("invalid_depth_value", f32),
         ("background", f32 * 4),
         ("raster_stats", u32),
+        ("capture_graph", u32),
     ]
 
 
@@ -150,6 +153,8 @@ SIGNATURES = {
     "gs_stream": (P, [P]),
     "gs_device_image": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
     "gs_capacity_next": (u32, [u32, u32, u32, C.c_double, C.c_double]),
+    "gs_profile_begin": (C.c_int, [P, u32]),
+    "gs_profile_end": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u32)]),
     "gs_encode_compact": (None, [C.c_size_t, P, P, P, P, P]),
     "gs_encode_quat_oct101012": (u32, [P]),
     "gs_pack_sh_rgb8e8": (u32, [P]),

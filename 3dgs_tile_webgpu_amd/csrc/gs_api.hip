@@ -29,6 +29,8 @@ struct DeviceBuffer {
 struct gs_ctx {
   gs_config cfg;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;  // second raster branch, forked per frame
+  hipEvent_t fork = nullptr, join = nullptr;
   std::string error;
   // Scene (GaussianData contract).
   uint32_t count = 0, active_count = 0, sh_degree = 0;
@@ -46,11 +48,29 @@ struct gs_ctx {
   DeviceBuffer hist, partials;
   // Tile ranges, sized for tile_capacity + 1.
   uint32_t tile_capacity = 0;
-  DeviceBuffer tile_offsets;
+  DeviceBuffer tile_offsets, chunk_first;
+  // Raster chunk tasks and partials (RasterChunkWorkspace.ts:12-59).
+  DeviceBuffer tasks, partials_px, partial_depth;
   // Output images.
   uint32_t image_w = 0, image_h = 0;
   DeviceBuffer color, depth;
   DeviceBuffer state;
+  // Object table: host copy, uploaded through a ring of pinned slots.
+  std::vector<float> objects_host;
+  bool objects_dirty = false;
+  static constexpr int kRing = 8;
+  float* ring[kRing] = {};
+  hipEvent_t ring_done[kRing] = {};
+  size_t ring_bytes = 0;
+  int ring_next = 0;
+  // Captured frame graph and the launch description it was captured from.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  FrameBuffers graph_key = {};
+  bool graph_valid = false;
+  // Stage profiling.
+  std::vector<hipEvent_t> marks;
+  uint32_t profile_capacity = 0, profile_frames = 0;
   // Last frame.
   FrameBuffers last = {};
   bool rendered = false;
@@ -174,6 +194,7 @@ extern "C" void gs_config_default(gs_config* c) {
   c->depth_alpha_threshold = 0.95f;
   c->invalid_depth_value = 1.0f;
   c->raster_stats = 0;
+  c->capture_graph = 1;
 }
 
 // GaussianPass.ts:157-260 option validation.
@@ -239,8 +260,11 @@ extern "C" int gs_create(const gs_config* config, gs_ctx** out) {
   gs_ctx* ctx = new gs_ctx();
   ctx->cfg = c;
   // Default stream priority only.
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete ctx;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess) {
+    gs_destroy(ctx);
     return GS_E_DEVICE;
   }
   ctx->capacity = c.intersection_capacity;
@@ -270,8 +294,19 @@ extern "C" void gs_destroy(gs_ctx* ctx) {
                          &ctx->depth_b,      &ctx->ordered_counts,  &ctx->intersection_offsets,
                          &ctx->records_a,    &ctx->records_b,       &ctx->hist,
                          &ctx->partials,     &ctx->tile_offsets,    &ctx->color,
-                         &ctx->depth,        &ctx->state};
+                         &ctx->depth,        &ctx->state,           &ctx->chunk_first,
+                         &ctx->tasks,        &ctx->partials_px,     &ctx->partial_depth};
   for (DeviceBuffer* b : all) release(*b);
+  for (int i = 0; i < gs_ctx::kRing; i++) {
+    if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
+    if (ctx->ring_done[i]) (void)hipEventDestroy(ctx->ring_done[i]);
+  }
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  for (hipEvent_t e : ctx->marks) (void)hipEventDestroy(e);
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+  if (ctx->join) (void)hipEventDestroy(ctx->join);
+  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -365,11 +400,48 @@ extern "C" int gs_update_active(gs_ctx* ctx, uint32_t start, uint32_t n, const u
 extern "C" int gs_set_objects(gs_ctx* ctx, uint32_t object_count, const float* frames) {
   if (!ctx) return GS_E_RANGE;
   if (object_count > 0 && !frames) return fail(ctx, GS_E_RANGE, "missing object frames");
-  (void)hipSetDevice(ctx->cfg.device);
-  int rc = copy_in(ctx, ctx->objects, frames, (size_t)object_count * 40 * 4, GS_MEMORY_HOST,
-                   "object frames");
-  if (rc) return rc;
+  ctx->objects_host.assign(frames, frames + (size_t)object_count * 40);
   ctx->object_count = object_count;
+  ctx->objects_dirty = true;
+  return GS_OK;
+}
+
+// Enqueue the object table upload (ObjectFrameState.update) without
+// blocking: the host copy goes to the next pinned ring slot once the copy
+// that last used that slot has completed.
+static int upload_objects(gs_ctx* ctx) {
+  if (!ctx->objects_dirty) return GS_OK;
+  size_t bytes = ctx->objects_host.size() * sizeof(float);
+  int rc = ensure(ctx, ctx->objects, bytes, "object frames");
+  if (rc) return rc;
+  if (bytes == 0) {
+    ctx->objects_dirty = false;
+    return GS_OK;
+  }
+  if (bytes > ctx->ring_bytes) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    for (int i = 0; i < gs_ctx::kRing; i++) {
+      if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
+      ctx->ring[i] = nullptr;
+      if (hipHostMalloc((void**)&ctx->ring[i], bytes, hipHostMallocDefault) != hipSuccess)
+        return fail(ctx, GS_E_OOM, "allocation-failed: object staging");
+      if (!ctx->ring_done[i] &&
+          hipEventCreateWithFlags(&ctx->ring_done[i], hipEventDisableTiming) != hipSuccess)
+        return fail(ctx, GS_E_DEVICE, "hipEventCreate");
+    }
+    ctx->ring_bytes = bytes;
+  }
+  int slot = ctx->ring_next;
+  ctx->ring_next = (slot + 1) % gs_ctx::kRing;
+  hipError_t e = hipEventSynchronize(ctx->ring_done[slot]);
+  if (e != hipSuccess) return hip_fail(ctx, e, "object staging wait");
+  if (bytes) memcpy(ctx->ring[slot], ctx->objects_host.data(), bytes);
+  if (bytes) e = hipMemcpyAsync(ctx->objects.ptr, ctx->ring[slot], bytes, hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ring_done[slot], ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "object upload");
+  ctx->objects_dirty = false;
   return GS_OK;
 }
 
@@ -501,14 +573,30 @@ static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* 
   uint32_t tiles = tiles_x * tiles_y;
   if (tiles > ctx->tile_capacity || !ctx->tile_offsets.ptr) {
     uint32_t cap = next_pow2(tiles);
-    if ((rc = ensure(ctx, ctx->tile_offsets, ((size_t)cap + 1) * 4, "tile-offsets"))) return rc;
+    if ((rc = ensure(ctx, ctx->tile_offsets, ((size_t)cap + 1) * 4, "tile-offsets")) ||
+        (rc = ensure(ctx, ctx->chunk_first, ((size_t)cap + 1) * 4, "raster-chunk-offsets")))
+      return rc;
     ctx->tile_capacity = cap;
   }
+  // rasterChunks.ts:73-78 (maxRasterChunkTasks): every chunked tile holds
+  // more than chunk_size records, so ceil(2K / chunk_size) tasks suffice.
+  uint32_t task_capacity = 1;
+  if (ctx->cfg.raster_chunk_size) {
+    task_capacity = div_up(2ull * ctx->capacity, ctx->cfg.raster_chunk_size);
+    if (task_capacity < 1) task_capacity = 1;
+  }
+  // Heavy raster units: the chunk tasks plus tiles above 1024 records.
+  uint32_t unit_capacity = task_capacity + div_up(ctx->capacity, 1024) + 1;
+  if ((rc = ensure(ctx, ctx->tasks, (size_t)unit_capacity * 8, "raster-heavy-units")) ||
+      (rc = ensure(ctx, ctx->partials_px, (size_t)task_capacity * gs::kBlock * 16,
+                   "raster-chunk-partials")) ||
+      (rc = ensure(ctx, ctx->partial_depth, (size_t)task_capacity * gs::kBlock * 4,
+                   "raster-chunk-depth")))
+    return rc;
   size_t pixels = (size_t)f->width * f->height;
   if ((rc = ensure(ctx, ctx->color, pixels * 8, "color-target"))) return rc;
   if (ctx->cfg.output_depth && (rc = ensure(ctx, ctx->depth, pixels * 4, "depth-target")))
     return rc;
-  if ((rc = ensure(ctx, ctx->objects, 160, "object frames"))) return rc;
   FrameParams& p = b.params;
   memset(&p, 0, sizeof(p));
   memcpy(p.projection, f->projection, sizeof(p.projection));
@@ -559,6 +647,12 @@ static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* 
   b.records_a = (uint2*)ctx->records_a.ptr;
   b.records_b = (uint2*)ctx->records_b.ptr;
   b.tile_offsets = (uint32_t*)ctx->tile_offsets.ptr;
+  b.chunk_first = (uint32_t*)ctx->chunk_first.ptr;
+  b.tasks = (uint2*)ctx->tasks.ptr;
+  b.chunk_partials = (float4*)ctx->partials_px.ptr;
+  b.partial_depth = (float*)ctx->partial_depth.ptr;
+  b.task_capacity = task_capacity;
+  b.unit_capacity = unit_capacity;
   b.hist = (uint32_t*)ctx->hist.ptr;
   b.partials = (uint32_t*)ctx->partials.ptr;
   b.color_out = (uint16_t*)(color_dst ? color_dst : ctx->color.ptr);
@@ -572,6 +666,64 @@ static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* 
   return GS_OK;
 }
 
+static int launch(gs_ctx* ctx, const FrameBuffers& b) {
+  hipStream_t s = ctx->stream;
+  if (ctx->profile_capacity) {
+    if (ctx->profile_frames >= ctx->profile_capacity)
+      return fail(ctx, GS_E_RANGE, "profiling capacity exhausted");
+    hipEvent_t* marks = &ctx->marks[(size_t)ctx->profile_frames * (gs::kStageCount + 1)];
+    hipError_t e = gs::launch_frame(b, s, ctx->aux, ctx->fork, ctx->join, marks);
+    if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
+    ctx->profile_frames++;
+    return GS_OK;
+  }
+  if (!ctx->cfg.capture_graph) {
+    hipError_t e = gs::launch_frame(b, s, ctx->aux, ctx->fork, ctx->join, nullptr);
+    if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
+    return GS_OK;
+  }
+  if (!ctx->graph_valid || memcmp(&ctx->graph_key, &b, sizeof(b)) != 0) {
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
+    hipError_t le = gs::launch_frame(b, s, ctx->aux, ctx->fork, ctx->join, nullptr);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (le != hipSuccess) e = le;
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      ctx->graph_valid = false;
+      return hip_fail(ctx, e, "frame capture");
+    }
+    bool updated = false;
+    if (ctx->graph_exec) {
+      hipGraphExecUpdateResult result;
+      hipGraphNode_t error_node = nullptr;
+      updated = hipGraphExecUpdate(ctx->graph_exec, g, &error_node, &result) == hipSuccess &&
+                result == hipGraphExecUpdateSuccess;
+      if (!updated) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+      }
+    }
+    if (!updated) {
+      e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        ctx->graph_valid = false;
+        return hip_fail(ctx, e, "hipGraphInstantiate");
+      }
+    }
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph = g;
+    ctx->graph_key = b;
+    ctx->graph_valid = true;
+  }
+  hipError_t e = hipGraphLaunch(ctx->graph_exec, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+  return GS_OK;
+}
+
 extern "C" int gs_render(gs_ctx* ctx, const gs_frame* f, const gs_image* out) {
   if (!ctx || !f) return GS_E_RANGE;
   (void)hipSetDevice(ctx->cfg.device);
@@ -581,11 +733,14 @@ extern "C" int gs_render(gs_ctx* ctx, const gs_frame* f, const gs_image* out) {
     color_dst = out->color;
     depth_dst = out->depth;
   }
-  FrameBuffers b;
-  int rc = prepare_frame(ctx, f, b, color_dst, depth_dst);
+  // The upload may grow the object buffer, so it precedes the launch
+  // description that captures its address.
+  int rc = upload_objects(ctx);
   if (rc) return rc;
-  hipError_t e = gs::launch_frame(b, ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
+  FrameBuffers b;
+  memset(&b, 0, sizeof(b));
+  if ((rc = prepare_frame(ctx, f, b, color_dst, depth_dst))) return rc;
+  if ((rc = launch(ctx, b))) return rc;
   ctx->last = b;
   ctx->rendered = true;
   ctx->width = f->width;
@@ -595,6 +750,49 @@ extern "C" int gs_render(gs_ctx* ctx, const gs_frame* f, const gs_image* out) {
   if (out && out->memory == GS_MEMORY_HOST)
     return gs_read_image(ctx, out->color, out->color ? (size_t)f->width * f->height * 8 : 0,
                          out->depth, out->depth ? (size_t)f->width * f->height * 4 : 0);
+  return GS_OK;
+}
+
+extern "C" int gs_profile_begin(gs_ctx* ctx, uint32_t max_frames) {
+  if (!ctx || max_frames == 0) return GS_E_RANGE;
+  (void)hipSetDevice(ctx->cfg.device);
+  size_t need = (size_t)max_frames * (gs::kStageCount + 1);
+  while (ctx->marks.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return fail(ctx, GS_E_DEVICE, "hipEventCreate");
+    ctx->marks.push_back(e);
+  }
+  ctx->profile_capacity = max_frames;
+  ctx->profile_frames = 0;
+  return GS_OK;
+}
+
+extern "C" int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], double* frame_ms,
+                              uint32_t* frames) {
+  if (!ctx) return GS_E_RANGE;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+  double sums[gs::kStageCount] = {0};
+  double total = 0.0;
+  for (uint32_t f = 0; f < ctx->profile_frames; f++) {
+    hipEvent_t* m = &ctx->marks[(size_t)f * (gs::kStageCount + 1)];
+    for (int st = 0; st < gs::kStageCount; st++) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, m[st], m[st + 1]) != hipSuccess)
+        return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
+      sums[st] += ms;
+    }
+    float whole = 0.0f;
+    if (hipEventElapsedTime(&whole, m[0], m[gs::kStageCount]) != hipSuccess)
+      return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
+    total += whole;
+  }
+  if (stage_ms)
+    for (int st = 0; st < gs::kStageCount; st++) stage_ms[st] = sums[st];
+  if (frame_ms) *frame_ms = total;
+  if (frames) *frames = ctx->profile_frames;
+  ctx->profile_capacity = 0;
+  ctx->profile_frames = 0;
   return GS_OK;
 }
 

@@ -48,6 +48,8 @@ struct DeviceState {
   uint32_t count;     // K published (0 on overflow)
   uint32_t overflow;
   uint32_t out_attempt, out_version, out_demand, out_overflow;
+  uint32_t chunk_tasks;  // raster chunk tasks of this frame
+  uint32_t heavy_units;  // raster heavy work-list length of this frame
   unsigned long long checked, blended;
 };
 
@@ -97,6 +99,12 @@ struct FrameBuffers {
   uint32_t* tile_offsets;
   uint32_t* hist;       // kRadix * ceil(max(N, capacity) / kPartition)
   uint32_t* partials;   // scan partials
+  uint32_t* chunk_first;  // per tile: first chunk task (tile capacity)
+  uint2* tasks;           // raster heavy units: (tile, chunk or ~0u)
+  float4* chunk_partials; // per task and pixel: (C.rgb, T)
+  float* partial_depth;   // per task and pixel: weighted depth
+  uint32_t task_capacity;  // chunk tasks (partials slots)
+  uint32_t unit_capacity;  // heavy units (chunk tasks + heavy tiles)
   uint16_t* color_out;  // W*H*4 binary16
   float* depth_out;     // W*H f32 or null
   uint32_t max_n;       // slot capacity of the per-Gaussian arrays
@@ -106,8 +114,26 @@ struct FrameBuffers {
   uint32_t tile_passes;
 };
 
+// Frame stages, in launch order; launch_frame can record an event before
+// each stage and one after the last (kStageCount + 1 events).
+enum Stage : int {
+  kStageProject = 0,
+  kStageVisible,
+  kStageDepthSort,
+  kStageScan,
+  kStageEmit,
+  kStageTileSort,
+  kStageTileOffsets,
+  kStageRaster,
+  kStageCount
+};
+
 // Enqueue one frame on `stream` (all kernels; sizes read on the device).
-hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream);
+// `marks` is null or kStageCount + 1 events recorded at stage boundaries.
+// The raster's light and heavy kernels run concurrently: `aux` is forked
+// from and joined back into `stream` through the two events.
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, hipStream_t aux,
+                        hipEvent_t fork, hipEvent_t join, hipEvent_t* marks);
 // Location of the sorted depth / tile records after launch_frame.
 const uint2* sorted_depth_records(const FrameBuffers& b);
 const uint2* sorted_tile_records(const FrameBuffers& b);

@@ -493,176 +493,475 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
 // tileOffsets.ts:4-128 in one pass: record i owns the tiles in
 // (tile[i-1], tile[i]], so every tile gets the index of the first record of
 // the first non-empty tile at or after it -- the suffix minimum -- and
-// off[T] ends the last range (K, or the first sentinel).
+// off[T] ends the last range (K, or the first sentinel). Grid-stride over
+// the device-side count.
 __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
                                                          const DeviceState* __restrict__ state,
                                                          const uint2* __restrict__ sorted,
                                                          uint32_t* __restrict__ off) {
   uint32_t count = state->count;
   uint32_t tiles = fp.tile_count;
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i > count) return;
-  uint32_t lo = i == 0 ? 0u : sorted[i - 1].x + 1u;
-  uint32_t hi = i == count ? tiles : sorted[i].x;
-  if (hi > tiles) hi = tiles;
-  for (uint32_t t = lo; t <= hi; t++) off[t] = i;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i <= count; i += gridDim.x * kBlock) {
+    uint32_t lo = i == 0 ? 0u : sorted[i - 1].x + 1u;
+    uint32_t hi = i == count ? tiles : sorted[i].x;
+    if (hi > tiles) hi = tiles;
+    for (uint32_t t = lo; t <= hi; t++) off[t] = i;
+  }
 }
 
 // ---------------------------------------------------------------- raster ----
 //
-// One 256-thread block per 16x16 tile; each wave owns an 8x8 quadrant.
-// Gaussians are staged 256 at a time in LDS (structure of arrays); each
-// pixel blends front to back and stops when T < threshold; the block leaves
-// a batch loop once every pixel has stopped. Tiles with more records than
-// chunk_size are blended as consecutive independent chunks (each from
-// T = 1) and composited in order, exactly as the reference's chunk tasks
-// and composite pass (TileRasterizer.ts:331-412, 807-902).
+// One 256-thread block per tile (or per chunk task of a heavy tile), one
+// pixel per thread. Records are taken 256 at a time, front to back:
+//  phase A: the thread owning record r stages it in LDS and tests only the
+//    pixels of its support rectangle (support_rect below): every pixel
+//    outside the rectangle is proven to fail the reference's power test, so
+//    leaving it out changes nothing. Pixels inside run the exact test and,
+//    when power lies in [-ln(255 a), 0], set bit r of their candidate mask;
+//  phase B: each pixel walks its candidate bits in record order and
+//    evaluates the reference's blend (alpha = clamp(a e^power, 0, 0.99),
+//    skip alpha < 1/255, C += c T alpha, T *= 1 - alpha, stop at T < 1e-4).
+// The blend visits exactly the records the reference blends, in the same
+// order with the same binary32 operations. The block leaves once every
+// pixel has stopped. Global loads of the next batch are issued before the
+// current batch is processed.
+//
+// Tiles with more than chunk_size records are split, as in the reference,
+// into consecutive chunks blended from T = 1 by independent tasks and
+// composited in order (TileRasterizer.ts:331-412, 807-902).
 
-__global__ __launch_bounds__(kBlock) void k_raster(const FrameParams fp,
-                                                   DeviceState* __restrict__ state,
-                                                   const uint2* __restrict__ sorted,
-                                                   const uint32_t* __restrict__ off,
-                                                   const float4* __restrict__ pmean,
-                                                   const float4* __restrict__ pconic,
-                                                   const float4* __restrict__ pcolor,
-                                                   uint16_t* __restrict__ color_out,
-                                                   float* __restrict__ depth_out) {
-  __shared__ float s_cx[kBlock], s_cy[kBlock], s_depth[kBlock], s_op[kBlock];
-  __shared__ float s_c0[kBlock], s_c1[kBlock], s_c2[kBlock], s_thr[kBlock];
-  __shared__ float s_r[kBlock], s_g[kBlock], s_b[kBlock];
-  if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
-  uint32_t tile = blockIdx.x;
-  if (tile >= fp.tile_count) return;
+// Tile-local rectangle [x0, x1] x [y0, y1] (global pixel indices) outside of
+// which the binary32 quadratic form q = c0 dx^2 + 2 c1 dx dy + c2 dy^2
+// provably exceeds 2 thr, i.e. the reference's test `power < -thr` holds.
+// With S = |c0| dx^2 + 2|c1 dx dy| + |c2| dy^2, the evaluated q obeys
+// q_hat >= q - g S (g = 3e-7 > 4 unit roundoffs), q >= (det / c2) dx^2 and
+// S <= (c0 + |c1|) dx^2 + (c2 + |c1|) dy^2; with |dy| <= Y over the tile,
+// q_hat > 2 thr whenever dx^2 (det / c2 - g (c0 + |c1|)) > 2 thr + g (c2 +
+// |c1|) Y^2. The same bound with x and y exchanged limits rows. Evaluated in
+// binary64 from the binary32 operands; pixels left out lie at least one
+// pixel beyond the bound.
+GS_DEV void support_rect(float cx, float cy, float c0, float c1, float c2, float thr, int tx0,
+                         int ty0, int& x0, int& x1, int& y0, int& y1) {
+  x0 = tx0;
+  x1 = tx0 + kTile - 1;
+  y0 = ty0;
+  y1 = ty0 + kTile - 1;
+  double C0 = c0, C1 = c1, C2 = c2, T = thr;
+  double det = C0 * C2 - C1 * C1;
+  if (!(det > 0.0 && C0 > 0.0 && C2 > 0.0 && T >= 0.0 && T < 1e30)) return;
+  const double g = 3.0e-7;
+  double a1 = C0 + fabs(C1), a2 = C2 + fabs(C1);
+  double X = fmax(fabs(tx0 + 0.5 - (double)cx), fabs(tx0 + 15.5 - (double)cx)) + 1.0;
+  double Y = fmax(fabs(ty0 + 0.5 - (double)cy), fabs(ty0 + 15.5 - (double)cy)) + 1.0;
+  double two_t = 2.0 * T * (1.0 + 1e-6);
+  // kx = det / c2 - g a1 > 0  <=>  det - g a1 c2 > 0 (c2 > 0); then
+  // hx^2 = (two_t + g a2 Y^2) c2 / (det - g a1 c2). The quotient and root
+  // are taken in binary32 (a few roundings, < 1e-6 relative) and widened by
+  // a 1e-5 relative margin.
+  double nx = det - g * a1 * C2, ny = det - g * a2 * C0;
+  if (nx > 0.0) {
+    float h2 = (float)((two_t + g * a2 * Y * Y) * C2) / (float)nx;
+    double hx = (double)gs_sqrtf(h2) * (1.0 + 1e-5) + 1e-6;
+    double lo = floor((double)cx - hx - 0.5), hi = ceil((double)cx + hx - 0.5);
+    if (lo > (double)x0) x0 = lo > (double)x1 ? x1 + 1 : (int)lo;
+    if (hi < (double)x1) x1 = hi < (double)x0 ? x0 - 1 : (int)hi;
+  }
+  if (ny > 0.0) {
+    float h2 = (float)((two_t + g * a1 * X * X) * C0) / (float)ny;
+    double hy = (double)gs_sqrtf(h2) * (1.0 + 1e-5) + 1e-6;
+    double lo = floor((double)cy - hy - 0.5), hi = ceil((double)cy + hy - 0.5);
+    if (lo > (double)y0) y0 = lo > (double)y1 ? y1 + 1 : (int)lo;
+    if (hi < (double)y1) y1 = hi < (double)y0 ? y0 - 1 : (int)hi;
+  }
+}
+
+// The reference's per-pair power (TileRasterizer.ts:624-656), exactly.
+GS_DEV float pair_power(float pxc, float pyc, float4 m, float4 c, float vw, float vh,
+                        uint32_t axis) {
+  float dx, dy;
+  if (axis == 0) {
+    dx = pxc - m.x;
+    dy = pyc - m.y;
+  } else {
+    boundary_delta(pxc, pyc, m.x, m.y, c.x, c.y, c.z, vw, vh, axis, dx, dy);
+  }
+  return (c.x * (dx * dx) + c.y * 2.0f * dx * dy + c.z * (dy * dy)) * -0.5f;
+}
+
+struct PixelSum {
+  float r, g, b, d, t;
+  uint32_t checked, blended;
+};
+
+// Light tiles use 256-thread blocks, heavy units 1024-thread blocks; a
+// round stages as many records as the block has threads.
+constexpr int kLightThreads = 256;
+constexpr int kHeavyThreads = 1024;
+constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
+
+template <int kThreads>
+struct RasterLds {
+  static constexpr int kWords = kThreads / 32;
+  float4 mean[kThreads];   // cx, cy, depth, opacity
+  float4 conic[kThreads];  // a, b, c, ln(255 opacity)
+  float4 color[kThreads];
+  uint32_t mask[kWords][kBlock];  // [word][pixel] candidate bits of the round
+};
+
+// Blend one candidate (record j of the round) into the pixel sum.
+struct Candidate {
+  float alpha, depth;
+  float4 color;
+};
+
+template <class Lds>
+GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, float vw,
+                                float vh, uint32_t axis) {
+  float4 gm = L.mean[j];
+  float4 gc = L.conic[j];
+  float power = pair_power(pxc, pyc, gm, gc, vw, vh, axis);
+  Candidate c;
+  c.alpha = gs_clampf(gm.w * gs_expf(power), 0.0f, 0.99f);
+  c.depth = gm.z;
+  c.color = L.color[j];
+  return c;
+}
+
+// Blend records [cb, ce) of `tile` into one pixel per thread (threads
+// 0..255), from T = 1. All kThreads threads of the block must call it.
+template <int kThreads>
+GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
+                                 const uint2* __restrict__ sorted,
+                                 const float4* __restrict__ pmean,
+                                 const float4* __restrict__ pconic,
+                                 const float4* __restrict__ pcolor, RasterLds<kThreads>& L,
+                                 uint32_t& x,
+                                 uint32_t& y, bool& active) {
   uint32_t t = threadIdx.x;
-  uint32_t wave = t >> 6, lane = t & 63u;
-  uint32_t lx = (wave & 1u) * 8u + (lane & 7u);
-  uint32_t ly = (wave >> 1) * 8u + (lane >> 3);
-  uint32_t tx = tile % fp.tiles_x, ty = tile / fp.tiles_x;
-  uint32_t x = tx * kTile + lx, y = ty * kTile + ly;
-  bool active = x < fp.width && y < fp.height;
+  bool pixel_thread = t < (uint32_t)kBlock;
+  int tx0 = (int)((tile % fp.tiles_x) * kTile), ty0 = (int)((tile / fp.tiles_x) * kTile);
+  x = (uint32_t)tx0 + (t & 15u);
+  y = (uint32_t)ty0 + ((t >> 4) & 15u);
+  active = pixel_thread && x < fp.width && y < fp.height;
   float pxc = (float)x + 0.5f, pyc = (float)y + 0.5f;
   float vw = fp.viewport[0], vh = fp.viewport[1];
   uint32_t axis = fp.boundary_axis;
   float threshold = fp.transmittance_threshold;
-  uint32_t begin = off[tile], end = off[tile + 1];
-  uint32_t count = end - begin;
-  uint32_t cs = fp.chunk_size;
-  bool chunked = cs != 0 && count > cs;
-  uint32_t nchunks = chunked ? (count + cs - 1) / cs : 1u;
-  float acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f, acc_d = 0.0f, trans = 1.0f;
-  uint32_t checked = 0, blended = 0;
-  for (uint32_t chunk = 0; chunk < nchunks; chunk++) {
-    uint32_t cb = chunked ? begin + chunk * cs : begin;
-    uint32_t ce = chunked ? min(cb + cs, end) : end;
-    float lr = 0.0f, lg = 0.0f, lb = 0.0f, ld = 0.0f, lt = 1.0f;
-    bool done = !active || (chunked && trans < threshold);
-    for (uint32_t batch = cb; batch < ce; batch += kBlock) {
-      if (__syncthreads_and(done)) break;
-      uint32_t idx = batch + t;
-      if (idx < ce) {
-        uint32_t gid = sorted[idx].y;
-        float4 m = pmean[gid];
-        float4 c = pconic[gid];
-        float4 k = pcolor[gid];
-        s_cx[t] = m.x;
-        s_cy[t] = m.y;
-        s_depth[t] = m.z;
-        s_op[t] = m.w;
-        s_thr[t] = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
-        s_c0[t] = c.x;
-        s_c1[t] = c.y;
-        s_c2[t] = c.z;
-        s_r[t] = k.x;
-        s_g[t] = k.y;
-        s_b[t] = k.z;
+  PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
+  bool done = !active;
+  // Prefetch the first round.
+  uint32_t idx = cb + t;
+  float4 m = make_float4(0.f, 0.f, 0.f, 0.f), c = m, k = m;
+  if (idx < ce) {
+    uint32_t gid = sorted[idx].y;
+    m = pmean[gid];
+    c = pconic[gid];
+    k = pcolor[gid];
+  }
+  __syncthreads();
+  constexpr int kRasterBatch = kThreads;
+  constexpr int kMaskWords = kThreads / 32;
+  for (uint32_t batch = cb; batch < ce; batch += kRasterBatch) {
+    for (uint32_t i = t; i < (uint32_t)(kMaskWords * kBlock); i += kThreads)
+      (&L.mask[0][0])[i] = 0u;
+    __syncthreads();
+    bool mine = batch + t < ce;
+    float thr = 0.0f;
+    if (mine) {
+      thr = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
+      L.mean[t] = m;
+      L.conic[t] = make_float4(c.x, c.y, c.z, thr);
+      L.color[t] = k;
+    }
+    float4 cm = m, cc = make_float4(c.x, c.y, c.z, thr);
+    uint32_t nidx = batch + kRasterBatch + t;
+    if (nidx < ce) {
+      uint32_t gid = sorted[nidx].y;
+      m = pmean[gid];
+      c = pconic[gid];
+      k = pcolor[gid];
+    }
+    // Candidate pixels of record t.
+    if (mine) {
+      int x0, x1, y0, y1;
+      if (axis == 0)
+        support_rect(cm.x, cm.y, cc.x, cc.y, cc.z, cc.w, tx0, ty0, x0, x1, y0, y1);
+      else {
+        x0 = tx0;
+        x1 = tx0 + kTile - 1;
+        y0 = ty0;
+        y1 = ty0 + kTile - 1;
       }
-      __syncthreads();
-      uint32_t n = min((uint32_t)kBlock, ce - batch);
-      if (!done) {
-        for (uint32_t j = 0; j < n; j++) {
-          checked++;
-          float c0 = s_c0[j], c1 = s_c1[j], c2 = s_c2[j];
-          float dx, dy;
-          if (axis == 0) {
-            dx = pxc - s_cx[j];
-            dy = pyc - s_cy[j];
-          } else {
-            boundary_delta(pxc, pyc, s_cx[j], s_cy[j], c0, c1, c2, vw, vh, axis, dx, dy);
-          }
-          float power = (c0 * (dx * dx) + c1 * 2.0f * dx * dy + c2 * (dy * dy)) * -0.5f;
-          if (power > 0.0f || power < -s_thr[j]) continue;
-          float alpha = gs_clampf(s_op[j] * gs_expf(power), 0.0f, 0.99f);
-          if (alpha < (float)(1.0 / 255.0)) continue;
-          float contribution = lt * alpha;
-          lr = lr + s_r[j] * contribution;
-          lg = lg + s_g[j] * contribution;
-          lb = lb + s_b[j] * contribution;
-          ld = ld + s_depth[j] * contribution;
-          blended++;
-          lt = lt * (1.0f - alpha);
-          if (lt < threshold) {
-            done = true;
-            break;
-          }
+      uint32_t bit = 1u << (t & 31u), word = t >> 5;
+      for (int py = y0; py <= y1; py++) {
+        float pyf = (float)py + 0.5f;
+        for (int px = x0; px <= x1; px++) {
+          float power = pair_power((float)px + 0.5f, pyf, cm, cc, vw, vh, axis);
+          if (!(power > 0.0f || power < -cc.w))
+            atomicOr(&L.mask[word][(uint32_t)((py - ty0) * kTile + (px - tx0))], bit);
         }
       }
     }
-    if (!chunked) {
-      acc_r = lr;
-      acc_g = lg;
-      acc_b = lb;
-      acc_d = ld;
-      trans = lt;
-    } else if (active && !(trans < threshold)) {
-      acc_r = acc_r + lr * trans;
-      acc_g = acc_g + lg * trans;
-      acc_b = acc_b + lb * trans;
-      acc_d = acc_d + ld * trans;
-      trans = trans * lt;
-    }
-  }
-  if (fp.raster_stats) {
-    // One atomic per wave keeps the counters cheap.
-    unsigned long long c64 = checked, b64 = blended;
+    __syncthreads();
+    // Ordered blend over this pixel's candidates: each lane walks its own
+    // candidate bits across the round's mask words, four at a time (their
+    // loads and exponentials are independent; only the blend is serial).
+    if (pixel_thread && !done) {
+      uint32_t n = min((uint32_t)kRasterBatch, ce - batch);
+      uint32_t stop = n;
+      uint32_t w = 0;
+      uint32_t bits = L.mask[0][t];
+      while (true) {
+        uint32_t j[4];
+        bool v[4];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-      c64 += __shfl_xor(c64, d, 64);
-      b64 += __shfl_xor(b64, d, 64);
+        for (int q = 0; q < 4; q++) {
+          while (bits == 0u && w + 1u < (uint32_t)kMaskWords) {
+            w++;
+            bits = L.mask[w][t];
+          }
+          v[q] = bits != 0u;
+          j[q] = w * 32u + (v[q] ? (uint32_t)__builtin_ctz(bits) : 0u);
+          bits &= bits - 1u;
+        }
+        if (!v[0]) break;
+        Candidate cd[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (v[q]) cd[q] = eval_candidate(L, j[q], pxc, pyc, vw, vh, axis);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (!v[q] || done) continue;
+          float alpha = cd[q].alpha;
+          if (alpha < (float)(1.0 / 255.0)) continue;
+          float contribution = s.t * alpha;
+          s.r = s.r + cd[q].color.x * contribution;
+          s.g = s.g + cd[q].color.y * contribution;
+          s.b = s.b + cd[q].color.z * contribution;
+          s.d = s.d + cd[q].depth * contribution;
+          s.blended++;
+          s.t = s.t * (1.0f - alpha);
+          if (s.t < threshold) {
+            done = true;
+            stop = j[q] + 1u;
+          }
+        }
+        if (done) break;
+      }
+      s.checked += stop;  // pairs the reference's loop evaluates
     }
-    if (lane == 0) {
-      atomicAdd(&state->checked, c64);
-      atomicAdd(&state->blended, b64);
-    }
+    if (__syncthreads_and(done || !pixel_thread)) break;
   }
-  if (!active) return;
-  // storeFinalPixel (TileRasterizer.ts:930-964)
+  return s;
+}
+
+GS_DEV void add_stats(DeviceState* state, uint32_t checked, uint32_t blended) {
+  unsigned long long c64 = checked, b64 = blended;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    c64 += __shfl_xor(c64, d, 64);
+    b64 += __shfl_xor(b64, d, 64);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(&state->checked, c64);
+    atomicAdd(&state->blended, b64);
+  }
+}
+
+// storeFinalPixel (TileRasterizer.ts:930-964) + finalizeGaussianDepth.ts:18-68
+// with the default depth node (GaussianContextNodes.ts:181-189).
+GS_DEV void store_pixel(const FrameParams& fp, uint32_t x, uint32_t y, float r, float g, float b,
+                        float d, float t, uint16_t* __restrict__ color_out,
+                        float* __restrict__ depth_out) {
   float bga = gs_clampf(fp.background[3], 0.0f, 1.0f);
-  float o0 = acc_r + fp.background[0] * trans * bga;
-  float o1 = acc_g + fp.background[1] * trans * bga;
-  float o2 = acc_b + fp.background[2] * trans * bga;
-  float o3 = 1.0f - trans * (1.0f - bga);
+  float o0 = r + fp.background[0] * t * bga;
+  float o1 = g + fp.background[1] * t * bga;
+  float o2 = b + fp.background[2] * t * bga;
+  float o3 = 1.0f - t * (1.0f - bga);
   size_t p = (size_t)y * fp.width + x;
   uint2 packed = make_uint2(gs_float_to_half(o0) | (gs_float_to_half(o1) << 16),
                             gs_float_to_half(o2) | (gs_float_to_half(o3) << 16));
   reinterpret_cast<uint2*>(color_out)[p] = packed;
   if (depth_out) {
-    // finalizeGaussianDepth.ts:18-68 with the default depth node
-    // (GaussianContextNodes.ts:181-189).
-    float galpha = 1.0f - trans;
+    float galpha = 1.0f - t;
     float od = 0.0f;
-    if (galpha > 0.0f) od = acc_d / galpha;
-    auto finite = [](float v) { return v == v && gs_fabsf(v) <= kMaxF32; };
-    bool scalar_valid = galpha > 0.0f && finite(acc_d) && finite(od) && od > 0.0f;
+    if (galpha > 0.0f) od = d / galpha;
+    bool scalar_valid = galpha > 0.0f && d == d && gs_fabsf(d) <= kMaxF32 && od == od &&
+                        gs_fabsf(od) <= kMaxF32 && od > 0.0f;
     bool valid = scalar_valid && galpha > fp.depth_alpha_threshold;
     float dev = fp.invalid_depth_value;
     if (valid) {
       float near = fp.viewport[2], far = fp.viewport[3];
       float vz = -od;
       float enc = gs_clampf((near + vz) * far / ((far - near) * vz), 0.0f, 1.0f);
-      if (finite(enc)) dev = enc;
+      if (enc == enc && gs_fabsf(enc) <= kMaxF32) dev = enc;
     }
     depth_out[p] = dev;
   }
+}
+
+// Light tiles (up to kLightMax records): one 256-thread block per tile.
+__global__ __launch_bounds__(kLightThreads) void k_raster_light(const FrameParams fp,
+                                                                DeviceState* __restrict__ state,
+                                                                const uint2* __restrict__ sorted,
+                                                                const uint32_t* __restrict__ off,
+                                                                const float4* __restrict__ pmean,
+                                                                const float4* __restrict__ pconic,
+                                                                const float4* __restrict__ pcolor,
+                                                                uint16_t* __restrict__ color_out,
+                                                                float* __restrict__ depth_out) {
+  __shared__ RasterLds<kLightThreads> lds;
+  if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
+  uint32_t tile = blockIdx.x;
+  if (tile >= fp.tile_count) return;
+  uint32_t begin = off[tile], end = off[tile + 1];
+  if (end - begin > kLightMax) return;  // heavy unit
+  uint32_t x, y;
+  bool active;
+  PixelSum s = blend_tile_range<kLightThreads>(fp, tile, begin, end, sorted, pmean, pconic,
+                                               pcolor, lds, x, y, active);
+  if (fp.raster_stats) add_stats(state, s.checked, s.blended);
+  if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+}
+
+// rasterChunks.ts:3-65 in one block, extended into the heavy work list:
+// first the chunk tasks of tiles above chunk_size (the reference's exact
+// chunks, each blended from T = 1 and composited later; chunk_first holds
+// each tile's first task), then the remaining tiles above kLightMax,
+// heaviest size class first. Units are (tile, chunk) with chunk = ~0u for a
+// whole tile.
+__global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
+                                                      DeviceState* __restrict__ state,
+                                                      const uint32_t* __restrict__ off,
+                                                      uint32_t* __restrict__ chunk_first,
+                                                      uint2* __restrict__ units,
+                                                      uint32_t unit_capacity) {
+  __shared__ uint32_t wave_sums[16];
+  __shared__ uint32_t carry_s;
+  __shared__ uint32_t class_count[5], class_next[5];
+  if (threadIdx.x == 0) carry_s = 0;
+  if (threadIdx.x < 5) class_count[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t tiles = state->overflow ? 0u : fp.tile_count;
+  uint32_t cs = fp.chunk_size;
+  for (uint32_t base = 0; base < tiles; base += 1024) {
+    uint32_t t = base + threadIdx.x;
+    uint32_t n = 0;
+    if (t < tiles) {
+      uint32_t count = off[t + 1] - off[t];
+      if (cs != 0 && count > cs) {
+        n = (count + cs - 1) / cs;
+      } else if (count > kLightMax) {
+        uint32_t c = 31u - (uint32_t)__builtin_clz((count - 1u) >> 10);
+        atomicAdd(&class_count[min(c, 4u)], 1u);
+      }
+    }
+    uint32_t incl = wave_inclusive_scan(n);
+    uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t first = carry_s;
+    for (uint32_t w = 0; w < wave; w++) first += wave_sums[w];
+    first += incl - n;
+    if (t < tiles) chunk_first[t] = first;
+    for (uint32_t c = 0; c < n; c++)
+      if (first + c < unit_capacity) units[first + c] = make_uint2(t, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t c = carry_s;
+      for (uint32_t w = 0; w < 16; w++) c += wave_sums[w];
+      carry_s = c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    uint32_t next = carry_s;
+    state->chunk_tasks = min(carry_s, unit_capacity);
+    for (int c = 4; c >= 0; c--) {
+      class_next[c] = next;
+      next += class_count[c];
+    }
+    state->heavy_units = min(next, unit_capacity);
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < tiles; t += 1024) {
+    uint32_t count = off[t + 1] - off[t];
+    if ((cs == 0 || count <= cs) && count > kLightMax) {
+      uint32_t c = min(31u - (uint32_t)__builtin_clz((count - 1u) >> 10), 4u);
+      uint32_t slot = atomicAdd(&class_next[c], 1u);
+      if (slot < unit_capacity) units[slot] = make_uint2(t, ~0u);
+    }
+  }
+}
+
+// Heavy units: one 1024-thread block per chunk task or heavy tile.
+__global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
+    const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
+    const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
+    const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
+    const uint2* __restrict__ units, float4* __restrict__ partials,
+    float* __restrict__ partial_depth, uint16_t* __restrict__ color_out,
+    float* __restrict__ depth_out) {
+  __shared__ RasterLds<kHeavyThreads> lds;
+  uint32_t u = blockIdx.x;
+  if (u >= state->heavy_units) return;
+  uint2 unit = units[u];
+  uint32_t tile = unit.x;
+  uint32_t begin = off[tile], end = off[tile + 1];
+  uint32_t cb = begin, ce = end;
+  bool chunk = unit.y != ~0u;
+  if (chunk) {
+    cb = begin + unit.y * fp.chunk_size;
+    ce = min(cb + fp.chunk_size, end);
+  }
+  uint32_t x, y;
+  bool active;
+  PixelSum s = blend_tile_range<kHeavyThreads>(fp, tile, cb, ce, sorted, pmean, pconic, pcolor,
+                                               lds, x, y, active);
+  if (fp.raster_stats) add_stats(state, s.checked, s.blended);
+  if (threadIdx.x >= (uint32_t)kBlock) return;
+  if (chunk) {
+    size_t p = (size_t)u * kBlock + threadIdx.x;  // chunk tasks lead the list
+    partials[p] = make_float4(s.r, s.g, s.b, s.t);
+    partial_depth[p] = s.d;
+  } else if (active) {
+    store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+  }
+}
+
+// TileRasterizer.ts:807-902: ordered over-composite of a tile's chunks.
+__global__ __launch_bounds__(kBlock) void k_composite(const FrameParams fp,
+                                                      const DeviceState* __restrict__ state,
+                                                      const uint32_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ chunk_first,
+                                                      const float4* __restrict__ partials,
+                                                      const float* __restrict__ partial_depth,
+                                                      uint16_t* __restrict__ color_out,
+                                                      float* __restrict__ depth_out) {
+  if (state->overflow) return;
+  uint32_t tile = blockIdx.x;
+  if (tile >= fp.tile_count || fp.chunk_size == 0) return;
+  uint32_t count = off[tile + 1] - off[tile];
+  uint32_t cs = fp.chunk_size;
+  if (count <= cs) return;
+  uint32_t n = (count + cs - 1) / cs;
+  uint32_t t = threadIdx.x;
+  uint32_t x = (tile % fp.tiles_x) * kTile + (t & 15u);
+  uint32_t y = (tile / fp.tiles_x) * kTile + (t >> 4);
+  if (x >= fp.width || y >= fp.height) return;
+  float r = 0.0f, g = 0.0f, b = 0.0f, d = 0.0f, tr = 1.0f;
+  uint32_t first = chunk_first[tile];
+  for (uint32_t k = 0; k < n; k++) {
+    size_t p = (size_t)(first + k) * kBlock + t;
+    float4 c = partials[p];
+    r = r + c.x * tr;
+    g = g + c.y * tr;
+    b = b + c.z * tr;
+    d = d + partial_depth[p] * tr;
+    tr = tr * c.w;
+    if (tr < fp.transmittance_threshold) break;
+  }
+  store_pixel(fp, x, y, r, g, b, d, tr, color_out, depth_out);
 }
 
 // --------------------------------------------------------------- driver ----
@@ -704,12 +1003,18 @@ void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_coun
 
 static_assert(kPartition == 4096, "hist LenSpec uses log2(kPartition) = 12");
 
-hipError_t launch_frame(const FrameBuffers& b, hipStream_t s) {
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, hipEvent_t fork,
+                        hipEvent_t join, hipEvent_t* marks) {
+  auto mark = [&](int stage) {
+    if (marks) (void)hipEventRecord(marks[stage], s);
+  };
   const FrameParams& fp = b.params;
   DeviceState* st = b.state;
   uint32_t n_blocks = div_up(b.max_n, kBlock);
   if (n_blocks == 0) n_blocks = 1;
+  mark(kStageProject);
   k_project<<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj);
+  mark(kStageVisible);
 
   // Visible flags -> exclusive offsets -> compacted (depth key, ordinal).
   LenSpec active_len = {nullptr, 0u, 1u, fp.active_count};
@@ -724,11 +1029,13 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s) {
       <<<part_blocks, kBlock, 0, s>>>(lv, sc, active_len, b.partials);
 
   // Depth sort over V.
+  mark(kStageDepthSort);
   radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.hist,
              b.partials, st, s);
   const uint2* sorted_depth = sorted_depth_records(b);
 
   // Depth-ordered tile counts -> write offsets; K, overflow, output state.
+  mark(kStageScan);
   LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
   LoadOrdered lo{sorted_depth, b.proj.tile_counts};
   k_scan_reduce<LoadOrdered><<<part_blocks, kBlock, 0, s>>>(lo, vis_len, b.partials);
@@ -737,18 +1044,40 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s) {
   k_scan_downsweep<LoadOrdered, StoreOrdered><<<part_blocks, kBlock, 0, s>>>(
       lo, StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, b.partials);
 
+  mark(kStageEmit);
   k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.mean, b.proj.conic,
                                      b.ordered_counts, b.intersection_offsets, b.records_a);
 
+  mark(kStageTileSort);
   radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.hist,
              b.partials, st, s);
   const uint2* sorted = sorted_tile_records(b);
 
-  k_tile_offsets<<<div_up((uint64_t)b.max_k + 1, kBlock), kBlock, 0, s>>>(fp, st, sorted,
-                                                                          b.tile_offsets);
-  k_raster<<<b.max_tiles, kBlock, 0, s>>>(fp, st, sorted, b.tile_offsets, b.proj.mean,
-                                          b.proj.conic, b.proj.color, b.color_out,
-                                          b.depth_out);
+  mark(kStageTileOffsets);
+  uint32_t off_blocks = div_up((uint64_t)b.max_k + 1, kBlock);
+  if (off_blocks > 2048) off_blocks = 2048;
+  k_tile_offsets<<<off_blocks, kBlock, 0, s>>>(fp, st, sorted, b.tile_offsets);
+  mark(kStageRaster);
+  k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
+                                   b.unit_capacity);
+  hipError_t e = hipEventRecord(fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+  if (e != hipSuccess) return e;
+  k_raster_heavy<<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
+      fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
+      b.chunk_partials, b.partial_depth, b.color_out, b.depth_out);
+  e = hipEventRecord(join, aux);
+  if (e != hipSuccess) return e;
+  k_raster_light<<<b.max_tiles, kLightThreads, 0, s>>>(fp, st, sorted, b.tile_offsets, b.proj.mean,
+                                                       b.proj.conic, b.proj.color, b.color_out,
+                                                       b.depth_out);
+  e = hipStreamWaitEvent(s, join, 0);
+  if (e != hipSuccess) return e;
+  if (fp.chunk_size != 0)
+    k_composite<<<b.max_tiles, kBlock, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first,
+                                               b.chunk_partials, b.partial_depth, b.color_out,
+                                               b.depth_out);
+  mark(kStageCount);
   return hipGetLastError();
 }
 

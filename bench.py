@@ -1,0 +1,207 @@
+"""Frame-pipeline benchmark: frames/sec and sorted keys/sec at 1M Gaussians,
+1920x1080, on one MI355X, with the CPU restatement timed beside it.
+
+A step is one full frame of the plain path (projection, visible compaction,
+depth sort, ordered scan, emission, tile sort, tile ranges, raster) over the
+1M-Gaussian synthetic scene, inputs resident in HBM, replayed as a captured
+graph. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "frames/sec and sorted keys/sec at 1M Gaussians 1080p; learned-cull speedup at 45dB"
+STAGES = ["project", "visible", "depth_sort", "scan", "emit", "tile_sort", "tile_offsets", "raster"]
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def stage_bytes(n_active, visible, keys, tiles, pixels, depth_passes, tile_passes):
+    """Algorithmic bytes per launch of each stage (minimum traffic of the
+    algorithm run, not the implementation's extra traffic)."""
+    return {
+        "project": 148 * n_active,                      # 96 B in + 52 B out per Gaussian
+        "visible": 4 * n_active + 4 * n_active + 8 * visible,  # flag in, offset out, record out
+        "depth_sort": 16 * visible * depth_passes,      # read + write 8 B pair per pass
+        "scan": 8 * visible + 4 * visible,              # gathered count in, offset out (+count)
+        "emit": 48 * visible + 8 * keys,
+        "tile_sort": 16 * keys * tile_passes,
+        "tile_offsets": 4 * keys + 4 * tiles,
+        "raster": 8 * keys + 48 * keys + 8 * pixels,
+    }
+
+
+def cpu_baseline(gs, scene, camera, workload, threads):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import gs_oracle  # the oracle, timed as the CPU restatement of the path
+
+    frame = gs.frame_for(camera, workload.width, workload.height)
+    objects = gs.object_frames_for(camera)
+    gs_oracle.lib()
+    t0 = time.perf_counter()
+    r = gs_oracle.run_frame(scene, frame, objects, capacity=1 << 28, threads=threads)
+    secs = time.perf_counter() - t0
+    return {
+        "value": round(1.0 / secs, 4),
+        "unit": "frames/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one full frame of {workload.name} ({workload.count} Gaussians, "
+                  f"{workload.width}x{workload.height}) on {threads} host threads",
+        "seconds": round(secs, 3),
+        "stage_seconds": {s: round(v, 4) for s, v in zip(STAGES, r.counts.stage_seconds)},
+        "keys": int(r.counts.count),
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="1m_1080p")
+    ap.add_argument("--profile-frames", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "note only (tile-band or camera sharding), run with --gpus 1"}))
+        sys.exit(2)
+
+    gs = importlib.import_module("3dgs_tile_webgpu_amd")
+    import torch  # device memory bookkeeping only
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    w = gs.WORKLOADS[args.workload]
+    scene = gs.workload_scene(args.workload)
+    camera = w.camera()
+    p = gs.gaussian_pass(camera, scene, {"initialIntersectionCapacity": 1 << 22})
+    p.render(w.width, w.height)  # grows capacity if needed, captures the graph
+    st = p.read_state()
+    if st.overflow:
+        p.render(w.width, w.height)
+        st = p.read_state()
+    assert not st.overflow, "intersection capacity did not converge"
+    p.set_objects(p.object_frames())
+    frame = p.frame_struct(w.width, w.height, p.latest_attempt + 1)
+    lib = gs.lib()
+    ctx = p._ctx
+
+    def render():
+        gs._native.check(lib.gs_render(ctx, C.byref(frame), None), ctx)
+
+    for _ in range(args.warmup):
+        render()
+    torch.cuda.synchronize()
+    p.read_state()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        render()
+    p.read_state()  # synchronises the context's stream
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    fps = args.steps / elapsed
+    st = p.read_state()
+    keys = int(st.intersections)
+
+    # Per-stage GPU time with HIP events on the pass's stream (eager launches).
+    nprof = args.profile_frames
+    gs._native.check(lib.gs_profile_begin(ctx, nprof), ctx)
+    for _ in range(nprof):
+        render()
+    sums = (C.c_double * 8)()
+    frame_ms = C.c_double()
+    frames = C.c_uint32()
+    gs._native.check(lib.gs_profile_end(ctx, sums, C.byref(frame_ms), C.byref(frames)), ctx)
+    stage_ms = {s: sums[i] / frames.value for i, s in enumerate(STAGES)}
+    eager_frame_ms = frame_ms.value / frames.value
+
+    tiles = st.tiles_x * st.tiles_y
+    tile_passes = max(1, -(-int(tiles).bit_length() // 8))
+    depth_passes = 4
+    algo = stage_bytes(scene.active_count, st.visible, keys, tiles, w.width * w.height,
+                       depth_passes, tile_passes)
+    sort_ms = stage_ms["depth_sort"] + stage_ms["tile_sort"]
+    dominant = max(STAGES, key=lambda s: stage_ms[s])
+    if dominant == "raster":
+        # FP32 roofline: checked pairs x (20 FP32 ops + 1 exp) per launch.
+        p.dispose()
+        p2 = gs.gaussian_pass(camera, scene, {"intersectionCapacity": p.capacity, "rasterStats": True})
+        p2.render(w.width, w.height, retry_overflow=False)
+        checked = p2.read_state().raster_checked
+        p2.dispose()
+        flops = checked * 21
+        achieved = flops / (stage_ms["raster"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                    "traffic": None, "kernel": "k_raster", "stage_ms": round(stage_ms["raster"], 4),
+                    "algorithmic_flops": int(flops), "checked_pairs": int(checked)}
+    else:
+        achieved = algo[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "stage": dominant, "stage_ms": round(stage_ms[dominant], 4),
+                    "algorithmic_bytes": int(algo[dominant])}
+        p.dispose()
+    stage_gbs = {s: round(algo[s] / (stage_ms[s] * 1e-3) / 1e9, 1) for s in STAGES if stage_ms[s] > 0}
+
+    result = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "gaussians": w.count, "width": w.width, "height": w.height,
+                   "sh_degree": 3, "tile": 16, "camera_distance": w.camera_distance,
+                   "geometry": "compact28+rgb8e8", "graph": True},
+        "sorted_keys_per_s": round(keys / (sort_ms * 1e-3), 1) if sort_ms > 0 else None,
+        "keys": keys,
+        "visible": int(st.visible),
+        "stage_ms": {s: round(v, 4) for s, v in stage_ms.items()},
+        "stage_gbs": stage_gbs,
+        "eager_frame_ms": round(eager_frame_ms, 4),
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        result["cpu_baseline"] = cpu_baseline(gs, scene, camera, w, threads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

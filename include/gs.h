@@ -70,6 +70,9 @@ typedef struct gs_config {
   float invalid_depth_value;         /* default 1 */
   float background[4];               /* default (0, 0, 0, 0) */
   uint32_t raster_stats;             /* count checked/blended pairs */
+  uint32_t capture_graph;            /* replay frames as a captured hipGraph
+                                        (default 1); re-captured when sizes,
+                                        buffers or uniforms change */
 } gs_config;
 
 /* Packed Gaussian buffers (GaussianData.ts:10-33 / GaussianBuffers), copied
@@ -188,6 +191,18 @@ int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t bytes,
  * output images, for callers that chain their own kernels. */
 void* gs_stream(gs_ctx* ctx);
 int gs_device_image(gs_ctx* ctx, void** color, void** depth);
+
+/* Per-stage GPU time with HIP events recorded on the context's stream
+ * (no reference counterpart; the reference's timestamp queries feed the
+ * sandbox's KernelTimingInspector). While profiling, frames launch stage by
+ * stage instead of as a graph. Stages: projection, visible scan+compaction,
+ * depth sort, ordered scan, emission, tile sort, tile ranges, raster. */
+#define GS_STAGE_COUNT 8
+int gs_profile_begin(gs_ctx* ctx, uint32_t max_frames);
+/* Synchronises; stage_ms receives the per-stage sums over the recorded
+ * frames, frame_ms the sum of whole-frame times, *frames the count. */
+int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], double* frame_ms,
+                   uint32_t* frames);
 
 /* Capacity growth rule (IntersectionCapacityPolicy.ts:90-101): returns the
  * next capacity, or 0 when demand exceeds maximum. */

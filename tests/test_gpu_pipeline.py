@@ -3,7 +3,9 @@
 Integer outputs (tile counts, visible offsets, depth keys and order, ordered
 counts, intersection offsets, sorted (tile, id) records, tile ranges) must be
 bit-exact, and so must every projected float and the rgba16float image: both
-sides evaluate the same binary32 operations in the reference's order.
+sides evaluate the same binary32 operations in the reference's order. The
+image is also held to the FP64 blend of the same lists (max |delta| <= 1/255
+per channel, PSNR stated).
 """
 from __future__ import annotations
 
@@ -16,25 +18,28 @@ STAGES_INT = ["tile_counts", "visible_offsets", "depth_records", "ordered_counts
               "intersection_offsets", "sorted_records", "tile_offsets"]
 
 
-def render_both(gs, oracle, scene, camera, width, height, options=None, capacity=1 << 22):
+def oracle_for(gs, oracle, p, scene, camera, width, height, objects=None):
+    frame = gs.frame_for(camera, width, height, p.options)
+    o = p.options
+    return oracle.run_frame(scene, frame, p.object_frames() if objects is None else objects,
+                            capacity=p.capacity, subpixel=o["subpixelSampleCulling"],
+                            compensated=o["antialiasMode"] == "compensated",
+                            depth_mode=0 if o["depthSortMode"] == "float32" else 1,
+                            chunk_size=o["rasterChunkSize"] or 0,
+                            transmittance=o["rasterTransmittanceThreshold"],
+                            background=o["background"])
+
+
+def render_both(gs, oracle, scene, camera, width, height, options=None, capacity=1 << 22,
+                objects=None):
     options = dict(options or {})
     options.setdefault("intersectionCapacity", capacity)
     p = gs.gaussian_pass(camera, scene, options)
-    p.render(width, height, retry_overflow=False)
-    frame = gs.frame_for(camera, width, height, options)
-    objects = p.object_frames()
-    o = p.options
-    ref = oracle.run_frame(scene, frame, objects, capacity=p.capacity,
-                           subpixel=o["subpixelSampleCulling"],
-                           compensated=o["antialiasMode"] == "compensated",
-                           depth_mode=0 if o["depthSortMode"] == "float32" else 1,
-                           chunk_size=o["rasterChunkSize"] or 0,
-                           transmittance=o["rasterTransmittanceThreshold"],
-                           background=o["background"])
-    return p, ref
+    p.render(width, height, objects=objects, retry_overflow=False)
+    return p, oracle_for(gs, oracle, p, scene, camera, width, height, objects)
 
 
-def assert_frame_matches(p, ref):
+def assert_frame_matches(p, ref, check_f64=True):
     st = p.read_state()
     c = ref.counts
     assert (st.visible, st.requested, st.intersections, st.overflow) == (
@@ -51,12 +56,121 @@ def assert_frame_matches(p, ref):
         np.testing.assert_array_equal(res[name], ref.get(name), err_msg=name)
     img = p.read_image()
     np.testing.assert_array_equal(img.view(np.uint16), ref.get("image_f16"))
+    if check_f64 and not c.overflow:
+        exact = ref.blend_f64()
+        diff = np.abs(img.astype(np.float64) - exact)
+        mse = float(np.mean(diff[..., :3] ** 2))
+        psnr = 10 * np.log10(1.0 / mse) if mse > 0 else float("inf")
+        print(f"vs FP64 blend: max|d|={diff.max():.3e} PSNR={psnr:.1f} dB")
+        assert diff.max() <= 1.0 / 255.0
+        assert psnr >= 45.0
 
 
-def test_small_workload_bit_exact(gs, oracle, gpu_available):
+@pytest.fixture(scope="module")
+def small(gs):
+    return gs.workload_scene("small")
+
+
+def test_small_workload_bit_exact(gs, oracle, gpu_available, small):
     w = gs.WORKLOADS["small"]
-    scene = gs.workload_scene("small")
-    p, ref = render_both(gs, oracle, scene, w.camera(), w.width, w.height)
+    p, ref = render_both(gs, oracle, small, w.camera(), w.width, w.height)
     assert ref.counts.count > 100_000
+    assert_frame_matches(p, ref)
+    p.dispose()
+
+
+@pytest.mark.parametrize("options", [
+    {"rasterChunkSize": 256},
+    {"rasterChunkSize": None},
+    {"depthSortMode": "packed16"},
+    {"antialiasMode": "classic", "filterVariance": 0.1},
+    {"subpixelSampleCulling": False},
+    {"background": (0.2, 0.4, 0.6, 0.5), "rasterTransmittanceThreshold": 0.01},
+    {"boundaryPolicy": "repeat-x"},
+    {"boundaryPolicy": "repeat-y"},
+], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
+def test_option_variants_bit_exact(gs, oracle, gpu_available, small, options):
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    p, ref = render_both(gs, oracle, small, cam, 640, 360, options)
+    if options.get("rasterChunkSize") == 256:
+        assert ref.counts.chunk_tasks > 0
+    assert_frame_matches(p, ref)
+    p.dispose()
+
+
+def test_active_slots_holes_and_objects(gs, oracle, gpu_available):
+    """Sparse slots (means.w = -1 holes), an active list in shuffled order,
+    two objects with different transforms, one hidden."""
+    scene = gs.workload_scene("small", count=40_000)
+    rng = np.random.default_rng(5)
+    means = scene.means.copy()
+    means[rng.random(means.shape[0]) < 0.1, 3] = -1.0          # unoccupied slots
+    obj = (rng.random(means.shape[0]) < 0.3) & (means[:, 3] >= 0)
+    means[obj, 3] = 1.0
+    hidden = (rng.random(means.shape[0]) < 0.05) & (means[:, 3] >= 0)
+    means[hidden, 3] = 2.0
+    active = rng.permutation(means.shape[0])[:30_000].astype(np.uint32)
+    scene = gs.PackedScene(means, scene.scales_opacity, scene.rotations, scene.sh, 3, active)
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    world1 = np.eye(4)
+    world1[:3, 3] = [1.5, -0.5, 2.0]
+    c, s = np.cos(0.4), np.sin(0.4)
+    world1[:3, :3] = [[c, 0, s], [0, 1, 0], [-s, 0, c]]
+    objects = np.concatenate([gs.object_frames_for(cam), gs.object_frames_for(cam, world1),
+                              gs.object_frames_for(cam, np.eye(4), visible=False)])
+    p, ref = render_both(gs, oracle, scene, cam, 800, 600, objects=objects)
+    assert_frame_matches(p, ref)
+    p.dispose()
+
+
+def test_overflow_publishes_nothing_then_grows(gs, oracle, gpu_available, small):
+    """intersections.ts:10-41/114-134 and docs/dynamic-intersections.md: an
+    overflowing frame emits and rasterises nothing and keeps the previous
+    image; the policy grows the capacity and the retry matches the oracle."""
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    p = gs.gaussian_pass(cam, small, {"initialIntersectionCapacity": 1024,
+                                      "maxIntersectionCapacity": 1 << 22})
+    p.render(w.width, w.height, retry_overflow=False)
+    st = p.read_state()
+    assert st.overflow == 1 and st.intersections == 0 and st.requested > 1024
+    assert p.get_output_status().state == "overflow-pending"
+    before = p.read_image().copy()
+    p.render(w.width, w.height, retry_overflow=False)  # still overflowing: image kept
+    np.testing.assert_array_equal(p.read_image().view(np.uint16), before.view(np.uint16))
+    p.render(w.width, w.height)  # grows and retries
+    assert p.growth_count == 1 and p.capacity >= st.requested
+    ref = oracle_for(gs, oracle, p, small, cam, w.width, w.height)
+    assert_frame_matches(p, ref, check_f64=False)
+    status = p.get_output_status()
+    assert status.state == "ready" and status.valid
+    p.dispose()
+
+
+def test_graph_replay_and_recapture(gs, oracle, gpu_available, small):
+    """A replayed frame is bit-identical; a new camera re-captures and still
+    matches the oracle."""
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    p, ref = render_both(gs, oracle, small, cam, w.width, w.height)
+    first = p.read_image().copy()
+    for _ in range(3):
+        p.render(w.width, w.height, retry_overflow=False)
+        np.testing.assert_array_equal(p.read_image().view(np.uint16), first.view(np.uint16))
+    cam2 = gs.PerspectiveCamera(fov=50.0, aspect=w.width / w.height, position=(4.0, 3.0, 12.0))
+    p.camera = cam2
+    p.render(w.width, w.height, retry_overflow=False)
+    ref2 = oracle_for(gs, oracle, p, small, cam2, w.width, w.height)
+    assert_frame_matches(p, ref2, check_f64=False)
+    p.dispose()
+
+
+def test_1m_1080p_bit_exact(gs, oracle, gpu_available):
+    w = gs.WORKLOADS["1m_1080p"]
+    scene = gs.workload_scene("1m_1080p")
+    p, ref = render_both(gs, oracle, scene, w.camera(), w.width, w.height)
+    assert ref.counts.chunk_tasks > 0  # dense cluster tiles take the chunk path
     assert_frame_matches(p, ref)
     p.dispose()
