@@ -90,4 +90,6 @@ def test_half_conversions_exhaustive(mathfn):
     x = np.concatenate([RNG.normal(scale=s, size=50_000) for s in (1e-6, 1e-3, 1.0, 100.0, 7e4)]).astype(np.float32)
     x = np.concatenate([x, ref[np.isfinite(ref)], ref[np.isfinite(ref)] * np.float32(1.0004883)])
     got = mathfn(F2H, x).view(np.uint32) & 0xFFFF
-    np.testing.assert_array_equal(got.astype(np.uint16), x.astype(np.float16).view(np.uint16))
+    with np.errstate(over="ignore"):
+        expect = x.astype(np.float16).view(np.uint16)
+    np.testing.assert_array_equal(got.astype(np.uint16), expect)
