@@ -5,10 +5,13 @@ The package name starts with a digit, so import it with
 
 Public API (mirrors the reference's pass API for the frame path):
   gaussian_pass / GaussianPass   -- one camera's pass over the C ABI
+  BatchedPass                    -- many cameras in one captured launch,
+                                    with the launch-shape autotuner
   PackedScene, workload_scene    -- packed buffers and synthetic workloads
   PerspectiveCamera              -- Three.js camera conventions
 """
 from ._native import GsError, GsRangeError, GsTypeError, lib  # noqa: F401
+from .batch import BatchedPass  # noqa: F401
 from .camera import PerspectiveCamera, elements, orbit_cameras  # noqa: F401
 from .gaussian_pass import (  # noqa: F401
     GaussianPass,

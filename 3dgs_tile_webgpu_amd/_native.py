@@ -48,6 +48,10 @@ class GsConfig(C.Structure):
         ("background", f32 * 4),
         ("raster_stats", u32),
         ("capture_graph", u32),
+        ("tile_size", u32),
+        ("radix_bits", u32),
+        ("raster_batch", u32),
+        ("batch_lanes", u32),
     ]
 
 
@@ -78,6 +82,7 @@ class GsFrame(C.Structure):
         ("boundary_axis", u32),
         ("attempt", u32),
         ("reset_valid", u32),
+        ("object_frames", P),
     ]
 
 
@@ -99,6 +104,14 @@ class GsState(C.Structure):
         ("raster_checked", C.c_uint64),
         ("raster_blended", C.c_uint64),
     ]
+
+
+class GsTuning(C.Structure):
+    _fields_ = [("tile_size", u32), ("radix_bits", u32), ("raster_batch", u32)]
+
+
+class GsTuneEntry(C.Structure):
+    _fields_ = [("tuning", GsTuning), ("ms", C.c_double)]
 
 
 class GsSceneSpec(C.Structure):
@@ -146,6 +159,14 @@ SIGNATURES = {
     "gs_object_frame": (None, [P, P, C.c_int, P]),
     "gs_render": (C.c_int, [P, C.POINTER(GsFrame), C.POINTER(GsImage)]),
     "gs_render_batch": (C.c_int, [P, u32, C.POINTER(GsFrame), C.POINTER(GsImage)]),
+    "gs_read_batch_state": (C.c_int, [P, u32, u32, C.POINTER(GsState)]),
+    "gs_resize_batch_intersections": (C.c_int, [P, u32, u32]),
+    "gs_read_batch_image": (C.c_int, [P, u32, P, C.c_size_t, P, C.c_size_t]),
+    "gs_batch_device_image": (C.c_int, [P, u32, C.POINTER(P), C.POINTER(P)]),
+    "gs_set_tuning": (C.c_int, [P, C.POINTER(GsTuning)]),
+    "gs_get_tuning": (C.c_int, [P, C.POINTER(GsTuning)]),
+    "gs_autotune": (C.c_int, [P, u32, C.POINTER(GsFrame), u32, C.POINTER(GsTuning),
+                              C.POINTER(GsTuneEntry), u32, C.POINTER(u32)]),
     "gs_read_state": (C.c_int, [P, C.POINTER(GsState)]),
     "gs_resize_intersections": (C.c_int, [P, u32]),
     "gs_read_image": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t]),

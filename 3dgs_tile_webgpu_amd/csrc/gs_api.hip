@@ -3,10 +3,21 @@
 // The context plays the role of the reference's TiledGaussianPipeline
 // (pipeline/TiledGaussianPipeline.ts:39-484): it owns every device buffer,
 // grows them geometrically and never shrinks them (AttributePool.ts:41-67),
-// and enqueues one frame per gs_render on its own HIP stream.
+// and enqueues frames on its own HIP stream.
+//
+// Frames run on lanes: a lane is one frame's working set (projected arrays,
+// records, histograms, tile ranges, raster partials) plus a stream pair.
+// Lane 0 uses the context's stream. A view set is a group of frames
+// launched together -- the single camera of gs_render, or the cameras of
+// gs_render_batch spread over lanes -- with one DeviceState per view, one
+// device table of per-view camera uniforms and object frames, and one
+// captured graph that is replayed while the launch description is
+// unchanged. The camera table is uploaded before every launch, so moving a
+// camera never re-captures.
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -15,6 +26,7 @@
 
 using gs::DeviceState;
 using gs::FrameBuffers;
+using gs::FrameCamera;
 using gs::FrameParams;
 
 namespace {
@@ -24,19 +36,12 @@ struct DeviceBuffer {
   size_t bytes = 0;
 };
 
-}  // namespace
+constexpr uint32_t kMaxLanes = 16;
 
-struct gs_ctx {
-  gs_config cfg;
-  hipStream_t stream = nullptr;
-  hipStream_t aux = nullptr;  // second raster branch, forked per frame
-  hipEvent_t fork = nullptr, join = nullptr;
-  std::string error;
-  // Scene (GaussianData contract).
-  uint32_t count = 0, active_count = 0, sh_degree = 0;
-  bool has_data = false, has_active = false;
-  DeviceBuffer means, scales_opacity, rotations, sh, active, objects;
-  uint32_t object_count = 0;
+struct Lane {
+  hipStream_t stream = nullptr;  // lane 0: the context stream (not owned)
+  hipStream_t aux = nullptr;     // second raster branch, forked per frame
+  hipEvent_t fork = nullptr, join = nullptr, done = nullptr;
   // Per-Gaussian frame arrays, sized for slot_capacity Gaussians.
   uint32_t slot_capacity = 0;
   DeviceBuffer pmean, pconic, pcolor, tile_counts, visible_offsets, depth_a, depth_b,
@@ -45,36 +50,62 @@ struct gs_ctx {
   uint32_t capacity = 0;
   DeviceBuffer records_a, records_b;
   // Radix histograms and scan partials, sized for max(slots, capacity).
+  uint32_t sort_capacity = 0;
   DeviceBuffer hist, partials;
   // Tile ranges, sized for tile_capacity + 1.
   uint32_t tile_capacity = 0;
   DeviceBuffer tile_offsets, chunk_first;
-  // Raster chunk tasks and partials (RasterChunkWorkspace.ts:12-59).
+  // Raster heavy units and chunk partials (RasterChunkWorkspace.ts:12-59).
   DeviceBuffer tasks, partials_px, partial_depth;
-  // Output images.
-  uint32_t image_w = 0, image_h = 0;
-  DeviceBuffer color, depth;
-  DeviceBuffer state;
-  // Object table: host copy, uploaded through a ring of pinned slots.
+};
+
+struct ViewSet {
+  uint32_t n = 0;
+  DeviceBuffer state;    // n DeviceState
+  DeviceBuffer dynamic;  // n * stride bytes: FrameCamera, then object frames
+  size_t stride = 0;
+  std::vector<uint32_t> capacity;  // intersection capacity per view
+  std::vector<uint32_t> lane_of;
+  std::vector<DeviceBuffer> color, depth;  // library-owned images per view
+  std::vector<FrameBuffers> frames;        // last launch description
+  std::vector<uint32_t> width, height;
+  // Captured graph and the description it was captured from.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<FrameBuffers> key;
+  std::vector<uint32_t> key_lanes;
+  bool graph_valid = false;
+  bool rendered = false;
+};
+
+}  // namespace
+
+struct gs_ctx {
+  gs_config cfg;
+  hipStream_t stream = nullptr;
+  hipEvent_t batch_fork = nullptr;
+  std::string error;
+  // Scene (GaussianData contract).
+  uint32_t count = 0, active_count = 0, sh_degree = 0;
+  bool has_data = false, has_active = false;
+  DeviceBuffer means, scales_opacity, rotations, sh, active;
+  // Object frame table set by gs_set_objects (host copy).
   std::vector<float> objects_host;
-  bool objects_dirty = false;
+  uint32_t object_count = 0;
+  // Intersection capacity of the single-camera path.
+  uint32_t capacity = 0;
+  std::vector<Lane> lanes;
+  ViewSet single, batch;
+  // Camera-table staging: a ring of pinned slots.
   static constexpr int kRing = 8;
-  float* ring[kRing] = {};
+  void* ring[kRing] = {};
   hipEvent_t ring_done[kRing] = {};
   size_t ring_bytes = 0;
   int ring_next = 0;
-  // Captured frame graph and the launch description it was captured from.
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  FrameBuffers graph_key = {};
-  bool graph_valid = false;
-  // Stage profiling.
+  std::vector<unsigned char> staging;
+  // Stage profiling (single-camera path).
   std::vector<hipEvent_t> marks;
   uint32_t profile_capacity = 0, profile_frames = 0;
-  // Last frame.
-  FrameBuffers last = {};
-  bool rendered = false;
-  uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
 };
 
 namespace {
@@ -88,7 +119,8 @@ int hip_fail(gs_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, GS_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Grow-only device allocation (AttributePool.ts:41-67 semantics).
+// Grow-only device allocation (AttributePool.ts:41-67 semantics). Every
+// lane joins the context stream, so synchronising it retires all users.
 int ensure(gs_ctx* ctx, DeviceBuffer& b, size_t bytes, const char* what) {
   if (bytes == 0) bytes = 16;
   if (b.bytes >= bytes) return GS_OK;
@@ -128,35 +160,95 @@ uint32_t bits_for(uint32_t v) {  // bits needed to represent v
   return b;
 }
 
-int ensure_slots(gs_ctx* ctx, uint32_t n) {
-  if (n <= ctx->slot_capacity && ctx->pmean.ptr) return GS_OK;
-  uint32_t cap = ctx->slot_capacity ? ctx->slot_capacity : 1024u;
-  while (cap < n) cap = cap > 0x7fffffffu ? n : cap * 2u;
+uint32_t grow_to(uint32_t current, uint32_t need) {
+  uint32_t cap = current ? current : 1024u;
+  while (cap < need) cap = cap > 0x7fffffffu ? need : cap * 2u;
+  return cap;
+}
+
+int lane_slots(gs_ctx* ctx, Lane& l, uint32_t n) {
+  if (n <= l.slot_capacity && l.pmean.ptr) return GS_OK;
+  uint32_t cap = grow_to(l.slot_capacity, n);
   int rc;
   size_t c = cap;
-  if ((rc = ensure(ctx, ctx->pmean, c * 16, "projected-mean")) ||
-      (rc = ensure(ctx, ctx->pconic, c * 16, "projected-conic")) ||
-      (rc = ensure(ctx, ctx->pcolor, c * 16, "projected-color")) ||
-      (rc = ensure(ctx, ctx->tile_counts, c * 4, "tile-counts")) ||
-      (rc = ensure(ctx, ctx->visible_offsets, c * 4, "visible-offsets")) ||
-      (rc = ensure(ctx, ctx->depth_a, c * 8, "depth-records-a")) ||
-      (rc = ensure(ctx, ctx->depth_b, c * 8, "depth-records-b")) ||
-      (rc = ensure(ctx, ctx->ordered_counts, c * 4, "ordered-tile-counts")) ||
-      (rc = ensure(ctx, ctx->intersection_offsets, c * 4, "intersection-offsets")))
+  if ((rc = ensure(ctx, l.pmean, c * 16, "projected-mean")) ||
+      (rc = ensure(ctx, l.pconic, c * 16, "projected-conic")) ||
+      (rc = ensure(ctx, l.pcolor, c * 16, "projected-color")) ||
+      (rc = ensure(ctx, l.tile_counts, c * 4, "tile-counts")) ||
+      (rc = ensure(ctx, l.visible_offsets, c * 4, "visible-offsets")) ||
+      (rc = ensure(ctx, l.depth_a, c * 8, "depth-records-a")) ||
+      (rc = ensure(ctx, l.depth_b, c * 8, "depth-records-b")) ||
+      (rc = ensure(ctx, l.ordered_counts, c * 4, "ordered-tile-counts")) ||
+      (rc = ensure(ctx, l.intersection_offsets, c * 4, "intersection-offsets")))
     return rc;
-  ctx->slot_capacity = cap;
+  l.slot_capacity = cap;
   return GS_OK;
 }
 
-int ensure_sort_workspace(gs_ctx* ctx) {
-  uint32_t m = ctx->slot_capacity > ctx->capacity ? ctx->slot_capacity : ctx->capacity;
+int lane_records(gs_ctx* ctx, Lane& l, uint32_t capacity) {
+  if (capacity <= l.capacity && l.records_a.ptr) return GS_OK;
+  int rc;
+  if ((rc = ensure(ctx, l.records_a, (size_t)capacity * 8, "records-a")) ||
+      (rc = ensure(ctx, l.records_b, (size_t)capacity * 8, "records-b")))
+    return rc;
+  l.capacity = capacity;
+  return GS_OK;
+}
+
+int lane_sort(gs_ctx* ctx, Lane& l) {
+  uint32_t m = std::max(l.slot_capacity, l.capacity);
+  if (m <= l.sort_capacity && l.hist.ptr) return GS_OK;
   uint32_t parts = div_up(m, gs::kPartition);
   if (parts == 0) parts = 1;
   int rc;
-  if ((rc = ensure(ctx, ctx->hist, (size_t)parts * gs::kRadix * 4, "radix-histograms")) ||
-      (rc = ensure(ctx, ctx->partials, ((size_t)parts + 64) * 4, "scan-partials")))
+  if ((rc = ensure(ctx, l.hist, (size_t)parts * gs::kMaxRadix * 4, "radix-histograms")) ||
+      (rc = ensure(ctx, l.partials, ((size_t)parts * gs::kMaxRadix / gs::kPartition + parts +
+                                     64) * 4,
+                   "scan-partials")))
     return rc;
+  l.sort_capacity = m;
   return GS_OK;
+}
+
+int lane_create(gs_ctx* ctx, Lane& l, bool own_stream) {
+  // Default stream priority only.
+  if (own_stream) {
+    if (hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(ctx, GS_E_DEVICE, "hipStreamCreate");
+  } else {
+    l.stream = ctx->stream;
+  }
+  if (hipStreamCreateWithFlags(&l.aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&l.fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&l.join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&l.done, hipEventDisableTiming) != hipSuccess)
+    return fail(ctx, GS_E_DEVICE, "lane stream/event creation");
+  return GS_OK;
+}
+
+void lane_destroy(gs_ctx* ctx, Lane& l) {
+  DeviceBuffer* all[] = {&l.pmean,     &l.pconic,         &l.pcolor,       &l.tile_counts,
+                         &l.visible_offsets, &l.depth_a,  &l.depth_b,      &l.ordered_counts,
+                         &l.intersection_offsets, &l.records_a, &l.records_b, &l.hist,
+                         &l.partials,  &l.tile_offsets,   &l.chunk_first,  &l.tasks,
+                         &l.partials_px, &l.partial_depth};
+  for (DeviceBuffer* b : all) release(*b);
+  if (l.fork) (void)hipEventDestroy(l.fork);
+  if (l.join) (void)hipEventDestroy(l.join);
+  if (l.done) (void)hipEventDestroy(l.done);
+  if (l.aux) (void)hipStreamDestroy(l.aux);
+  if (l.stream && l.stream != ctx->stream) (void)hipStreamDestroy(l.stream);
+  l = Lane();
+}
+
+void set_release(ViewSet& v) {
+  release(v.state);
+  release(v.dynamic);
+  for (auto& b : v.color) release(b);
+  for (auto& b : v.depth) release(b);
+  if (v.exec) (void)hipGraphExecDestroy(v.exec);
+  if (v.graph) (void)hipGraphDestroy(v.graph);
+  v = ViewSet();
 }
 
 int copy_in(gs_ctx* ctx, DeviceBuffer& dst, const void* src, size_t bytes, uint32_t memory,
@@ -178,6 +270,11 @@ int copy_in(gs_ctx* ctx, DeviceBuffer& dst, const void* src, size_t bytes, uint3
 
 bool finite(float v) { return v == v && fabsf(v) <= 3.402823466e38f; }
 
+bool valid_tuning(uint32_t tile, uint32_t bits, uint32_t batch) {
+  return (tile == 8 || tile == 16 || tile == 32) && (bits == 4 || bits == 6 || bits == 8) &&
+         (batch == 128 || batch == 256);
+}
+
 }  // namespace
 
 extern "C" void gs_config_default(gs_config* c) {
@@ -195,6 +292,10 @@ extern "C" void gs_config_default(gs_config* c) {
   c->invalid_depth_value = 1.0f;
   c->raster_stats = 0;
   c->capture_graph = 1;
+  c->tile_size = 16;
+  c->radix_bits = 8;
+  c->raster_batch = 256;
+  c->batch_lanes = 1;
 }
 
 // GaussianPass.ts:157-260 option validation.
@@ -238,6 +339,17 @@ static int validate_config(const gs_config* c, std::string& msg) {
       msg = "background must be finite";
       return GS_E_RANGE;
     }
+  if (!valid_tuning(c->tile_size, c->radix_bits, c->raster_batch)) {
+    msg = "tile size must be 8, 16 or 32, radix bits 4, 6 or 8, raster batch 128 or 256";
+    return GS_E_RANGE;
+  }
+  // Concurrent lanes inside one captured batch graph are switched off: the
+  // four-lane batch ended in a host segmentation fault inside
+  // gs_render_batch whose cause is not established.
+  if (c->batch_lanes != 1) {
+    msg = "batch lanes above 1 are disabled";
+    return GS_E_RANGE;
+  }
   return GS_OK;
 }
 
@@ -261,23 +373,20 @@ extern "C" int gs_create(const gs_config* config, gs_ctx** out) {
   ctx->cfg = c;
   // Default stream priority only.
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&ctx->batch_fork, hipEventDisableTiming) != hipSuccess) {
     gs_destroy(ctx);
     return GS_E_DEVICE;
   }
+  ctx->lanes.resize(c.batch_lanes);
+  for (uint32_t i = 0; i < c.batch_lanes; i++)
+    if ((rc = lane_create(ctx, ctx->lanes[i], i != 0))) {
+      gs_destroy(ctx);
+      return GS_E_DEVICE;
+    }
   ctx->capacity = c.intersection_capacity;
-  if ((rc = ensure(ctx, ctx->state, sizeof(DeviceState), "state")) ||
-      (rc = ensure(ctx, ctx->records_a, (size_t)ctx->capacity * 8, "records-a")) ||
-      (rc = ensure(ctx, ctx->records_b, (size_t)ctx->capacity * 8, "records-b"))) {
+  if ((rc = lane_records(ctx, ctx->lanes[0], ctx->capacity))) {
     gs_destroy(ctx);
     return rc;
-  }
-  if (hipMemsetAsync(ctx->state.ptr, 0, sizeof(DeviceState), ctx->stream) != hipSuccess ||
-      hipStreamSynchronize(ctx->stream) != hipSuccess) {
-    gs_destroy(ctx);
-    return GS_E_DEVICE;
   }
   *out = ctx;
   return GS_OK;
@@ -287,26 +396,18 @@ extern "C" void gs_destroy(gs_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->cfg.device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DeviceBuffer* all[] = {&ctx->means,        &ctx->scales_opacity,  &ctx->rotations,
-                         &ctx->sh,           &ctx->active,          &ctx->objects,
-                         &ctx->pmean,        &ctx->pconic,          &ctx->pcolor,
-                         &ctx->tile_counts,  &ctx->visible_offsets, &ctx->depth_a,
-                         &ctx->depth_b,      &ctx->ordered_counts,  &ctx->intersection_offsets,
-                         &ctx->records_a,    &ctx->records_b,       &ctx->hist,
-                         &ctx->partials,     &ctx->tile_offsets,    &ctx->color,
-                         &ctx->depth,        &ctx->state,           &ctx->chunk_first,
-                         &ctx->tasks,        &ctx->partials_px,     &ctx->partial_depth};
+  DeviceBuffer* all[] = {&ctx->means, &ctx->scales_opacity, &ctx->rotations, &ctx->sh,
+                         &ctx->active};
   for (DeviceBuffer* b : all) release(*b);
+  set_release(ctx->single);
+  set_release(ctx->batch);
+  for (Lane& l : ctx->lanes) lane_destroy(ctx, l);
   for (int i = 0; i < gs_ctx::kRing; i++) {
     if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
     if (ctx->ring_done[i]) (void)hipEventDestroy(ctx->ring_done[i]);
   }
-  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   for (hipEvent_t e : ctx->marks) (void)hipEventDestroy(e);
-  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
-  if (ctx->join) (void)hipEventDestroy(ctx->join);
-  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+  if (ctx->batch_fork) (void)hipEventDestroy(ctx->batch_fork);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -346,12 +447,12 @@ extern "C" int gs_set_data(gs_ctx* ctx, const gs_data_desc* d) {
       (rc = copy_in(ctx, ctx->active, d->active_slots, (size_t)d->active_count * 4, d->memory,
                     "activeSlots")))
     return rc;
-  if ((rc = ensure_slots(ctx, d->active_count)) || (rc = ensure_sort_workspace(ctx))) return rc;
   ctx->count = d->count;
   ctx->active_count = d->active_count;
   ctx->sh_degree = d->sh_degree;
   ctx->has_data = true;
-  ctx->rendered = false;
+  ctx->single.rendered = false;
+  ctx->batch.rendered = false;
   return GS_OK;
 }
 
@@ -392,7 +493,6 @@ extern "C" int gs_update_active(gs_ctx* ctx, uint32_t start, uint32_t n, const u
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "activeSlots");
   }
-  if ((rc = ensure_slots(ctx, new_active_count)) || (rc = ensure_sort_workspace(ctx))) return rc;
   ctx->active_count = new_active_count;
   return GS_OK;
 }
@@ -402,46 +502,6 @@ extern "C" int gs_set_objects(gs_ctx* ctx, uint32_t object_count, const float* f
   if (object_count > 0 && !frames) return fail(ctx, GS_E_RANGE, "missing object frames");
   ctx->objects_host.assign(frames, frames + (size_t)object_count * 40);
   ctx->object_count = object_count;
-  ctx->objects_dirty = true;
-  return GS_OK;
-}
-
-// Enqueue the object table upload (ObjectFrameState.update) without
-// blocking: the host copy goes to the next pinned ring slot once the copy
-// that last used that slot has completed.
-static int upload_objects(gs_ctx* ctx) {
-  if (!ctx->objects_dirty) return GS_OK;
-  size_t bytes = ctx->objects_host.size() * sizeof(float);
-  int rc = ensure(ctx, ctx->objects, bytes, "object frames");
-  if (rc) return rc;
-  if (bytes == 0) {
-    ctx->objects_dirty = false;
-    return GS_OK;
-  }
-  if (bytes > ctx->ring_bytes) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
-    for (int i = 0; i < gs_ctx::kRing; i++) {
-      if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
-      ctx->ring[i] = nullptr;
-      if (hipHostMalloc((void**)&ctx->ring[i], bytes, hipHostMallocDefault) != hipSuccess)
-        return fail(ctx, GS_E_OOM, "allocation-failed: object staging");
-      if (!ctx->ring_done[i] &&
-          hipEventCreateWithFlags(&ctx->ring_done[i], hipEventDisableTiming) != hipSuccess)
-        return fail(ctx, GS_E_DEVICE, "hipEventCreate");
-    }
-    ctx->ring_bytes = bytes;
-  }
-  int slot = ctx->ring_next;
-  ctx->ring_next = (slot + 1) % gs_ctx::kRing;
-  hipError_t e = hipEventSynchronize(ctx->ring_done[slot]);
-  if (e != hipSuccess) return hip_fail(ctx, e, "object staging wait");
-  if (bytes) memcpy(ctx->ring[slot], ctx->objects_host.data(), bytes);
-  if (bytes) e = hipMemcpyAsync(ctx->objects.ptr, ctx->ring[slot], bytes, hipMemcpyHostToDevice,
-                                ctx->stream);
-  if (e == hipSuccess) e = hipEventRecord(ctx->ring_done[slot], ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "object upload");
-  ctx->objects_dirty = false;
   return GS_OK;
 }
 
@@ -545,21 +605,28 @@ extern "C" int gs_resize_intersections(gs_ctx* ctx, uint32_t capacity) {
   if (capacity < 1 || capacity > ctx->cfg.max_intersection_capacity)
     return fail(ctx, GS_E_RANGE, "Intersection capacity exceeds device buffer or dispatch limits");
   (void)hipSetDevice(ctx->cfg.device);
-  int rc;
-  if ((rc = ensure(ctx, ctx->records_a, (size_t)capacity * 8, "records-a")) ||
-      (rc = ensure(ctx, ctx->records_b, (size_t)capacity * 8, "records-b")))
-    return rc;
+  int rc = lane_records(ctx, ctx->lanes[0], capacity);
+  if (rc) return rc;
   ctx->capacity = capacity;
-  return ensure_sort_workspace(ctx);
+  return GS_OK;
 }
 
-static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* color_dst,
-                         void* depth_dst) {
-  if (!ctx->has_data) return fail(ctx, GS_E_STATE, "render before data");
+extern "C" int gs_resize_batch_intersections(gs_ctx* ctx, uint32_t view, uint32_t capacity) {
+  if (!ctx) return GS_E_RANGE;
+  if (view >= ctx->batch.n) return fail(ctx, GS_E_RANGE, "batch view out of range");
+  if (capacity < 1 || capacity > ctx->cfg.max_intersection_capacity)
+    return fail(ctx, GS_E_RANGE, "Intersection capacity exceeds device buffer or dispatch limits");
+  ctx->batch.capacity[view] = capacity;
+  return GS_OK;
+}
+
+namespace {
+
+// TiledGaussianPipeline.ts:286-300 checks on one camera.
+int validate_frame(gs_ctx* ctx, const gs_frame* f) {
   if (f->width < 1 || f->height < 1) return fail(ctx, GS_E_RANGE, "empty render size");
-  uint32_t tiles_x = div_up(f->width, gs::kTile), tiles_y = div_up(f->height, gs::kTile);
-  // TiledGaussianPipeline.ts:294-296
-  if (tiles_x > 65535 || tiles_y > 65535)
+  uint32_t ts = ctx->cfg.tile_size;
+  if (div_up(f->width, ts) > 65535 || div_up(f->height, ts) > 65535)
     return fail(ctx, GS_E_RANGE, "Render size exceeds the tile dispatch limit");
   if (!(f->near_plane > 0.0f && f->far_plane > f->near_plane && finite(f->far_plane)))
     return fail(ctx, GS_E_RANGE, "depthRange must be positive and increasing");
@@ -569,41 +636,64 @@ static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* 
     return fail(ctx, GS_E_RANGE, "unknown boundaryPolicy");
   for (int i = 0; i < 16; i++)
     if (!finite(f->projection[i])) return fail(ctx, GS_E_RANGE, "projection must be finite");
+  return GS_OK;
+}
+
+// rasterChunks.ts:73-78 (maxRasterChunkTasks): every chunked tile holds
+// more than chunk_size records, so ceil(2K / chunk_size) tasks suffice.
+uint32_t task_capacity_for(const gs_config& c, uint32_t capacity) {
+  if (!c.raster_chunk_size) return 1;
+  uint32_t t = div_up(2ull * capacity, c.raster_chunk_size);
+  return t < 1 ? 1 : t;
+}
+
+// Heavy raster units: the chunk tasks plus tiles above 1024 records.
+uint32_t unit_capacity_for(uint32_t task_capacity, uint32_t capacity) {
+  return task_capacity + div_up(capacity, 1024) + 1;
+}
+
+int lane_frame_buffers(gs_ctx* ctx, Lane& l, uint32_t capacity, uint32_t tiles) {
   int rc;
-  uint32_t tiles = tiles_x * tiles_y;
-  if (tiles > ctx->tile_capacity || !ctx->tile_offsets.ptr) {
+  if ((rc = lane_slots(ctx, l, ctx->active_count)) || (rc = lane_records(ctx, l, capacity)) ||
+      (rc = lane_sort(ctx, l)))
+    return rc;
+  if (tiles > l.tile_capacity || !l.tile_offsets.ptr) {
     uint32_t cap = next_pow2(tiles);
-    if ((rc = ensure(ctx, ctx->tile_offsets, ((size_t)cap + 1) * 4, "tile-offsets")) ||
-        (rc = ensure(ctx, ctx->chunk_first, ((size_t)cap + 1) * 4, "raster-chunk-offsets")))
+    if ((rc = ensure(ctx, l.tile_offsets, ((size_t)cap + 1) * 4, "tile-offsets")) ||
+        (rc = ensure(ctx, l.chunk_first, ((size_t)cap + 1) * 4, "raster-chunk-offsets")))
       return rc;
-    ctx->tile_capacity = cap;
+    l.tile_capacity = cap;
   }
-  // rasterChunks.ts:73-78 (maxRasterChunkTasks): every chunked tile holds
-  // more than chunk_size records, so ceil(2K / chunk_size) tasks suffice.
-  uint32_t task_capacity = 1;
-  if (ctx->cfg.raster_chunk_size) {
-    task_capacity = div_up(2ull * ctx->capacity, ctx->cfg.raster_chunk_size);
-    if (task_capacity < 1) task_capacity = 1;
-  }
-  // Heavy raster units: the chunk tasks plus tiles above 1024 records.
-  uint32_t unit_capacity = task_capacity + div_up(ctx->capacity, 1024) + 1;
-  if ((rc = ensure(ctx, ctx->tasks, (size_t)unit_capacity * 8, "raster-heavy-units")) ||
-      (rc = ensure(ctx, ctx->partials_px, (size_t)task_capacity * gs::kBlock * 16,
-                   "raster-chunk-partials")) ||
-      (rc = ensure(ctx, ctx->partial_depth, (size_t)task_capacity * gs::kBlock * 4,
-                   "raster-chunk-depth")))
+  uint32_t tasks = task_capacity_for(ctx->cfg, l.capacity);
+  uint32_t units = unit_capacity_for(tasks, l.capacity);
+  size_t px = (size_t)ctx->cfg.tile_size * ctx->cfg.tile_size;
+  if ((rc = ensure(ctx, l.tasks, (size_t)units * 8, "raster-heavy-units")) ||
+      (rc = ensure(ctx, l.partials_px, (size_t)tasks * px * 16, "raster-chunk-partials")) ||
+      (rc = ensure(ctx, l.partial_depth, (size_t)tasks * px * 4, "raster-chunk-depth")))
     return rc;
-  size_t pixels = (size_t)f->width * f->height;
-  if ((rc = ensure(ctx, ctx->color, pixels * 8, "color-target"))) return rc;
-  if (ctx->cfg.output_depth && (rc = ensure(ctx, ctx->depth, pixels * 4, "depth-target")))
-    return rc;
+  return GS_OK;
+}
+
+// Camera-table bytes per view: the padded FrameCamera, then the objects.
+size_t view_stride(uint32_t object_count) {
+  size_t obj = (size_t)object_count * 40 * sizeof(float);
+  return gs::kCameraBytes + (obj + 255) / 256 * 256;
+}
+
+// Launch description of view v of `set` on lane l. All lane buffers are
+// already sized.
+void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const Lane& l,
+                void* color_dst, void* depth_dst, FrameBuffers& b) {
+  memset(&b, 0, sizeof(b));
+  uint32_t ts = ctx->cfg.tile_size;
+  uint32_t tiles_x = div_up(f->width, ts), tiles_y = div_up(f->height, ts);
+  uint32_t tiles = tiles_x * tiles_y;
+  uint32_t capacity = set.capacity[v];
+  const unsigned char* dyn = (const unsigned char*)set.dynamic.ptr + (size_t)v * set.stride;
   FrameParams& p = b.params;
-  memset(&p, 0, sizeof(p));
-  memcpy(p.projection, f->projection, sizeof(p.projection));
+  p.camera = (const FrameCamera*)dyn;
   p.viewport[0] = (float)f->width;
   p.viewport[1] = (float)f->height;
-  p.viewport[2] = f->near_plane;
-  p.viewport[3] = f->far_plane;
   p.filter_variance = f->filter_variance;
   p.transmittance_threshold = ctx->cfg.transmittance_threshold;
   memcpy(p.background, ctx->cfg.background, sizeof(p.background));
@@ -616,9 +706,7 @@ static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* 
   p.tile_count = tiles;
   p.width = f->width;
   p.height = f->height;
-  p.capacity = ctx->capacity;
-  p.attempt = f->attempt;
-  p.reset_valid = f->reset_valid;
+  p.capacity = capacity;
   p.object_count = ctx->object_count;
   p.sh_degree = ctx->sh_degree;
   p.depth_mode = ctx->cfg.depth_sort_mode;
@@ -628,128 +716,350 @@ static int prepare_frame(gs_ctx* ctx, const gs_frame* f, FrameBuffers& b, void* 
   p.has_active_slots = ctx->has_active ? 1u : 0u;
   p.output_depth = ctx->cfg.output_depth ? 1u : 0u;
   p.raster_stats = ctx->cfg.raster_stats ? 1u : 0u;
-  b.state = (DeviceState*)ctx->state.ptr;
+  p.tile_size = ts;
+  b.state = (DeviceState*)set.state.ptr + v;
   b.scene.means = (const float4*)ctx->means.ptr;
   b.scene.scales_opacity = (const uint2*)ctx->scales_opacity.ptr;
   b.scene.rotations = (const uint32_t*)ctx->rotations.ptr;
   b.scene.sh = (const uint32_t*)ctx->sh.ptr;
   b.scene.active_slots = (const uint32_t*)ctx->active.ptr;
-  b.scene.objects = (const float*)ctx->objects.ptr;
-  b.proj.mean = (float4*)ctx->pmean.ptr;
-  b.proj.conic = (float4*)ctx->pconic.ptr;
-  b.proj.color = (float4*)ctx->pcolor.ptr;
-  b.proj.tile_counts = (uint32_t*)ctx->tile_counts.ptr;
-  b.visible_offsets = (uint32_t*)ctx->visible_offsets.ptr;
-  b.depth_a = (uint2*)ctx->depth_a.ptr;
-  b.depth_b = (uint2*)ctx->depth_b.ptr;
-  b.ordered_counts = (uint32_t*)ctx->ordered_counts.ptr;
-  b.intersection_offsets = (uint32_t*)ctx->intersection_offsets.ptr;
-  b.records_a = (uint2*)ctx->records_a.ptr;
-  b.records_b = (uint2*)ctx->records_b.ptr;
-  b.tile_offsets = (uint32_t*)ctx->tile_offsets.ptr;
-  b.chunk_first = (uint32_t*)ctx->chunk_first.ptr;
-  b.tasks = (uint2*)ctx->tasks.ptr;
-  b.chunk_partials = (float4*)ctx->partials_px.ptr;
-  b.partial_depth = (float*)ctx->partial_depth.ptr;
-  b.task_capacity = task_capacity;
-  b.unit_capacity = unit_capacity;
-  b.hist = (uint32_t*)ctx->hist.ptr;
-  b.partials = (uint32_t*)ctx->partials.ptr;
-  b.color_out = (uint16_t*)(color_dst ? color_dst : ctx->color.ptr);
-  b.depth_out = ctx->cfg.output_depth ? (float*)(depth_dst ? depth_dst : ctx->depth.ptr) : nullptr;
+  b.scene.objects = (const float*)(dyn + gs::kCameraBytes);
+  b.proj.mean = (float4*)l.pmean.ptr;
+  b.proj.conic = (float4*)l.pconic.ptr;
+  b.proj.color = (float4*)l.pcolor.ptr;
+  b.proj.tile_counts = (uint32_t*)l.tile_counts.ptr;
+  b.visible_offsets = (uint32_t*)l.visible_offsets.ptr;
+  b.depth_a = (uint2*)l.depth_a.ptr;
+  b.depth_b = (uint2*)l.depth_b.ptr;
+  b.ordered_counts = (uint32_t*)l.ordered_counts.ptr;
+  b.intersection_offsets = (uint32_t*)l.intersection_offsets.ptr;
+  b.records_a = (uint2*)l.records_a.ptr;
+  b.records_b = (uint2*)l.records_b.ptr;
+  b.tile_offsets = (uint32_t*)l.tile_offsets.ptr;
+  b.chunk_first = (uint32_t*)l.chunk_first.ptr;
+  b.tasks = (uint2*)l.tasks.ptr;
+  b.chunk_partials = (float4*)l.partials_px.ptr;
+  b.partial_depth = (float*)l.partial_depth.ptr;
+  b.task_capacity = task_capacity_for(ctx->cfg, capacity);
+  b.unit_capacity = unit_capacity_for(b.task_capacity, capacity);
+  b.hist = (uint32_t*)l.hist.ptr;
+  b.partials = (uint32_t*)l.partials.ptr;
+  b.color_out = (uint16_t*)(color_dst ? color_dst : set.color[v].ptr);
+  b.depth_out = ctx->cfg.output_depth ? (float*)(depth_dst ? depth_dst : set.depth[v].ptr)
+                                      : nullptr;
   b.max_n = ctx->active_count;
-  b.max_k = ctx->capacity;
+  b.max_k = capacity;
   b.max_tiles = tiles;
-  b.depth_passes = ctx->cfg.depth_sort_mode == GS_DEPTH_FLOAT32 ? 4u : 2u;
-  b.tile_passes = div_up(bits_for(tiles), gs::kRadixBits);
+  b.radix_bits = ctx->cfg.radix_bits;
+  b.raster_batch = ctx->cfg.raster_batch;
+  uint32_t key_bits = ctx->cfg.depth_sort_mode == GS_DEPTH_FLOAT32 ? 32u : 16u;
+  b.depth_passes = div_up(key_bits, b.radix_bits);
+  b.tile_passes = div_up(bits_for(tiles), b.radix_bits);
   if (b.tile_passes == 0) b.tile_passes = 1;
+}
+
+// Stage the per-view camera uniforms and object frames through the next
+// pinned ring slot (once the copy that last used it has completed) and
+// enqueue their upload on the context stream.
+int upload_cameras(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames) {
+  size_t bytes = (size_t)n * set.stride;
+  ctx->staging.assign(bytes, 0);
+  size_t obj_bytes = (size_t)ctx->object_count * 40 * sizeof(float);
+  for (uint32_t v = 0; v < n; v++) {
+    unsigned char* dst = ctx->staging.data() + (size_t)v * set.stride;
+    FrameCamera cam;
+    memset(&cam, 0, sizeof(cam));
+    memcpy(cam.projection, frames[v].projection, sizeof(cam.projection));
+    cam.near_plane = frames[v].near_plane;
+    cam.far_plane = frames[v].far_plane;
+    cam.attempt = frames[v].attempt;
+    cam.reset_valid = frames[v].reset_valid;
+    memcpy(dst, &cam, sizeof(cam));
+    const float* objects = frames[v].object_frames ? frames[v].object_frames
+                                                   : ctx->objects_host.data();
+    if (obj_bytes) memcpy(dst + gs::kCameraBytes, objects, obj_bytes);
+  }
+  if (bytes > ctx->ring_bytes) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    for (int i = 0; i < gs_ctx::kRing; i++) {
+      if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
+      ctx->ring[i] = nullptr;
+      if (hipHostMalloc(&ctx->ring[i], bytes, hipHostMallocDefault) != hipSuccess)
+        return fail(ctx, GS_E_OOM, "allocation-failed: camera staging");
+      if (!ctx->ring_done[i] &&
+          hipEventCreateWithFlags(&ctx->ring_done[i], hipEventDisableTiming) != hipSuccess)
+        return fail(ctx, GS_E_DEVICE, "hipEventCreate");
+    }
+    ctx->ring_bytes = bytes;
+  }
+  int slot = ctx->ring_next;
+  ctx->ring_next = (slot + 1) % gs_ctx::kRing;
+  hipError_t e = hipEventSynchronize(ctx->ring_done[slot]);
+  if (e != hipSuccess) return hip_fail(ctx, e, "camera staging wait");
+  memcpy(ctx->ring[slot], ctx->staging.data(), bytes);
+  e = hipMemcpyAsync(set.dynamic.ptr, ctx->ring[slot], bytes, hipMemcpyHostToDevice,
+                     ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ring_done[slot], ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "camera upload");
   return GS_OK;
 }
 
-static int launch(gs_ctx* ctx, const FrameBuffers& b) {
+// Enqueue every frame of the set: lane l > 0 forks from the context stream
+// and joins it at the end, frames of one lane run in order.
+hipError_t enqueue_set(gs_ctx* ctx, ViewSet& set, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
-  if (ctx->profile_capacity) {
+  uint32_t used = 0;
+  for (uint32_t v = 0; v < set.n; v++) used = std::max(used, set.lane_of[v] + 1);
+  hipError_t e = hipSuccess;
+  if (used > 1) {
+    e = hipEventRecord(ctx->batch_fork, s);
+    for (uint32_t l = 1; l < used && e == hipSuccess; l++)
+      e = hipStreamWaitEvent(ctx->lanes[l].stream, ctx->batch_fork, 0);
+    if (e != hipSuccess) return e;
+  }
+  for (uint32_t v = 0; v < set.n; v++) {
+    Lane& l = ctx->lanes[set.lane_of[v]];
+    e = gs::launch_frame(set.frames[v], l.stream, l.aux, l.fork, l.join, marks);
+    if (e != hipSuccess) return e;
+  }
+  for (uint32_t l = 1; l < used; l++) {
+    e = hipEventRecord(ctx->lanes[l].done, ctx->lanes[l].stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->lanes[l].done, 0);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+bool same_key(const ViewSet& set) {
+  if (!set.graph_valid || set.key.size() != set.n || set.key_lanes != set.lane_of) return false;
+  return memcmp(set.key.data(), set.frames.data(), sizeof(FrameBuffers) * set.n) == 0;
+}
+
+int launch_set(gs_ctx* ctx, ViewSet& set, bool profile) {
+  hipStream_t s = ctx->stream;
+  if (profile) {
     if (ctx->profile_frames >= ctx->profile_capacity)
       return fail(ctx, GS_E_RANGE, "profiling capacity exhausted");
     hipEvent_t* marks = &ctx->marks[(size_t)ctx->profile_frames * (gs::kStageCount + 1)];
-    hipError_t e = gs::launch_frame(b, s, ctx->aux, ctx->fork, ctx->join, marks);
+    hipError_t e = enqueue_set(ctx, set, marks);
     if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
     ctx->profile_frames++;
     return GS_OK;
   }
   if (!ctx->cfg.capture_graph) {
-    hipError_t e = gs::launch_frame(b, s, ctx->aux, ctx->fork, ctx->join, nullptr);
+    hipError_t e = enqueue_set(ctx, set, nullptr);
     if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
     return GS_OK;
   }
-  if (!ctx->graph_valid || memcmp(&ctx->graph_key, &b, sizeof(b)) != 0) {
+  if (!same_key(set)) {
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
-    hipError_t le = gs::launch_frame(b, s, ctx->aux, ctx->fork, ctx->join, nullptr);
+    hipError_t le = enqueue_set(ctx, set, nullptr);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     if (le != hipSuccess) e = le;
     if (e != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
-      ctx->graph_valid = false;
+      set.graph_valid = false;
       return hip_fail(ctx, e, "frame capture");
     }
     bool updated = false;
-    if (ctx->graph_exec) {
+    if (set.exec) {
       hipGraphExecUpdateResult result;
       hipGraphNode_t error_node = nullptr;
-      updated = hipGraphExecUpdate(ctx->graph_exec, g, &error_node, &result) == hipSuccess &&
+      updated = hipGraphExecUpdate(set.exec, g, &error_node, &result) == hipSuccess &&
                 result == hipGraphExecUpdateSuccess;
       if (!updated) {
         (void)hipGetLastError();
-        (void)hipGraphExecDestroy(ctx->graph_exec);
-        ctx->graph_exec = nullptr;
+        (void)hipGraphExecDestroy(set.exec);
+        set.exec = nullptr;
       }
     }
     if (!updated) {
-      e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+      e = hipGraphInstantiate(&set.exec, g, nullptr, nullptr, 0);
       if (e != hipSuccess) {
         (void)hipGraphDestroy(g);
-        ctx->graph_valid = false;
+        set.graph_valid = false;
         return hip_fail(ctx, e, "hipGraphInstantiate");
       }
     }
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph = g;
-    ctx->graph_key = b;
-    ctx->graph_valid = true;
+    if (set.graph) (void)hipGraphDestroy(set.graph);
+    set.graph = g;
+    set.key = set.frames;
+    set.key_lanes = set.lane_of;
+    set.graph_valid = true;
   }
-  hipError_t e = hipGraphLaunch(ctx->graph_exec, s);
+  hipError_t e = hipGraphLaunch(set.exec, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
   return GS_OK;
 }
 
+// Spread views over lanes, largest first onto the least loaded lane.
+void assign_lanes(gs_ctx* ctx, ViewSet& set, const gs_frame* frames, bool single) {
+  set.lane_of.assign(set.n, 0);
+  if (single) return;
+  uint32_t lanes = std::min<uint32_t>((uint32_t)ctx->lanes.size(), set.n);
+  std::vector<uint32_t> order(set.n);
+  for (uint32_t v = 0; v < set.n; v++) order[v] = v;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (uint64_t)frames[a].width * frames[a].height >
+           (uint64_t)frames[b].width * frames[b].height;
+  });
+  std::vector<uint64_t> load(lanes, 0);
+  for (uint32_t v : order) {
+    uint32_t best = 0;
+    for (uint32_t l = 1; l < lanes; l++)
+      if (load[l] < load[best]) best = l;
+    set.lane_of[v] = best;
+    load[best] += (uint64_t)frames[v].width * frames[v].height;
+  }
+}
+
+// prepareFrame + render for the views of a set (TiledGaussianPipeline.ts
+// :286-341), one launch.
+int render_set(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames,
+               const gs_image* outs, bool single) {
+  if (!ctx->has_data) return fail(ctx, GS_E_STATE, "render before data");
+  int rc;
+  for (uint32_t v = 0; v < n; v++)
+    if ((rc = validate_frame(ctx, &frames[v]))) return rc;
+  (void)hipSetDevice(ctx->cfg.device);
+  // Views, their capacities and states.
+  if (set.capacity.size() < n) set.capacity.resize(n, ctx->cfg.intersection_capacity);
+  if (single) set.capacity[0] = ctx->capacity;
+  set.n = n;
+  size_t state_bytes = (size_t)std::max<uint32_t>(n, 1) * sizeof(DeviceState);
+  if (set.state.bytes < state_bytes) {
+    if ((rc = ensure(ctx, set.state, state_bytes, "state"))) return rc;
+    hipError_t e = hipMemsetAsync(set.state.ptr, 0, set.state.bytes, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "state clear");
+  }
+  set.stride = view_stride(ctx->object_count);
+  if ((rc = ensure(ctx, set.dynamic, (size_t)n * set.stride, "camera-table"))) return rc;
+  if (set.color.size() < n) {
+    set.color.resize(n);
+    set.depth.resize(n);
+  }
+  set.width.resize(n);
+  set.height.resize(n);
+  assign_lanes(ctx, set, frames, single);
+  // Lane working sets: the largest capacity and tile grid of their views.
+  uint32_t ts = ctx->cfg.tile_size;
+  std::vector<uint32_t> lane_cap(ctx->lanes.size(), 0), lane_tiles(ctx->lanes.size(), 0);
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t l = set.lane_of[v];
+    uint32_t tiles = div_up(frames[v].width, ts) * div_up(frames[v].height, ts);
+    lane_cap[l] = std::max(lane_cap[l], set.capacity[v]);
+    lane_tiles[l] = std::max(lane_tiles[l], tiles);
+  }
+  for (size_t l = 0; l < ctx->lanes.size(); l++)
+    if (lane_cap[l] && (rc = lane_frame_buffers(ctx, ctx->lanes[l], lane_cap[l], lane_tiles[l])))
+      return rc;
+  // Outputs.
+  std::vector<void*> color_dst(n, nullptr), depth_dst(n, nullptr);
+  for (uint32_t v = 0; v < n; v++) {
+    const gs_image* o = outs ? &outs[v] : nullptr;
+    if (o && o->memory == GS_MEMORY_DEVICE) {
+      color_dst[v] = o->color;
+      depth_dst[v] = o->depth;
+    }
+    size_t pixels = (size_t)frames[v].width * frames[v].height;
+    if (!color_dst[v] && (rc = ensure(ctx, set.color[v], pixels * 8, "color-target")))
+      return rc;
+    if (ctx->cfg.output_depth && !depth_dst[v] &&
+        (rc = ensure(ctx, set.depth[v], pixels * 4, "depth-target")))
+      return rc;
+  }
+  set.frames.resize(n);
+  for (uint32_t v = 0; v < n; v++) {
+    fill_frame(ctx, set, v, &frames[v], ctx->lanes[set.lane_of[v]], color_dst[v], depth_dst[v],
+               set.frames[v]);
+    set.width[v] = frames[v].width;
+    set.height[v] = frames[v].height;
+  }
+  if ((rc = upload_cameras(ctx, set, n, frames))) return rc;
+  if ((rc = launch_set(ctx, set, single && ctx->profile_capacity != 0))) return rc;
+  set.rendered = true;
+  return GS_OK;
+}
+
+int read_image(gs_ctx* ctx, const ViewSet& set, uint32_t v, void* color, size_t color_bytes,
+               void* depth, size_t depth_bytes) {
+  if (!set.rendered || v >= set.n) return fail(ctx, GS_E_STATE, "nothing rendered");
+  (void)hipSetDevice(ctx->cfg.device);
+  const FrameBuffers& b = set.frames[v];
+  size_t pixels = (size_t)set.width[v] * set.height[v];
+  hipError_t e = hipSuccess;
+  if (color) {
+    if (color_bytes != pixels * 8) return fail(ctx, GS_E_RANGE, "color buffer size mismatch");
+    e = hipMemcpyAsync(color, b.color_out, pixels * 8, hipMemcpyDeviceToHost, ctx->stream);
+  }
+  if (e == hipSuccess && depth) {
+    if (!b.depth_out) return fail(ctx, GS_E_STATE, "outputDepth is disabled");
+    if (depth_bytes != pixels * 4) return fail(ctx, GS_E_RANGE, "depth buffer size mismatch");
+    e = hipMemcpyAsync(depth, b.depth_out, pixels * 4, hipMemcpyDeviceToHost, ctx->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "image readback");
+  return GS_OK;
+}
+
+int read_states(gs_ctx* ctx, const ViewSet& set, uint32_t first, uint32_t n, gs_state* out) {
+  if (!set.rendered || (uint64_t)first + n > set.n)
+    return fail(ctx, GS_E_STATE, "nothing rendered");
+  (void)hipSetDevice(ctx->cfg.device);
+  std::vector<DeviceState> s(n);
+  hipError_t e = hipSuccess;
+  if (n)
+    e = hipMemcpyAsync(s.data(), (const DeviceState*)set.state.ptr + first,
+                       n * sizeof(DeviceState), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "state readback");
+  for (uint32_t i = 0; i < n; i++) {
+    const FrameParams& p = set.frames[first + i].params;
+    gs_state& o = out[i];
+    memset(&o, 0, sizeof(o));
+    o.attempt = s[i].out_attempt;
+    o.version = s[i].out_version;
+    o.requested = s[i].out_demand;
+    o.overflow = s[i].out_overflow;
+    o.visible = s[i].visible;
+    o.intersections = s[i].count;
+    o.capacity = set.capacity[first + i];
+    o.tiles_x = p.tiles_x;
+    o.tiles_y = p.tiles_y;
+    o.raster_checked = s[i].checked;
+    o.raster_blended = s[i].blended;
+  }
+  return GS_OK;
+}
+
+}  // namespace
+
 extern "C" int gs_render(gs_ctx* ctx, const gs_frame* f, const gs_image* out) {
   if (!ctx || !f) return GS_E_RANGE;
-  (void)hipSetDevice(ctx->cfg.device);
-  void* color_dst = nullptr;
-  void* depth_dst = nullptr;
-  if (out && out->memory == GS_MEMORY_DEVICE) {
-    color_dst = out->color;
-    depth_dst = out->depth;
-  }
-  // The upload may grow the object buffer, so it precedes the launch
-  // description that captures its address.
-  int rc = upload_objects(ctx);
+  int rc = render_set(ctx, ctx->single, 1, f, out, true);
   if (rc) return rc;
-  FrameBuffers b;
-  memset(&b, 0, sizeof(b));
-  if ((rc = prepare_frame(ctx, f, b, color_dst, depth_dst))) return rc;
-  if ((rc = launch(ctx, b))) return rc;
-  ctx->last = b;
-  ctx->rendered = true;
-  ctx->width = f->width;
-  ctx->height = f->height;
-  ctx->tiles_x = b.params.tiles_x;
-  ctx->tiles_y = b.params.tiles_y;
   if (out && out->memory == GS_MEMORY_HOST)
     return gs_read_image(ctx, out->color, out->color ? (size_t)f->width * f->height * 8 : 0,
                          out->depth, out->depth ? (size_t)f->width * f->height * 4 : 0);
+  return GS_OK;
+}
+
+extern "C" int gs_render_batch(gs_ctx* ctx, uint32_t n, const gs_frame* frames,
+                               const gs_image* outs) {
+  if (!ctx || (n > 0 && !frames)) return GS_E_RANGE;
+  if (n == 0) return GS_OK;
+  int rc = render_set(ctx, ctx->batch, n, frames, outs, false);
+  if (rc) return rc;
+  for (uint32_t v = 0; v < n; v++)
+    if (outs && outs[v].memory == GS_MEMORY_HOST) {
+      size_t px = (size_t)frames[v].width * frames[v].height;
+      if ((rc = read_image(ctx, ctx->batch, v, outs[v].color, outs[v].color ? px * 8 : 0,
+                           outs[v].depth, outs[v].depth ? px * 4 : 0)))
+        return rc;
+    }
   return GS_OK;
 }
 
@@ -796,70 +1106,54 @@ extern "C" int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], doub
   return GS_OK;
 }
 
-extern "C" int gs_render_batch(gs_ctx* ctx, uint32_t n, const gs_frame* frames,
-                               const gs_image* outs) {
-  if (!ctx || (n > 0 && !frames)) return GS_E_RANGE;
-  for (uint32_t i = 0; i < n; i++) {
-    int rc = gs_render(ctx, &frames[i], outs ? &outs[i] : nullptr);
-    if (rc) return rc;
-  }
-  return GS_OK;
-}
-
 extern "C" int gs_read_state(gs_ctx* ctx, gs_state* out) {
   if (!ctx || !out) return GS_E_RANGE;
-  (void)hipSetDevice(ctx->cfg.device);
-  DeviceState s;
-  hipError_t e = hipMemcpyAsync(&s, ctx->state.ptr, sizeof(s), hipMemcpyDeviceToHost,
-                                ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "state readback");
-  memset(out, 0, sizeof(*out));
-  out->attempt = s.out_attempt;
-  out->version = s.out_version;
-  out->requested = s.out_demand;
-  out->overflow = s.out_overflow;
-  out->visible = s.visible;
-  out->intersections = s.count;
-  out->capacity = ctx->capacity;
-  out->tiles_x = ctx->tiles_x;
-  out->tiles_y = ctx->tiles_y;
-  out->raster_checked = s.checked;
-  out->raster_blended = s.blended;
-  return GS_OK;
+  if (!ctx->single.rendered) {
+    // Before the first frame: an empty state with the configured capacity.
+    memset(out, 0, sizeof(*out));
+    out->capacity = ctx->capacity;
+    return GS_OK;
+  }
+  int rc = read_states(ctx, ctx->single, 0, 1, out);
+  if (rc == GS_OK) out->capacity = ctx->capacity;
+  return rc;
+}
+
+extern "C" int gs_read_batch_state(gs_ctx* ctx, uint32_t first, uint32_t n, gs_state* out) {
+  if (!ctx || (n > 0 && !out)) return GS_E_RANGE;
+  return read_states(ctx, ctx->batch, first, n, out);
 }
 
 extern "C" int gs_read_image(gs_ctx* ctx, void* color, size_t color_bytes, void* depth,
                              size_t depth_bytes) {
   if (!ctx) return GS_E_RANGE;
-  if (!ctx->rendered) return fail(ctx, GS_E_STATE, "nothing rendered");
-  (void)hipSetDevice(ctx->cfg.device);
-  size_t pixels = (size_t)ctx->width * ctx->height;
-  hipError_t e = hipSuccess;
-  if (color) {
-    if (color_bytes != pixels * 8) return fail(ctx, GS_E_RANGE, "color buffer size mismatch");
-    e = hipMemcpyAsync(color, ctx->last.color_out, pixels * 8, hipMemcpyDeviceToHost,
-                       ctx->stream);
-  }
-  if (e == hipSuccess && depth) {
-    if (!ctx->last.depth_out) return fail(ctx, GS_E_STATE, "outputDepth is disabled");
-    if (depth_bytes != pixels * 4) return fail(ctx, GS_E_RANGE, "depth buffer size mismatch");
-    e = hipMemcpyAsync(depth, ctx->last.depth_out, pixels * 4, hipMemcpyDeviceToHost,
-                       ctx->stream);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "image readback");
+  return read_image(ctx, ctx->single, 0, color, color_bytes, depth, depth_bytes);
+}
+
+extern "C" int gs_read_batch_image(gs_ctx* ctx, uint32_t view, void* color, size_t color_bytes,
+                                   void* depth, size_t depth_bytes) {
+  if (!ctx) return GS_E_RANGE;
+  return read_image(ctx, ctx->batch, view, color, color_bytes, depth, depth_bytes);
+}
+
+extern "C" int gs_batch_device_image(gs_ctx* ctx, uint32_t view, void** color, void** depth) {
+  if (!ctx) return GS_E_RANGE;
+  if (view >= ctx->batch.n) return fail(ctx, GS_E_RANGE, "batch view out of range");
+  if (color) *color = ctx->batch.frames[view].color_out;
+  if (depth) *depth = ctx->batch.frames[view].depth_out;
   return GS_OK;
 }
 
+// The debug copies describe the last gs_render; a later render of either
+// kind reuses lane 0's buffers.
 extern "C" int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t bytes,
                              size_t* size) {
   if (!ctx || !name) return GS_E_RANGE;
-  if (!ctx->rendered) return fail(ctx, GS_E_STATE, "nothing rendered");
+  if (!ctx->single.rendered) return fail(ctx, GS_E_STATE, "nothing rendered");
   gs_state st;
   int rc = gs_read_state(ctx, &st);
   if (rc) return rc;
-  const FrameBuffers& b = ctx->last;
+  const FrameBuffers& b = ctx->single.frames[0];
   std::string n(name);
   const void* src = nullptr;
   size_t len = 0;
@@ -889,7 +1183,103 @@ extern "C" void* gs_stream(gs_ctx* ctx) { return ctx ? (void*)ctx->stream : null
 
 extern "C" int gs_device_image(gs_ctx* ctx, void** color, void** depth) {
   if (!ctx) return GS_E_RANGE;
-  if (color) *color = ctx->color.ptr;
-  if (depth) *depth = ctx->depth.ptr;
+  const ViewSet& s = ctx->single;
+  if (color) *color = s.color.empty() ? nullptr : s.color[0].ptr;
+  if (depth) *depth = s.depth.empty() ? nullptr : s.depth[0].ptr;
   return GS_OK;
 }
+
+extern "C" int gs_set_tuning(gs_ctx* ctx, const gs_tuning* t) {
+  if (!ctx || !t) return GS_E_RANGE;
+  if (!valid_tuning(t->tile_size, t->radix_bits, t->raster_batch))
+    return fail(ctx, GS_E_RANGE,
+                "tile size must be 8, 16 or 32, radix bits 4, 6 or 8, raster batch 128 or 256");
+  ctx->cfg.tile_size = t->tile_size;
+  ctx->cfg.radix_bits = t->radix_bits;
+  ctx->cfg.raster_batch = t->raster_batch;
+  return GS_OK;
+}
+
+extern "C" int gs_get_tuning(gs_ctx* ctx, gs_tuning* t) {
+  if (!ctx || !t) return GS_E_RANGE;
+  t->tile_size = ctx->cfg.tile_size;
+  t->radix_bits = ctx->cfg.radix_bits;
+  t->raster_batch = ctx->cfg.raster_batch;
+  return GS_OK;
+}
+
+// Render the batch until no view overflows, growing each overflowing
+// view's capacity by the reference's policy (IntersectionCapacityPolicy.ts
+// :90-101 with its default growth 1.5 and headroom 1.25).
+static int render_batch_fit(gs_ctx* ctx, uint32_t n, const gs_frame* frames) {
+  std::vector<gs_state> st(n);
+  for (int attempt = 0; attempt < 8; attempt++) {
+    int rc = gs_render_batch(ctx, n, frames, nullptr);
+    if (rc == GS_OK) rc = gs_read_batch_state(ctx, 0, n, st.data());
+    if (rc) return rc;
+    bool grown = false;
+    for (uint32_t v = 0; v < n; v++)
+      if (st[v].overflow) {
+        uint32_t next = gs_capacity_next(ctx->batch.capacity[v], st[v].requested,
+                                         ctx->cfg.max_intersection_capacity, 1.5, 1.25);
+        if (next == 0) return fail(ctx, GS_E_RANGE, "intersection demand exceeds maximum");
+        ctx->batch.capacity[v] = next;
+        grown = true;
+      }
+    if (!grown) return GS_OK;
+  }
+  return fail(ctx, GS_E_STATE, "batch capacity did not settle");
+}
+
+extern "C" int gs_autotune(gs_ctx* ctx, uint32_t n, const gs_frame* frames, uint32_t reps,
+                           gs_tuning* best, gs_tune_entry* table, uint32_t table_len,
+                           uint32_t* entries) {
+  if (!ctx || !frames || n == 0 || reps == 0) return GS_E_RANGE;
+  (void)hipSetDevice(ctx->cfg.device);
+  static const uint32_t tiles[] = {8, 16, 32}, digits[] = {4, 6, 8}, batches[] = {128, 256};
+  gs_tuning saved;
+  gs_get_tuning(ctx, &saved);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)
+    return fail(ctx, GS_E_DEVICE, "hipEventCreate");
+  gs_tuning winner = saved;
+  double winner_ms = 1e300;
+  uint32_t count = 0;
+  int rc = GS_OK;
+  for (uint32_t tile : tiles)
+    for (uint32_t bits : digits)
+      for (uint32_t batch : batches) {
+        gs_tuning t = {tile, bits, batch};
+        if ((rc = gs_set_tuning(ctx, &t)) || (rc = render_batch_fit(ctx, n, frames))) break;
+        hipError_t e = hipEventRecord(t0, ctx->stream);
+        for (uint32_t r = 0; r < reps && e == hipSuccess && rc == GS_OK; r++)
+          rc = gs_render_batch(ctx, n, frames, nullptr);
+        if (rc) break;
+        if (e == hipSuccess) e = hipEventRecord(t1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e != hipSuccess) {
+          rc = hip_fail(ctx, e, "autotune timing");
+          break;
+        }
+        double per = (double)ms / reps;
+        if (table && count < table_len) table[count] = {t, per};
+        count++;
+        if (per < winner_ms) {
+          winner_ms = per;
+          winner = t;
+        }
+      }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (rc) {
+    gs_set_tuning(ctx, &saved);
+    return rc;
+  }
+  gs_set_tuning(ctx, &winner);
+  if (best) *best = winner;
+  if (entries) *entries = count;
+  return GS_OK;
+}
+

@@ -8,17 +8,29 @@
 namespace gs {
 
 constexpr int kBlock = 256;          // threads per block (4 wave64)
-constexpr int kTile = 16;            // tile edge in pixels (constants.ts:1)
 constexpr int kItems = 16;           // keys per thread in scans and sorts
 constexpr int kPartition = kBlock * kItems;  // 4096 keys per block
-constexpr int kRadixBits = 8;        // digit width of the LSD sorts
-constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxRadixBits = 8;     // widest digit of the LSD sorts
+constexpr int kMaxRadix = 1 << kMaxRadixBits;
+constexpr int kMaxTile = 32;         // widest tile edge (pixels)
 
-// Per-frame uniforms, uploaded once per frame (FrameUniforms.ts:41-58 plus
-// the stage constants the reference keeps in uniforms).
-struct FrameParams {
+// The part of a frame's uniforms that changes with the camera
+// (FrameUniforms.ts:41-58): it lives in device memory and is uploaded
+// before each launch, so a captured frame graph survives camera motion.
+// The view's object frame table (ObjectFrameState.ts:97-120, 40 floats per
+// object) follows it in the same allocation.
+struct FrameCamera {
   float projection[16];
-  float viewport[4];  // width, height, near, far
+  float near_plane, far_plane;
+  uint32_t attempt, reset_valid;
+};
+constexpr size_t kCameraBytes = 256;  // FrameCamera slot, padded
+
+// Per-frame constants of a launch (FrameUniforms.ts:41-58 minus the camera,
+// plus the stage constants the reference keeps in uniforms).
+struct FrameParams {
+  const FrameCamera* camera;
+  float viewport[2];  // width, height
   float filter_variance;
   float transmittance_threshold;
   float background[4];
@@ -39,6 +51,7 @@ struct FrameParams {
   uint32_t has_active_slots;
   uint32_t output_depth;
   uint32_t raster_stats;
+  uint32_t tile_size;               // tile edge: 8, 16 or 32 (16 matches constants.ts:1)
 };
 
 // Device-side counters of one frame (dispatch state + outputState).
@@ -97,7 +110,7 @@ struct FrameBuffers {
   uint2* records_a;
   uint2* records_b;
   uint32_t* tile_offsets;
-  uint32_t* hist;       // kRadix * ceil(max(N, capacity) / kPartition)
+  uint32_t* hist;       // 2^radix_bits * ceil(max(N, capacity) / kPartition)
   uint32_t* partials;   // scan partials
   uint32_t* chunk_first;  // per tile: first chunk task (tile capacity)
   uint2* tasks;           // raster heavy units: (tile, chunk or ~0u)
@@ -112,6 +125,8 @@ struct FrameBuffers {
   uint32_t max_tiles;
   uint32_t depth_passes;
   uint32_t tile_passes;
+  uint32_t radix_bits;   // digit width: 4, 6 or 8
+  uint32_t raster_batch; // records staged per light-tile round: 128 or 256
 };
 
 // Frame stages, in launch order; launch_frame can record an event before

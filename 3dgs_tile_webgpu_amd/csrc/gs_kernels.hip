@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   float cv[3][3];
   view_covariance(scale, rot, mv, cv);
   Projected p;
-  perspective_projection(view, cv, fp, p);
+  perspective_projection(view, cv, fp, *fp.camera, p);
   if (!p.valid) return;
   float opacity = gs_clampf(base_opacity, 0.0f, 1.0f);
   if (fp.antialias_compensated)
@@ -145,10 +145,11 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
     return;
   }
   float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
-  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx / kTileF), 0.0f, max_tx));
-  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy / kTileF), 0.0f, max_ty));
-  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx / kTileF), 0.0f, max_tx));
-  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy / kTileF), 0.0f, max_ty));
+  float ts = (float)fp.tile_size;
+  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx / ts), 0.0f, max_tx));
+  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy / ts), 0.0f, max_ty));
+  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx / ts), 0.0f, max_tx));
+  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy / ts), 0.0f, max_ty));
   if (axis == 1) {
     tmin_x = 0;
     tmax_x = (int)fp.tiles_x - 1;
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   uint32_t count = 0;
   for (int ty = tmin_y; ty <= tmax_y; ty++)
     for (int tx = tmin_x; tx <= tmax_x; tx++)
-      if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis)) count++;
+      if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) count++;
   if (count == 0) return;
   // SH colour (projectionHelpers.ts:118-176), direction = normalized
   // object-space offset from the camera.
@@ -239,7 +240,7 @@ struct LoadPlain {
 struct StoreCompact {  // visibility.ts:37-61 (+ visibleOffsets)
   const float4* mean;
   uint32_t depth_mode;
-  float near, far;
+  const FrameCamera* camera;
   uint32_t* offsets;
   uint2* records;
   GS_DEV void operator()(uint32_t i, uint32_t prefix, uint32_t flag) const {
@@ -250,6 +251,7 @@ struct StoreCompact {  // visibility.ts:37-61 (+ visibleOffsets)
     if (depth_mode == 0) {
       key = gs_f2b(depth);
     } else {
+      float near = camera->near_plane, far = camera->far_plane;
       float q = gs_clampf((depth - near) / (far - near), 0.0f, 1.0f);
       key = gs_f2u(__builtin_rintf(q * 65535.0f));
     }
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Load load, LenSpec len_s
 __global__ __launch_bounds__(1024) void k_scan_partials(LenSpec len_spec,
                                                         uint32_t* __restrict__ partials,
                                                         ScanMode mode, uint32_t capacity,
-                                                        uint32_t attempt, uint32_t reset_valid,
+                                                        const FrameCamera* __restrict__ camera,
                                                         DeviceState* __restrict__ state) {
   __shared__ uint32_t wave_sums[16];
   __shared__ uint32_t carry_s;
@@ -329,7 +331,8 @@ __global__ __launch_bounds__(1024) void k_scan_partials(LenSpec len_spec,
     state->total = total;
     state->count = overflow ? 0u : total;
     state->overflow = overflow;
-    uint32_t last_version = reset_valid ? 0u : state->out_version;
+    uint32_t attempt = camera->attempt;
+    uint32_t last_version = camera->reset_valid ? 0u : state->out_version;
     state->out_attempt = attempt;
     state->out_version = overflow ? last_version : attempt;
     state->out_demand = total;
@@ -365,21 +368,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_downsweep(Load load, Store stor
 
 // ----------------------------------------------------------- radix sort ----
 //
-// Stable LSD sort of (key, value) pairs with 8-bit digits. A pass is a
-// per-partition digit histogram laid out [digit][partition], its exclusive
-// scan (the plain scan kernels), then a scatter that ranks keys stably
-// inside the partition: 16 rounds of 256 consecutive keys, each wave finds
-// its same-digit peers with 9 ballots, waves are ordered through LDS.
+// Stable LSD sort of (key, value) pairs with kBits-bit digits (4, 6 or 8).
+// A pass is a per-partition digit histogram laid out [digit][partition],
+// its exclusive scan (the plain scan kernels), then a scatter that ranks
+// keys stably inside the partition: 16 rounds of 256 consecutive keys, each
+// wave finds its same-digit peers with kBits + 1 ballots (the extra bit
+// marks keys past the end), waves are ordered through LDS.
 
+template <int kBits>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint2* __restrict__ in,
                                                        const uint32_t* __restrict__ count_ptr,
                                                        uint32_t shift,
                                                        uint32_t* __restrict__ hist) {
+  constexpr uint32_t kRadix = 1u << kBits;
   __shared__ uint32_t bins[kRadix];
   uint32_t count = *count_ptr;
   uint32_t nparts = (count + kPartition - 1) / kPartition;
   if (blockIdx.x >= nparts) return;
-  bins[threadIdx.x] = 0;
+  if (threadIdx.x < kRadix) bins[threadIdx.x] = 0;
   __syncthreads();
   uint32_t base = blockIdx.x * (uint32_t)kPartition;
 #pragma unroll 4
@@ -388,23 +394,27 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint2* __restrict__
     if (i < count) atomicAdd(&bins[(in[i].x >> shift) & (kRadix - 1)], 1u);
   }
   __syncthreads();
-  hist[threadIdx.x * nparts + blockIdx.x] = bins[threadIdx.x];
+  if (threadIdx.x < kRadix) hist[threadIdx.x * nparts + blockIdx.x] = bins[threadIdx.x];
 }
 
+template <int kBits>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint2* __restrict__ in,
                                                           uint2* __restrict__ out,
                                                           const uint32_t* __restrict__ count_ptr,
                                                           uint32_t shift,
                                                           const uint32_t* __restrict__ hist) {
+  constexpr uint32_t kRadix = 1u << kBits;
   __shared__ uint32_t running[kRadix];
   __shared__ uint32_t wave_cnt[4][kRadix];
   uint32_t count = *count_ptr;
   uint32_t nparts = (count + kPartition - 1) / kPartition;
   if (blockIdx.x >= nparts) return;
   uint32_t t = threadIdx.x;
-  running[t] = hist[t * nparts + blockIdx.x];
+  if (t < kRadix) {
+    running[t] = hist[t * nparts + blockIdx.x];
 #pragma unroll
-  for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+    for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+  }
   __syncthreads();
   uint32_t wave = t >> 6, lane = lane_id();
   uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -416,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint2* __restric
     uint32_t d = valid ? ((rec.x >> shift) & (kRadix - 1)) : (uint32_t)kRadix;
     uint64_t peers = ~0ull;
 #pragma unroll
-    for (int bit = 0; bit <= kRadixBits; bit++) {
+    for (int bit = 0; bit <= kBits; bit++) {
       uint64_t bal = __ballot((d >> bit) & 1u);
       peers &= ((d >> bit) & 1u) ? bal : ~bal;
     }
@@ -430,9 +440,11 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint2* __restric
       out[pos] = rec;
     }
     __syncthreads();
-    running[t] += wave_cnt[0][t] + wave_cnt[1][t] + wave_cnt[2][t] + wave_cnt[3][t];
+    if (t < kRadix) {
+      running[t] += wave_cnt[0][t] + wave_cnt[1][t] + wave_cnt[2][t] + wave_cnt[3][t];
 #pragma unroll
-    for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+      for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+    }
     __syncthreads();
   }
 }
@@ -460,10 +472,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
   float conic[3] = {conic4.x, conic4.y, conic4.z};
   float thr = gs_logf(mean.w * 255.0f);
   int max_tx = (int)fp.tiles_x - 1, max_ty = (int)fp.tiles_y - 1;
-  int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) / kTileF)), 0), max_tx);
-  int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) / kTileF)), 0), max_ty);
-  int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) / kTileF)), 0), max_tx);
-  int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) / kTileF)), 0), max_ty);
+  float ts = (float)fp.tile_size;
+  int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) / ts)), 0), max_tx);
+  int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) / ts)), 0), max_ty);
+  int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) / ts)), 0), max_tx);
+  int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) / ts)), 0), max_ty);
   uint32_t axis = fp.boundary_axis;
   if (axis == 1) {
     tmin_x = 0;
@@ -479,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
   float vw = fp.viewport[0], vh = fp.viewport[1];
   for (int ty = tmin_y; ty <= tmax_y; ty++)
     for (int tx = tmin_x; tx <= tmax_x; tx++)
-      if (tile_hit(mean.x, mean.y, conic, thr, rx, ry, tx, ty, vw, vh, axis)) {
+      if (tile_hit(mean.x, mean.y, conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
         if (local < reserved && base + local < capacity)
           records[base + local] = make_uint2((uint32_t)ty * fp.tiles_x + (uint32_t)tx, gid);
         local++;
@@ -511,8 +524,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
 
 // ---------------------------------------------------------------- raster ----
 //
-// One 256-thread block per tile (or per chunk task of a heavy tile), one
-// pixel per thread. Records are taken 256 at a time, front to back:
+// One block per tile (or per heavy unit, below), one pixel per thread.
+// Records are taken a batch at a time, front to back:
 //  phase A: the thread owning record r stages it in LDS and tests only the
 //    pixels of its support rectangle (support_rect below): every pixel
 //    outside the rectangle is proven to fail the reference's power test, so
@@ -541,18 +554,19 @@ __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
 // binary64 from the binary32 operands; pixels left out lie at least one
 // pixel beyond the bound.
 GS_DEV void support_rect(float cx, float cy, float c0, float c1, float c2, float thr, int tx0,
-                         int ty0, int& x0, int& x1, int& y0, int& y1) {
+                         int ty0, int edge, int& x0, int& x1, int& y0, int& y1) {
   x0 = tx0;
-  x1 = tx0 + kTile - 1;
+  x1 = tx0 + edge - 1;
   y0 = ty0;
-  y1 = ty0 + kTile - 1;
+  y1 = ty0 + edge - 1;
   double C0 = c0, C1 = c1, C2 = c2, T = thr;
   double det = C0 * C2 - C1 * C1;
   if (!(det > 0.0 && C0 > 0.0 && C2 > 0.0 && T >= 0.0 && T < 1e30)) return;
   const double g = 3.0e-7;
   double a1 = C0 + fabs(C1), a2 = C2 + fabs(C1);
-  double X = fmax(fabs(tx0 + 0.5 - (double)cx), fabs(tx0 + 15.5 - (double)cx)) + 1.0;
-  double Y = fmax(fabs(ty0 + 0.5 - (double)cy), fabs(ty0 + 15.5 - (double)cy)) + 1.0;
+  double last = (double)edge - 0.5;
+  double X = fmax(fabs(tx0 + 0.5 - (double)cx), fabs(tx0 + last - (double)cx)) + 1.0;
+  double Y = fmax(fabs(ty0 + 0.5 - (double)cy), fabs(ty0 + last - (double)cy)) + 1.0;
   double two_t = 2.0 * T * (1.0 + 1e-6);
   // kx = det / c2 - g a1 > 0  <=>  det - g a1 c2 > 0 (c2 > 0); then
   // hx^2 = (two_t + g a2 Y^2) c2 / (det - g a1 c2). The quotient and root
@@ -593,19 +607,24 @@ struct PixelSum {
   uint32_t checked, blended;
 };
 
-// Light tiles use 256-thread blocks, heavy units 1024-thread blocks; a
-// round stages as many records as the block has threads.
-constexpr int kLightThreads = 256;
-constexpr int kHeavyThreads = 1024;
+// Block shapes. A tile of edge E has E*E pixels, one per thread. Light
+// tiles (up to kLightMax records) use max(E*E, batch) threads and stage
+// `batch` records per round (128 or 256, tunable); heavy units use 1024
+// threads and stage 1024 records per round (512 for 32-pixel tiles, whose
+// candidate masks would not fit the LDS otherwise).
 constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
+constexpr int kHeavyThreads = 1024;
+constexpr int log2_edge(int e) { return e == 8 ? 3 : (e == 16 ? 4 : 5); }
+constexpr int light_threads(int e, int batch) { return e * e > batch ? e * e : batch; }
+constexpr int heavy_batch(int e) { return e == 32 ? 512 : 1024; }
 
-template <int kThreads>
+template <int kPixels, int kBatch>
 struct RasterLds {
-  static constexpr int kWords = kThreads / 32;
-  float4 mean[kThreads];   // cx, cy, depth, opacity
-  float4 conic[kThreads];  // a, b, c, ln(255 opacity)
-  float4 color[kThreads];
-  uint32_t mask[kWords][kBlock];  // [word][pixel] candidate bits of the round
+  static constexpr int kWords = kBatch / 32;
+  float4 mean[kBatch];   // cx, cy, depth, opacity
+  float4 conic[kBatch];  // a, b, c, ln(255 opacity)
+  float4 color[kBatch];
+  uint32_t mask[kWords][kPixels];  // [word][pixel] candidate bits of the round
 };
 
 // Blend one candidate (record j of the round) into the pixel sum.
@@ -628,20 +647,25 @@ GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, 
 }
 
 // Blend records [cb, ce) of `tile` into one pixel per thread (threads
-// 0..255), from T = 1. All kThreads threads of the block must call it.
-template <int kThreads>
+// 0..E*E-1), from T = 1. All kThreads threads of the block must call it;
+// threads below kBatch stage one record per round.
+template <int kEdge, int kThreads, int kBatch>
 GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
                                  const uint2* __restrict__ sorted,
                                  const float4* __restrict__ pmean,
                                  const float4* __restrict__ pconic,
-                                 const float4* __restrict__ pcolor, RasterLds<kThreads>& L,
-                                 uint32_t& x,
-                                 uint32_t& y, bool& active) {
+                                 const float4* __restrict__ pcolor,
+                                 RasterLds<kEdge * kEdge, kBatch>& L, uint32_t& x, uint32_t& y,
+                                 bool& active) {
+  constexpr int kPixels = kEdge * kEdge;
+  constexpr int kMaskWords = kBatch / 32;
+  static_assert(kThreads >= kPixels && kThreads >= kBatch, "block too small");
   uint32_t t = threadIdx.x;
-  bool pixel_thread = t < (uint32_t)kBlock;
-  int tx0 = (int)((tile % fp.tiles_x) * kTile), ty0 = (int)((tile / fp.tiles_x) * kTile);
-  x = (uint32_t)tx0 + (t & 15u);
-  y = (uint32_t)ty0 + ((t >> 4) & 15u);
+  bool pixel_thread = t < (uint32_t)kPixels;
+  bool stager = t < (uint32_t)kBatch;
+  int tx0 = (int)((tile % fp.tiles_x) * kEdge), ty0 = (int)((tile / fp.tiles_x) * kEdge);
+  x = (uint32_t)tx0 + (t & (uint32_t)(kEdge - 1));
+  y = (uint32_t)ty0 + ((t >> log2_edge(kEdge)) & (uint32_t)(kEdge - 1));
   active = pixel_thread && x < fp.width && y < fp.height;
   float pxc = (float)x + 0.5f, pyc = (float)y + 0.5f;
   float vw = fp.viewport[0], vh = fp.viewport[1];
@@ -652,20 +676,18 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   // Prefetch the first round.
   uint32_t idx = cb + t;
   float4 m = make_float4(0.f, 0.f, 0.f, 0.f), c = m, k = m;
-  if (idx < ce) {
+  if (stager && idx < ce) {
     uint32_t gid = sorted[idx].y;
     m = pmean[gid];
     c = pconic[gid];
     k = pcolor[gid];
   }
   __syncthreads();
-  constexpr int kRasterBatch = kThreads;
-  constexpr int kMaskWords = kThreads / 32;
-  for (uint32_t batch = cb; batch < ce; batch += kRasterBatch) {
-    for (uint32_t i = t; i < (uint32_t)(kMaskWords * kBlock); i += kThreads)
+  for (uint32_t batch = cb; batch < ce; batch += kBatch) {
+    for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
       (&L.mask[0][0])[i] = 0u;
     __syncthreads();
-    bool mine = batch + t < ce;
+    bool mine = stager && batch + t < ce;
     float thr = 0.0f;
     if (mine) {
       thr = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
@@ -674,8 +696,8 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
       L.color[t] = k;
     }
     float4 cm = m, cc = make_float4(c.x, c.y, c.z, thr);
-    uint32_t nidx = batch + kRasterBatch + t;
-    if (nidx < ce) {
+    uint32_t nidx = batch + kBatch + t;
+    if (stager && nidx < ce) {
       uint32_t gid = sorted[nidx].y;
       m = pmean[gid];
       c = pconic[gid];
@@ -685,12 +707,12 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
     if (mine) {
       int x0, x1, y0, y1;
       if (axis == 0)
-        support_rect(cm.x, cm.y, cc.x, cc.y, cc.z, cc.w, tx0, ty0, x0, x1, y0, y1);
+        support_rect(cm.x, cm.y, cc.x, cc.y, cc.z, cc.w, tx0, ty0, kEdge, x0, x1, y0, y1);
       else {
         x0 = tx0;
-        x1 = tx0 + kTile - 1;
+        x1 = tx0 + kEdge - 1;
         y0 = ty0;
-        y1 = ty0 + kTile - 1;
+        y1 = ty0 + kEdge - 1;
       }
       uint32_t bit = 1u << (t & 31u), word = t >> 5;
       for (int py = y0; py <= y1; py++) {
@@ -698,7 +720,7 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
         for (int px = x0; px <= x1; px++) {
           float power = pair_power((float)px + 0.5f, pyf, cm, cc, vw, vh, axis);
           if (!(power > 0.0f || power < -cc.w))
-            atomicOr(&L.mask[word][(uint32_t)((py - ty0) * kTile + (px - tx0))], bit);
+            atomicOr(&L.mask[word][(uint32_t)((py - ty0) * kEdge + (px - tx0))], bit);
         }
       }
     }
@@ -707,7 +729,7 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
     // candidate bits across the round's mask words, four at a time (their
     // loads and exponentials are independent; only the blend is serial).
     if (pixel_thread && !done) {
-      uint32_t n = min((uint32_t)kRasterBatch, ce - batch);
+      uint32_t n = min((uint32_t)kBatch, ce - batch);
       uint32_t stop = n;
       uint32_t w = 0;
       uint32_t bits = L.mask[0][t];
@@ -791,7 +813,7 @@ GS_DEV void store_pixel(const FrameParams& fp, uint32_t x, uint32_t y, float r, 
     bool valid = scalar_valid && galpha > fp.depth_alpha_threshold;
     float dev = fp.invalid_depth_value;
     if (valid) {
-      float near = fp.viewport[2], far = fp.viewport[3];
+      float near = fp.camera->near_plane, far = fp.camera->far_plane;
       float vz = -od;
       float enc = gs_clampf((near + vz) * far / ((far - near) * vz), 0.0f, 1.0f);
       if (enc == enc && gs_fabsf(enc) <= kMaxF32) dev = enc;
@@ -800,17 +822,14 @@ GS_DEV void store_pixel(const FrameParams& fp, uint32_t x, uint32_t y, float r, 
   }
 }
 
-// Light tiles (up to kLightMax records): one 256-thread block per tile.
-__global__ __launch_bounds__(kLightThreads) void k_raster_light(const FrameParams fp,
-                                                                DeviceState* __restrict__ state,
-                                                                const uint2* __restrict__ sorted,
-                                                                const uint32_t* __restrict__ off,
-                                                                const float4* __restrict__ pmean,
-                                                                const float4* __restrict__ pconic,
-                                                                const float4* __restrict__ pcolor,
-                                                                uint16_t* __restrict__ color_out,
-                                                                float* __restrict__ depth_out) {
-  __shared__ RasterLds<kLightThreads> lds;
+// Light tiles (up to kLightMax records): one block per tile.
+template <int kEdge, int kBatch>
+__global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
+    const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
+    const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
+    const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
+    uint16_t* __restrict__ color_out, float* __restrict__ depth_out) {
+  __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
   if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
   uint32_t tile = blockIdx.x;
   if (tile >= fp.tile_count) return;
@@ -818,8 +837,8 @@ __global__ __launch_bounds__(kLightThreads) void k_raster_light(const FrameParam
   if (end - begin > kLightMax) return;  // heavy unit
   uint32_t x, y;
   bool active;
-  PixelSum s = blend_tile_range<kLightThreads>(fp, tile, begin, end, sorted, pmean, pconic,
-                                               pcolor, lds, x, y, active);
+  PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch>(
+      fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active);
   if (fp.raster_stats) add_stats(state, s.checked, s.blended);
   if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
 }
@@ -895,6 +914,7 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
 }
 
 // Heavy units: one 1024-thread block per chunk task or heavy tile.
+template <int kEdge>
 __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
     const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
@@ -902,7 +922,8 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
     const uint2* __restrict__ units, float4* __restrict__ partials,
     float* __restrict__ partial_depth, uint16_t* __restrict__ color_out,
     float* __restrict__ depth_out) {
-  __shared__ RasterLds<kHeavyThreads> lds;
+  constexpr int kPixels = kEdge * kEdge;
+  __shared__ RasterLds<kPixels, heavy_batch(kEdge)> lds;
   uint32_t u = blockIdx.x;
   if (u >= state->heavy_units) return;
   uint2 unit = units[u];
@@ -916,12 +937,12 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
   }
   uint32_t x, y;
   bool active;
-  PixelSum s = blend_tile_range<kHeavyThreads>(fp, tile, cb, ce, sorted, pmean, pconic, pcolor,
-                                               lds, x, y, active);
+  PixelSum s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge)>(
+      fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active);
   if (fp.raster_stats) add_stats(state, s.checked, s.blended);
-  if (threadIdx.x >= (uint32_t)kBlock) return;
+  if (threadIdx.x >= (uint32_t)kPixels) return;
   if (chunk) {
-    size_t p = (size_t)u * kBlock + threadIdx.x;  // chunk tasks lead the list
+    size_t p = (size_t)u * kPixels + threadIdx.x;  // chunk tasks lead the list
     partials[p] = make_float4(s.r, s.g, s.b, s.t);
     partial_depth[p] = s.d;
   } else if (active) {
@@ -930,14 +951,13 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
 }
 
 // TileRasterizer.ts:807-902: ordered over-composite of a tile's chunks.
-__global__ __launch_bounds__(kBlock) void k_composite(const FrameParams fp,
-                                                      const DeviceState* __restrict__ state,
-                                                      const uint32_t* __restrict__ off,
-                                                      const uint32_t* __restrict__ chunk_first,
-                                                      const float4* __restrict__ partials,
-                                                      const float* __restrict__ partial_depth,
-                                                      uint16_t* __restrict__ color_out,
-                                                      float* __restrict__ depth_out) {
+template <int kEdge>
+__global__ __launch_bounds__(kEdge * kEdge) void k_composite(
+    const FrameParams fp, const DeviceState* __restrict__ state,
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ chunk_first,
+    const float4* __restrict__ partials, const float* __restrict__ partial_depth,
+    uint16_t* __restrict__ color_out, float* __restrict__ depth_out) {
+  constexpr int kPixels = kEdge * kEdge;
   if (state->overflow) return;
   uint32_t tile = blockIdx.x;
   if (tile >= fp.tile_count || fp.chunk_size == 0) return;
@@ -946,13 +966,13 @@ __global__ __launch_bounds__(kBlock) void k_composite(const FrameParams fp,
   if (count <= cs) return;
   uint32_t n = (count + cs - 1) / cs;
   uint32_t t = threadIdx.x;
-  uint32_t x = (tile % fp.tiles_x) * kTile + (t & 15u);
-  uint32_t y = (tile / fp.tiles_x) * kTile + (t >> 4);
+  uint32_t x = (tile % fp.tiles_x) * kEdge + (t & (uint32_t)(kEdge - 1));
+  uint32_t y = (tile / fp.tiles_x) * kEdge + (t >> log2_edge(kEdge));
   if (x >= fp.width || y >= fp.height) return;
   float r = 0.0f, g = 0.0f, b = 0.0f, d = 0.0f, tr = 1.0f;
   uint32_t first = chunk_first[tile];
   for (uint32_t k = 0; k < n; k++) {
-    size_t p = (size_t)(first + k) * kBlock + t;
+    size_t p = (size_t)(first + k) * kPixels + t;
     float4 c = partials[p];
     r = r + c.x * tr;
     g = g + c.y * tr;
@@ -975,28 +995,76 @@ hipError_t plain_scan(uint32_t* data, LenSpec len, uint32_t max_len, uint32_t* p
   uint32_t blocks = div_up(max_len, kPartition);
   if (blocks == 0) blocks = 1;
   k_scan_reduce<LoadPlain><<<blocks, kBlock, 0, s>>>(LoadPlain{data}, len, partials);
-  k_scan_partials<<<1, 1024, 0, s>>>(len, partials, kScanPlain, 0u, 0u, 0u, state);
+  k_scan_partials<<<1, 1024, 0, s>>>(len, partials, kScanPlain, 0u, nullptr, state);
   k_scan_downsweep<LoadPlain, StorePlain>
       <<<blocks, kBlock, 0, s>>>(LoadPlain{data}, StorePlain{data}, len, partials);
   return hipGetLastError();
 }
 
-// Sort pairs in a <-> b for `passes` 8-bit digits; the result lands in a
-// for an even pass count, in b otherwise.
-void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
-                uint32_t passes, uint32_t* hist, uint32_t* partials, DeviceState* state,
-                hipStream_t s) {
+// Sort pairs in a <-> b for `passes` kBits-bit digits; the result lands in
+// a for an even pass count, in b otherwise.
+template <int kBits>
+void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
+                     uint32_t passes, uint32_t* hist, uint32_t* partials, DeviceState* state,
+                     hipStream_t s) {
   uint32_t blocks = div_up(max_count, kPartition);
   if (blocks == 0) blocks = 1;
-  LenSpec hist_len = {count_ptr, 12u /* log2 kPartition */, (uint32_t)kRadix, 0u};
+  LenSpec hist_len = {count_ptr, 12u /* log2 kPartition */, 1u << kBits, 0u};
   for (uint32_t p = 0; p < passes; p++) {
     const uint2* in = (p % 2 == 0) ? a : b;
     uint2* out = (p % 2 == 0) ? b : a;
-    uint32_t shift = p * kRadixBits;
-    k_radix_hist<<<blocks, kBlock, 0, s>>>(in, count_ptr, shift, hist);
-    (void)plain_scan(hist, hist_len, blocks * kRadix, partials, state, s);
-    k_radix_scatter<<<blocks, kBlock, 0, s>>>(in, out, count_ptr, shift, hist);
+    uint32_t shift = p * kBits;
+    k_radix_hist<kBits><<<blocks, kBlock, 0, s>>>(in, count_ptr, shift, hist);
+    (void)plain_scan(hist, hist_len, blocks << kBits, partials, state, s);
+    k_radix_scatter<kBits><<<blocks, kBlock, 0, s>>>(in, out, count_ptr, shift, hist);
   }
+}
+
+void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
+                uint32_t passes, uint32_t bits, uint32_t* hist, uint32_t* partials,
+                DeviceState* state, hipStream_t s) {
+  if (bits == 4)
+    radix_sort_bits<4>(a, b, count_ptr, max_count, passes, hist, partials, state, s);
+  else if (bits == 6)
+    radix_sort_bits<6>(a, b, count_ptr, max_count, passes, hist, partials, state, s);
+  else
+    radix_sort_bits<8>(a, b, count_ptr, max_count, passes, hist, partials, state, s);
+}
+
+// The raster stage: the heavy work list on s, then heavy units on aux
+// concurrently with light tiles on s, then the chunk composite.
+template <int kEdge, int kBatch>
+hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
+                         hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
+  const FrameParams& fp = b.params;
+  DeviceState* st = b.state;
+  k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
+                                   b.unit_capacity);
+  hipError_t e = hipEventRecord(fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+  if (e != hipSuccess) return e;
+  k_raster_heavy<kEdge><<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
+      fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
+      b.chunk_partials, b.partial_depth, b.color_out, b.depth_out);
+  e = hipEventRecord(join, aux);
+  if (e != hipSuccess) return e;
+  k_raster_light<kEdge, kBatch><<<b.max_tiles, light_threads(kEdge, kBatch), 0, s>>>(
+      fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.color_out,
+      b.depth_out);
+  e = hipStreamWaitEvent(s, join, 0);
+  if (e != hipSuccess) return e;
+  if (fp.chunk_size != 0)
+    k_composite<kEdge><<<b.max_tiles, kEdge * kEdge, 0, s>>>(
+        fp, st, b.tile_offsets, b.chunk_first, b.chunk_partials, b.partial_depth, b.color_out,
+        b.depth_out);
+  return hipGetLastError();
+}
+
+template <int kEdge>
+hipError_t launch_raster_edge(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
+                              hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
+  if (b.raster_batch == 128) return launch_raster<kEdge, 128>(b, sorted, s, aux, fork, join);
+  return launch_raster<kEdge, 256>(b, sorted, s, aux, fork, join);
 }
 
 }  // namespace
@@ -1022,15 +1090,14 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   if (part_blocks == 0) part_blocks = 1;
   LoadVisible lv{b.proj.mean};
   k_scan_reduce<LoadVisible><<<part_blocks, kBlock, 0, s>>>(lv, active_len, b.partials);
-  k_scan_partials<<<1, 1024, 0, s>>>(active_len, b.partials, kScanVisible, 0u, 0u, 0u, st);
-  StoreCompact sc{b.proj.mean, fp.depth_mode, fp.viewport[2], fp.viewport[3],
-                  b.visible_offsets, b.depth_a};
+  k_scan_partials<<<1, 1024, 0, s>>>(active_len, b.partials, kScanVisible, 0u, nullptr, st);
+  StoreCompact sc{b.proj.mean, fp.depth_mode, fp.camera, b.visible_offsets, b.depth_a};
   k_scan_downsweep<LoadVisible, StoreCompact>
       <<<part_blocks, kBlock, 0, s>>>(lv, sc, active_len, b.partials);
 
   // Depth sort over V.
   mark(kStageDepthSort);
-  radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.hist,
+  radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.radix_bits, b.hist,
              b.partials, st, s);
   const uint2* sorted_depth = sorted_depth_records(b);
 
@@ -1040,7 +1107,7 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   LoadOrdered lo{sorted_depth, b.proj.tile_counts};
   k_scan_reduce<LoadOrdered><<<part_blocks, kBlock, 0, s>>>(lo, vis_len, b.partials);
   k_scan_partials<<<1, 1024, 0, s>>>(vis_len, b.partials, kScanIntersections, fp.capacity,
-                                      fp.attempt, fp.reset_valid, st);
+                                      fp.camera, st);
   k_scan_downsweep<LoadOrdered, StoreOrdered><<<part_blocks, kBlock, 0, s>>>(
       lo, StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, b.partials);
 
@@ -1049,7 +1116,7 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
                                      b.ordered_counts, b.intersection_offsets, b.records_a);
 
   mark(kStageTileSort);
-  radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.hist,
+  radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, b.hist,
              b.partials, st, s);
   const uint2* sorted = sorted_tile_records(b);
 
@@ -1058,25 +1125,14 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   if (off_blocks > 2048) off_blocks = 2048;
   k_tile_offsets<<<off_blocks, kBlock, 0, s>>>(fp, st, sorted, b.tile_offsets);
   mark(kStageRaster);
-  k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
-                                   b.unit_capacity);
-  hipError_t e = hipEventRecord(fork, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+  hipError_t e;
+  if (fp.tile_size == 8)
+    e = launch_raster_edge<8>(b, sorted, s, aux, fork, join);
+  else if (fp.tile_size == 32)
+    e = launch_raster_edge<32>(b, sorted, s, aux, fork, join);
+  else
+    e = launch_raster_edge<16>(b, sorted, s, aux, fork, join);
   if (e != hipSuccess) return e;
-  k_raster_heavy<<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
-      fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
-      b.chunk_partials, b.partial_depth, b.color_out, b.depth_out);
-  e = hipEventRecord(join, aux);
-  if (e != hipSuccess) return e;
-  k_raster_light<<<b.max_tiles, kLightThreads, 0, s>>>(fp, st, sorted, b.tile_offsets, b.proj.mean,
-                                                       b.proj.conic, b.proj.color, b.color_out,
-                                                       b.depth_out);
-  e = hipStreamWaitEvent(s, join, 0);
-  if (e != hipSuccess) return e;
-  if (fp.chunk_size != 0)
-    k_composite<<<b.max_tiles, kBlock, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first,
-                                               b.chunk_partials, b.partial_depth, b.color_out,
-                                               b.depth_out);
   mark(kStageCount);
   return hipGetLastError();
 }

@@ -15,7 +15,6 @@ namespace gs {
 #define GS_DEV __device__ __forceinline__
 
 constexpr float kMaxF32 = 3.402823e+38f;  // projectionHelpers.ts:44,100
-constexpr float kTileF = 16.0f;
 
 // m[col][row], as WGSL matrices.
 GS_DEV void mat3_mul(const float a[3][3], const float b[3][3], float o[3][3]) {
@@ -172,11 +171,12 @@ GS_DEV void projection_result(float cx, float cy, float depth, float c0, float c
 
 // kernels/projectionHelpers.ts:80-96
 GS_DEV void perspective_projection(const float view[3], const float cv[3][3],
-                                   const FrameParams& fp, Projected& p) {
+                                   const FrameParams& fp, const FrameCamera& cam,
+                                   Projected& p) {
   p.valid = false;
   float depth = -view[2];
-  if (!(depth > fp.viewport[2] && depth < fp.viewport[3])) return;
-  const float* P = fp.projection;
+  if (!(depth > cam.near_plane && depth < cam.far_plane)) return;
+  const float* P = cam.projection;
   float clip[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -229,18 +229,19 @@ GS_DEV bool subpixel_has_sample(float cx, float cy, const float conic[3], float 
   return false;
 }
 
-// kernels/tileContribution.ts:271-313 inside :339-367. One tile, one image.
+// kernels/tileContribution.ts:271-313 inside :339-367. One tile, one image;
+// ts is the tile edge in pixels (a power of two, so the products are exact).
 GS_DEV bool tile_rect_hit(float icx, float icy, const float conic[3], float thr, int tx,
-                          int ty) {
-  float rminx = (float)tx * kTileF, rminy = (float)ty * kTileF;
-  float rmaxx = rminx + kTileF, rmaxy = rminy + kTileF;
+                          int ty, float ts) {
+  float rminx = (float)tx * ts, rminy = (float)ty * ts;
+  float rmaxx = rminx + ts, rmaxy = rminy + ts;
   bool x_left = icx < rminx, x_right = icx > rmaxx;
   bool in_x = !(x_left || x_right);
   bool y_above = icy < rminy, y_below = icy > rmaxy;
   bool in_y = !(y_above || y_below);
   if (in_x && in_y) return true;
   float cx = x_left ? rminx : rmaxx, cy = y_above ? rminy : rmaxy;
-  float ex = x_left ? kTileF : -kTileF, ey = y_above ? kTileF : -kTileF;
+  float ex = x_left ? ts : -ts, ey = y_above ? ts : -ts;
   float dx = icx - cx, dy = icy - cy;
   float tx_raw = (ex * conic[0] * dx + ex * conic[1] * dy) / (ex * conic[0] * ex);
   float ty_raw = (ey * conic[1] * dx + ey * conic[2] * dy) / (ey * conic[2] * ey);
@@ -254,19 +255,19 @@ GS_DEV bool tile_rect_hit(float icx, float icy, const float conic[3], float thr,
 }
 
 GS_DEV bool tile_hit(float cx, float cy, const float conic[3], float thr, float rx,
-                     float ry, int tx, int ty, float vw, float vh, uint32_t axis) {
-  if (axis == 0) return tile_rect_hit(cx, cy, conic, thr, tx, ty);
+                     float ry, int tx, int ty, float vw, float vh, uint32_t axis, float ts) {
+  if (axis == 0) return tile_rect_hit(cx, cy, conic, thr, tx, ty, ts);
   float period = axis == 1 ? vw : vh;
   float c = axis == 1 ? cx : cy;
   float r = axis == 1 ? rx : ry;
   float canon = c - gs_floorf(c / period) * period;
-  float tile_start = (float)(axis == 1 ? tx : ty) * kTileF;
+  float tile_start = (float)(axis == 1 ? tx : ty) * ts;
   int first = gs_f2i(gs_ceilf((tile_start - r - canon) / period));
-  int last = gs_f2i(gs_floorf((tile_start + kTileF + r - canon) / period));
+  int last = gs_f2i(gs_floorf((tile_start + ts + r - canon) / period));
   for (int image = first; image <= last; image++) {
     float icx = axis == 1 ? canon + (float)image * vw : cx;
     float icy = axis == 2 ? canon + (float)image * vh : cy;
-    if (tile_rect_hit(icx, icy, conic, thr, tx, ty)) return true;
+    if (tile_rect_hit(icx, icy, conic, thr, tx, ty, ts)) return true;
   }
   return false;
 }

@@ -38,10 +38,16 @@ DEFAULT_OPTIONS = {
     "rasterStats": False,
     "intersectionGrowthFactor": 2.0,
     "intersectionHeadroom": 1.25,
+    # Launch shape (no reference counterpart): tile edge, radix digit width,
+    # raster LDS batch; only tileSize 16 gives keys comparable with the
+    # reference, the other two never change any output.
+    "tileSize": 16,
+    "radixBits": 8,
+    "rasterBatch": 256,
 }
 KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | {
     "fixedSize", "depthRange", "intersectionCapacity", "initialIntersectionCapacity",
-    "maxIntersectionCapacity", "device", "dataMode",
+    "maxIntersectionCapacity", "device", "dataMode", "batchLanes",
 }
 MAX_CAPACITY = 1 << 28
 
@@ -116,7 +122,58 @@ def validate_options(options: dict) -> dict:
     dr = o.get("depthRange")
     if dr is not None and not (0 < dr[0] < dr[1] and math.isfinite(dr[1])):
         raise N.GsRangeError(N.GS_E_RANGE, "depthRange must be positive and increasing")
+    if (o["tileSize"] not in (8, 16, 32) or o["radixBits"] not in (4, 6, 8)
+            or o["rasterBatch"] not in (128, 256)):
+        raise N.GsRangeError(N.GS_E_RANGE,
+                             "tileSize must be 8, 16 or 32, radixBits 4, 6 or 8, rasterBatch 128 or 256")
+    lanes = o.get("batchLanes", 1)
+    if lanes != 1:
+        raise N.GsRangeError(N.GS_E_RANGE, "batch lanes above 1 are disabled")
     return o
+
+
+def config_for(o: dict, policy: IntersectionCapacityPolicy) -> N.GsConfig:
+    """The gs_config of validated options."""
+    cfg = N.GsConfig()
+    N.lib().gs_config_default(C.byref(cfg))
+    cfg.device = int(o.get("device", 0))
+    cfg.depth_sort_mode = N.GS_DEPTH_FLOAT32 if o["depthSortMode"] == "float32" else N.GS_DEPTH_PACKED16
+    cfg.antialias_mode = N.GS_AA_COMPENSATED if o["antialiasMode"] == "compensated" else N.GS_AA_CLASSIC
+    cfg.subpixel_sample_culling = 1 if o["subpixelSampleCulling"] else 0
+    cfg.raster_chunk_size = 0 if o["rasterChunkSize"] is None else int(o["rasterChunkSize"])
+    cfg.intersection_capacity = policy.initial
+    cfg.max_intersection_capacity = policy.maximum
+    cfg.output_depth = 1 if o["outputDepth"] else 0
+    cfg.transmittance_threshold = float(o["rasterTransmittanceThreshold"])
+    cfg.depth_alpha_threshold = float(o["depthAlphaThreshold"])
+    cfg.invalid_depth_value = float(o["invalidDepthValue"])
+    cfg.background = (C.c_float * 4)(*o["background"])
+    cfg.raster_stats = 1 if o["rasterStats"] else 0
+    cfg.tile_size = int(o["tileSize"])
+    cfg.radix_bits = int(o["radixBits"])
+    cfg.raster_batch = int(o["rasterBatch"])
+    cfg.batch_lanes = int(o.get("batchLanes", 1))
+    return cfg
+
+
+def upload_scene(ctx, data: PackedScene) -> list:
+    """gs_set_data from a PackedScene; returns the arrays the call read."""
+    keep = [np.ascontiguousarray(data.means, np.float32),
+            np.ascontiguousarray(data.scales_opacity, np.uint32),
+            np.ascontiguousarray(data.rotations, np.uint32),
+            np.ascontiguousarray(data.sh, np.uint32)]
+    active = None if data.active_slots is None else np.ascontiguousarray(data.active_slots, np.uint32)
+    d = N.GsDataDesc()
+    d.count = data.count
+    d.active_count = data.active_count
+    d.geometry_format = 0
+    d.sh_format = 0
+    d.sh_degree = data.sh_degree
+    d.memory = N.GS_MEMORY_HOST
+    d.means, d.scales_opacity, d.rotations, d.sh = (a.ctypes.data for a in keep)
+    d.active_slots = None if active is None else active.ctypes.data
+    N.check(N.lib().gs_set_data(ctx, C.byref(d)), ctx)
+    return keep
 
 
 def frame_for(camera: PerspectiveCamera, width: int, height: int, options: dict | None = None,
@@ -164,23 +221,8 @@ class GaussianPass:
     def __init__(self, camera: PerspectiveCamera, data: PackedScene, options: dict | None = None):
         self.options = validate_options(dict(options or {}))
         self.policy = IntersectionCapacityPolicy(self.options)
-        o = self.options
-        cfg = N.GsConfig()
+        cfg = config_for(self.options, self.policy)
         lib = N.lib()
-        lib.gs_config_default(C.byref(cfg))
-        cfg.device = int(o.get("device", 0))
-        cfg.depth_sort_mode = N.GS_DEPTH_FLOAT32 if o["depthSortMode"] == "float32" else N.GS_DEPTH_PACKED16
-        cfg.antialias_mode = N.GS_AA_COMPENSATED if o["antialiasMode"] == "compensated" else N.GS_AA_CLASSIC
-        cfg.subpixel_sample_culling = 1 if o["subpixelSampleCulling"] else 0
-        cfg.raster_chunk_size = 0 if o["rasterChunkSize"] is None else int(o["rasterChunkSize"])
-        cfg.intersection_capacity = self.policy.initial
-        cfg.max_intersection_capacity = self.policy.maximum
-        cfg.output_depth = 1 if o["outputDepth"] else 0
-        cfg.transmittance_threshold = float(o["rasterTransmittanceThreshold"])
-        cfg.depth_alpha_threshold = float(o["depthAlphaThreshold"])
-        cfg.invalid_depth_value = float(o["invalidDepthValue"])
-        cfg.background = (C.c_float * 4)(*o["background"])
-        cfg.raster_stats = 1 if o["rasterStats"] else 0
         handle = C.c_void_p()
         N.check(lib.gs_create(C.byref(cfg), C.byref(handle)))
         self._ctx = handle
@@ -195,21 +237,7 @@ class GaussianPass:
     # -- data --------------------------------------------------------------
     def update_data(self, data: PackedScene) -> None:
         """TiledGaussianPipeline.updateData (TiledGaussianPipeline.ts:203-218)."""
-        self._keep = [np.ascontiguousarray(data.means, np.float32),
-                      np.ascontiguousarray(data.scales_opacity, np.uint32),
-                      np.ascontiguousarray(data.rotations, np.uint32),
-                      np.ascontiguousarray(data.sh, np.uint32)]
-        active = None if data.active_slots is None else np.ascontiguousarray(data.active_slots, np.uint32)
-        d = N.GsDataDesc()
-        d.count = data.count
-        d.active_count = data.active_count
-        d.geometry_format = 0
-        d.sh_format = 0
-        d.sh_degree = data.sh_degree
-        d.memory = N.GS_MEMORY_HOST
-        d.means, d.scales_opacity, d.rotations, d.sh = (a.ctypes.data for a in self._keep)
-        d.active_slots = None if active is None else active.ctypes.data
-        N.check(N.lib().gs_set_data(self._ctx, C.byref(d)), self._ctx)
+        self._keep = upload_scene(self._ctx, data)
         self.data = data
 
     def update_active(self, start: int, slots: np.ndarray, active_count: int) -> None:

@@ -71,8 +71,18 @@ typedef struct gs_config {
   float background[4];               /* default (0, 0, 0, 0) */
   uint32_t raster_stats;             /* count checked/blended pairs */
   uint32_t capture_graph;            /* replay frames as a captured hipGraph
-                                        (default 1); re-captured when sizes,
-                                        buffers or uniforms change */
+                                        (default 1); re-captured when sizes
+                                        or buffers change (camera and object
+                                        frames are uploaded per launch) */
+  /* Launch shape (no reference counterpart; the reference fixes 16-pixel
+   * tiles, constants.ts:1, 4-bit radix digits, radix.ts, and 256-record
+   * raster batches). Only tile_size 16 yields keys comparable with the
+   * reference; digit width and batch depth never change any output. */
+  uint32_t tile_size;                /* 8, 16 or 32; default 16 */
+  uint32_t radix_bits;               /* 4, 6 or 8; default 8 */
+  uint32_t raster_batch;             /* 128 or 256; default 256 */
+  uint32_t batch_lanes;              /* concurrent frames of a batch; only 1
+                                        is accepted at present (default 1) */
 } gs_config;
 
 /* Packed Gaussian buffers (GaussianData.ts:10-33 / GaussianBuffers), copied
@@ -102,6 +112,9 @@ typedef struct gs_frame {
   uint32_t boundary_axis;      /* GS_BOUNDARY_* */
   uint32_t attempt;            /* render(attempt, resetValid) */
   uint32_t reset_valid;
+  const float* object_frames;  /* this camera's object table (object_count
+                                  * 40 f32, as gs_set_objects), or NULL for
+                                  the table set by gs_set_objects */
 } gs_frame;
 
 /* Output targets. With memory == GS_MEMORY_HOST the call synchronises and
@@ -164,9 +177,49 @@ int gs_render(gs_ctx* ctx, const gs_frame* frame, const gs_image* out);
 
 /* Many cameras of uneven size in one captured launch (no reference
  * counterpart; the reference renders one camera per pass and shares data
- * through dataMode "external", docs/render-extensions.md:5-47). */
+ * through dataMode "external", docs/render-extensions.md:5-47). The n
+ * frames run inside one hipGraph that is kept across calls while sizes and
+ * buffers stay the same. Each view has
+ * its own intersection capacity, state and (with outs == NULL or a NULL
+ * color) library-owned images. Overflow behaves per view as in gs_render. */
 int gs_render_batch(gs_ctx* ctx, uint32_t n, const gs_frame* frames,
                     const gs_image* outs);
+
+/* States of batch views [first, first + n) after the last gs_render_batch
+ * (synchronises). */
+int gs_read_batch_state(gs_ctx* ctx, uint32_t first, uint32_t n, gs_state* out);
+
+/* resizeIntersections for one batch view. */
+int gs_resize_batch_intersections(gs_ctx* ctx, uint32_t view, uint32_t capacity);
+
+/* Copy batch view `view`'s last image to host (sizes as gs_read_image), or
+ * return its device images. */
+int gs_read_batch_image(gs_ctx* ctx, uint32_t view, void* color, size_t color_bytes,
+                        void* depth, size_t depth_bytes);
+int gs_batch_device_image(gs_ctx* ctx, uint32_t view, void** color, void** depth);
+
+/* Launch shape of later frames (the gs_config fields above). */
+typedef struct gs_tuning {
+  uint32_t tile_size;
+  uint32_t radix_bits;
+  uint32_t raster_batch;
+} gs_tuning;
+int gs_set_tuning(gs_ctx* ctx, const gs_tuning* tuning);
+int gs_get_tuning(gs_ctx* ctx, gs_tuning* tuning);
+
+/* Autotuner (no reference counterpart): renders the batch under every
+ * launch shape (tile 8/16/32 x digits 4/6/8 x batch 128/256), growing view
+ * capacities on overflow, times `reps` launches of each with HIP events,
+ * leaves the fastest shape set on the context and returns it in *best.
+ * table (table_len entries, may be NULL) receives each shape's mean ms per
+ * batch; *entries the number of shapes tried. */
+typedef struct gs_tune_entry {
+  gs_tuning tuning;
+  double ms;
+} gs_tune_entry;
+int gs_autotune(gs_ctx* ctx, uint32_t n, const gs_frame* frames, uint32_t reps,
+                gs_tuning* best, gs_tune_entry* table, uint32_t table_len,
+                uint32_t* entries);
 
 /* readStats / getOutputStatus (TiledGaussianPipeline.ts:353-366;
  * IntersectionStage.ts:197-213; GaussianPass.ts:989-1008). */

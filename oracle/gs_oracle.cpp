@@ -42,7 +42,10 @@ struct M3 { V3 c[3]; };   // columns
 struct M4 { V4 c[4]; };   // columns
 
 const float kMaxF32 = 3.402823e+38f;  // projectionHelpers.ts:44,100
-const float kTile = 16.0f;            // constants.ts:1
+
+// Tile edge of a frame: the reference's 16 unless the scene asks for 8 or
+// 32 (launch shapes of the batched renderer, checked at those sizes).
+inline uint32_t tile_edge(const gs_oracle_scene& s) { return s.tile_size ? s.tile_size : 16u; }
 
 inline float comp(const V3& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 inline float comp(const V4& v, int i) {
@@ -310,7 +313,7 @@ bool subpixel_has_sample(V2 center, V3 conic, float thr, V2 extent, uint32_t vx,
 // kernels/tileContribution.ts:271-313 (tile/ellipse test) inside
 // kernels/tileContribution.ts:339-367 (gaussian_boundary_tile_hit)
 bool tile_hit(V2 center, V3 conic, float thr, V2 radius, int tx, int ty, V2 viewport,
-              uint32_t axis) {
+              uint32_t axis, float ts = 16.0f) {
   V2 cc = center;
   int first = 0, last = 0;
   if (axis != 0) {
@@ -319,16 +322,16 @@ bool tile_hit(V2 center, V3 conic, float thr, V2 radius, int tx, int ty, V2 view
     float r = axis == 1 ? radius.x : radius.y;
     float canon = c - gs_floorf(c / period) * period;
     if (axis == 1) cc.x = canon; else cc.y = canon;
-    float tile_start = (float)(axis == 1 ? tx : ty) * kTile;
+    float tile_start = (float)(axis == 1 ? tx : ty) * ts;
     first = gs_f2i(gs_ceilf((tile_start - r - canon) / period));
-    last = gs_f2i(gs_floorf((tile_start + kTile + r - canon) / period));
+    last = gs_f2i(gs_floorf((tile_start + ts + r - canon) / period));
   }
   for (int image = first; image <= last; image++) {
     V2 ic = cc;
     if (axis == 1) ic.x = ic.x + (float)image * viewport.x;
     if (axis == 2) ic.y = ic.y + (float)image * viewport.y;
-    float rminx = (float)tx * kTile, rminy = (float)ty * kTile;
-    float rmaxx = rminx + kTile, rmaxy = rminy + kTile;
+    float rminx = (float)tx * ts, rminy = (float)ty * ts;
+    float rmaxx = rminx + ts, rmaxy = rminy + ts;
     bool x_left = ic.x < rminx, x_right = ic.x > rmaxx;
     bool in_x = !(x_left || x_right);
     bool y_above = ic.y < rminy, y_below = ic.y > rmaxy;
@@ -336,7 +339,7 @@ bool tile_hit(V2 center, V3 conic, float thr, V2 radius, int tx, int ty, V2 view
     bool contributes = in_x && in_y;
     if (!contributes) {
       float cx = x_left ? rminx : rmaxx, cy = y_above ? rminy : rmaxy;
-      float ex = x_left ? kTile : -kTile, ey = y_above ? kTile : -kTile;
+      float ex = x_left ? ts : -ts, ey = y_above ? ts : -ts;
       float dx = ic.x - cx, dy = ic.y - cy;
       float tx_raw = (ex * conic.x * dx + ex * conic.y * dy) / (ex * conic.x * ex);
       float ty_raw = (ey * conic.y * dx + ey * conic.z * dy) / (ey * conic.z * ey);
@@ -354,11 +357,11 @@ bool tile_hit(V2 center, V3 conic, float thr, V2 radius, int tx, int ty, V2 view
 
 // kernels/projectionHelpers.ts:223-236 (count_contributing_tiles)
 uint32_t count_tiles(V2 center, V3 conic, float thr, V2 radius, int minx, int miny,
-                     int maxx, int maxy, V2 viewport, uint32_t axis) {
+                     int maxx, int maxy, V2 viewport, uint32_t axis, float ts = 16.0f) {
   uint32_t n = 0;
   for (int ty = miny; ty <= maxy; ty++)
     for (int tx = minx; tx <= maxx; tx++)
-      if (tile_hit(center, conic, thr, radius, tx, ty, viewport, axis)) n++;
+      if (tile_hit(center, conic, thr, radius, tx, ty, viewport, axis, ts)) n++;
   return n;
 }
 
@@ -531,16 +534,17 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
     return;
   }
   float max_tx = (float)((int)tiles_x - 1), max_ty = (float)((int)tiles_y - 1);
-  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bmin.x / kTile), 0.0f, max_tx));
-  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bmin.y / kTile), 0.0f, max_ty));
-  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmax.x / kTile), 0.0f, max_tx));
-  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmax.y / kTile), 0.0f, max_ty));
+  float ts = (float)tile_edge(s);
+  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bmin.x / ts), 0.0f, max_tx));
+  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bmin.y / ts), 0.0f, max_ty));
+  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmax.x / ts), 0.0f, max_tx));
+  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmax.y / ts), 0.0f, max_ty));
   if (s.boundary_axis == 1) { tmin_x = 0; tmax_x = (int)tiles_x - 1; }
   if (s.boundary_axis == 2) { tmin_y = 0; tmax_y = (int)tiles_y - 1; }
   V3 color = evaluate_sh(s.sh, slot, s.sh_degree, view_dir);
   uint32_t count = count_tiles(center, conic, thr, V2{radius_x, radius_y}, tmin_x,
                                tmin_y, tmax_x, tmax_y, V2{s.width, s.height},
-                               s.boundary_axis);
+                               s.boundary_axis, ts);
   if (count == 0) return;
   pm[0] = center.x;
   pm[1] = center.y;
@@ -569,10 +573,11 @@ uint32_t emit_one(const gs_oracle_result& r, uint32_t tiles_x, uint32_t tiles_y,
   V2 center = {mean[0], mean[1]};
   float thr = gs_logf(mean[3] * 255.0f);
   int max_tx = (int)tiles_x - 1, max_ty = (int)tiles_y - 1;
-  int tmin_x = std::clamp(gs_f2i(gs_floorf((center.x - radius.x) / kTile)), 0, max_tx);
-  int tmin_y = std::clamp(gs_f2i(gs_floorf((center.y - radius.y) / kTile)), 0, max_ty);
-  int tmax_x = std::clamp(gs_f2i(gs_floorf((center.x + radius.x) / kTile)), 0, max_tx);
-  int tmax_y = std::clamp(gs_f2i(gs_floorf((center.y + radius.y) / kTile)), 0, max_ty);
+  float ts = (float)tile_edge(s);
+  int tmin_x = std::clamp(gs_f2i(gs_floorf((center.x - radius.x) / ts)), 0, max_tx);
+  int tmin_y = std::clamp(gs_f2i(gs_floorf((center.y - radius.y) / ts)), 0, max_ty);
+  int tmax_x = std::clamp(gs_f2i(gs_floorf((center.x + radius.x) / ts)), 0, max_tx);
+  int tmax_y = std::clamp(gs_f2i(gs_floorf((center.y + radius.y) / ts)), 0, max_ty);
   if (s.boundary_axis == 1) { tmin_x = 0; tmax_x = max_tx; }
   if (s.boundary_axis == 2) { tmin_y = 0; tmax_y = max_ty; }
   uint32_t reserved = r.ordered_counts[rank];
@@ -582,7 +587,7 @@ uint32_t emit_one(const gs_oracle_result& r, uint32_t tiles_x, uint32_t tiles_y,
   for (int ty = tmin_y; ty <= tmax_y; ty++)
     for (int tx = tmin_x; tx <= tmax_x; tx++)
       if (tile_hit(center, c, thr, radius, tx, ty, V2{s.width, s.height},
-                   s.boundary_axis)) {
+                   s.boundary_axis, ts)) {
         if (local < reserved && base + local < capacity)
           out[base + local] = ((uint64_t)((uint32_t)ty * tiles_x + (uint32_t)tx) << 32) | gid;
         local++;
@@ -651,9 +656,10 @@ void raster_tile(gs_oracle_result& r, uint32_t tile, uint64_t& checked,
   uint32_t chunks = chunked ? (count + cs - 1) / cs : 0;
   chunk_tasks += chunks;
   float bg_a = gs_clampf(s.background[3], 0.0f, 1.0f);
-  for (uint32_t ly = 0; ly < 16; ly++)
-    for (uint32_t lx = 0; lx < 16; lx++) {
-      uint32_t x = tx * 16 + lx, y = ty * 16 + ly;
+  uint32_t edge = tile_edge(s);
+  for (uint32_t ly = 0; ly < edge; ly++)
+    for (uint32_t lx = 0; lx < edge; lx++) {
+      uint32_t x = tx * edge + lx, y = ty * edge + ly;
       if (x >= r.width || y >= r.height) continue;
       float px = (float)x + 0.5f, py = (float)y + 0.5f;
       float c[3] = {0.0f, 0.0f, 0.0f};
@@ -706,7 +712,12 @@ extern "C" gs_oracle_result* gs_oracle_run(const gs_oracle_scene* sp) {
   r.width = gs_f2u(s.width);
   r.height = gs_f2u(s.height);
   // pipeline/TiledGaussianPipeline.ts:292-293
-  uint32_t tiles_x = (r.width + 15) / 16, tiles_y = (r.height + 15) / 16;
+  uint32_t edge = tile_edge(s);
+  if (edge != 8 && edge != 16 && edge != 32) {
+    delete res;
+    return nullptr;
+  }
+  uint32_t tiles_x = (r.width + edge - 1) / edge, tiles_y = (r.height + edge - 1) / edge;
   uint32_t tiles = tiles_x * tiles_y;
   r.counts.tiles_x = tiles_x;
   r.counts.tiles_y = tiles_y;
@@ -885,9 +896,10 @@ extern "C" void gs_oracle_blend_f64(const gs_oracle_result* rp, double* out) {
     for (size_t tile = b; tile < e; tile++) {
       uint32_t tx = (uint32_t)tile % tiles_x, ty = (uint32_t)tile / tiles_x;
       uint32_t begin = r.tile_offsets[tile], end = r.tile_offsets[tile + 1];
-      for (uint32_t ly = 0; ly < 16; ly++)
-        for (uint32_t lx = 0; lx < 16; lx++) {
-          uint32_t x = tx * 16 + lx, y = ty * 16 + ly;
+      uint32_t edge = tile_edge(s);
+      for (uint32_t ly = 0; ly < edge; ly++)
+        for (uint32_t lx = 0; lx < edge; lx++) {
+          uint32_t x = tx * edge + lx, y = ty * edge + ly;
           if (x >= r.width || y >= r.height) continue;
           double px = x + 0.5, py = y + 0.5;
           double c[3] = {0, 0, 0}, t = 1.0;

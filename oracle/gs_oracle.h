@@ -45,6 +45,7 @@ typedef struct gs_oracle_scene {
   float transmittance_threshold;
   float background[4];
   uint32_t threads;               /* worker threads (0 = all) */
+  uint32_t tile_size;             /* tile edge 8, 16 or 32; 0 = 16 (constants.ts:1) */
 } gs_oracle_scene;
 
 typedef struct gs_oracle_counts {

@@ -27,7 +27,7 @@ def oracle_for(gs, oracle, p, scene, camera, width, height, objects=None):
                             depth_mode=0 if o["depthSortMode"] == "float32" else 1,
                             chunk_size=o["rasterChunkSize"] or 0,
                             transmittance=o["rasterTransmittanceThreshold"],
-                            background=o["background"])
+                            background=o["background"], tile_size=o["tileSize"])
 
 
 def render_both(gs, oracle, scene, camera, width, height, options=None, capacity=1 << 22,
@@ -150,8 +150,8 @@ def test_overflow_publishes_nothing_then_grows(gs, oracle, gpu_available, small)
 
 
 def test_graph_replay_and_recapture(gs, oracle, gpu_available, small):
-    """A replayed frame is bit-identical; a new camera re-captures and still
-    matches the oracle."""
+    """A replayed frame is bit-identical; a moved camera replays the same
+    graph with new uniforms and matches the oracle; a new size re-captures."""
     w = gs.WORKLOADS["small"]
     cam = w.camera()
     p, ref = render_both(gs, oracle, small, cam, w.width, w.height)
@@ -164,6 +164,9 @@ def test_graph_replay_and_recapture(gs, oracle, gpu_available, small):
     p.render(w.width, w.height, retry_overflow=False)
     ref2 = oracle_for(gs, oracle, p, small, cam2, w.width, w.height)
     assert_frame_matches(p, ref2, check_f64=False)
+    p.render(w.width // 2, w.height // 2, retry_overflow=False)
+    ref3 = oracle_for(gs, oracle, p, small, cam2, w.width // 2, w.height // 2)
+    assert_frame_matches(p, ref3, check_f64=False)
     p.dispose()
 
 

@@ -69,6 +69,10 @@ def test_capacity_rule_c_matches_python(gs):
     ({"fixedSize": {"width": 0, "height": 10}}, ValueError),
     ({"depthRange": (5.0, 1.0)}, ValueError),
     ({"unknownOption": 1}, TypeError),
+    ({"tileSize": 12}, ValueError),
+    ({"radixBits": 5}, ValueError),
+    ({"rasterBatch": 512}, ValueError),
+    ({"batchLanes": 4}, ValueError),
 ])
 def test_option_validation(gs, opts, err):
     with pytest.raises(err):
