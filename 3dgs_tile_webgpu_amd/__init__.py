@@ -7,6 +7,8 @@ Public API (mirrors the reference's pass API for the frame path):
   gaussian_pass / GaussianPass   -- one camera's pass over the C ABI
   BatchedPass                    -- many cameras in one captured launch,
                                     with the launch-shape autotuner
+  LearnedCull                    -- the MLP cull fused into projection,
+                                    trained on the GPU from plain frames
   PackedScene, workload_scene    -- packed buffers and synthetic workloads
   PerspectiveCamera              -- Three.js camera conventions
 """
@@ -21,4 +23,6 @@ from .gaussian_pass import (  # noqa: F401
     object_frames_for,
     validate_options,
 )
-from .scene import WORKLOADS, PackedScene, Workload, generate, spec_for, workload_scene  # noqa: F401
+from .learned_cull import LearnedCull, psnr  # noqa: F401
+from .scene import (BATCH_RESOLUTIONS, WORKLOADS, PackedScene, Workload, generate,  # noqa: F401
+                    spec_for, workload_scene)

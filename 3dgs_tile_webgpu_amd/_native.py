@@ -167,6 +167,20 @@ SIGNATURES = {
     "gs_get_tuning": (C.c_int, [P, C.POINTER(GsTuning)]),
     "gs_autotune": (C.c_int, [P, u32, C.POINTER(GsFrame), u32, C.POINTER(GsTuning),
                               C.POINTER(GsTuneEntry), u32, C.POINTER(u32)]),
+    "gs_cull_init": (C.c_int, [P, C.c_uint64]),
+    "gs_cull_set_params": (C.c_int, [P, P]),
+    "gs_cull_get_params": (C.c_int, [P, P]),
+    "gs_cull_set_normalization": (C.c_int, [P, P, P]),
+    "gs_cull_get_normalization": (C.c_int, [P, P, P]),
+    "gs_cull_enable": (C.c_int, [P, C.c_int, f32]),
+    "gs_cull_reserve": (C.c_int, [P, u32]),
+    "gs_cull_collect": (C.c_int, [P, C.POINTER(GsFrame), f32, u32, C.POINTER(GsState)]),
+    "gs_cull_dataset_size": (C.c_int, [P, C.POINTER(u32)]),
+    "gs_cull_read_dataset": (C.c_int, [P, u32, u32, P, P]),
+    "gs_cull_fit_normalization": (C.c_int, [P]),
+    "gs_cull_train": (C.c_int, [P, u32, u32, f32, f32, C.c_uint64, C.POINTER(f32)]),
+    "gs_cull_eval": (C.c_int, [P, u32, P, P]),
+    "gs_cull_gradient": (C.c_int, [P, u32, P, P, f32, P, C.POINTER(f32)]),
     "gs_read_state": (C.c_int, [P, C.POINTER(GsState)]),
     "gs_resize_intersections": (C.c_int, [P, u32]),
     "gs_read_image": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t]),

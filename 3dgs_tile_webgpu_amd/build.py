@@ -20,7 +20,7 @@ LIB_DIR = HERE / "_lib"
 LIB_PATH = LIB_DIR / "libgs.so"
 ROOT = HERE.parent
 
-SOURCES = ["gs_kernels.hip", "gs_api.hip", "gs_producer.cpp"]
+SOURCES = ["gs_kernels.hip", "gs_cull.hip", "gs_api.hip", "gs_producer.cpp"]
 HEADERS = ["gs_internal.h", "gs_project.h", "gs_math.h"]
 
 ARCH = os.environ.get("GS_OFFLOAD_ARCH", "gfx950")

@@ -15,6 +15,7 @@
 // unchanged. The camera table is uploaded before every launch, so moving a
 // camera never re-captures.
 #include <math.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "../../include/gs.h"
+#include "gs_cull.h"
 #include "gs_internal.h"
 
 using gs::DeviceState;
@@ -78,6 +80,21 @@ struct ViewSet {
   bool rendered = false;
 };
 
+// The learned cull's device state: the network the kernels read, the fp32
+// parameters with their Adam moments and gradient, the instrumented frame's
+// features and contribution record, and the training set.
+struct CullState {
+  DeviceBuffer net, params, m, v, grad, loss, moments;
+  DeviceBuffer feats, record;
+  DeviceBuffer ds_feats, ds_targets, ds_count;
+  DeviceBuffer tmp_in, tmp_target, tmp_out;
+  uint32_t ds_capacity = 0;
+  uint32_t step = 0;
+  bool ready = false;
+  bool enabled = false;
+  bool collecting = false;
+};
+
 }  // namespace
 
 struct gs_ctx {
@@ -106,6 +123,7 @@ struct gs_ctx {
   // Stage profiling (single-camera path).
   std::vector<hipEvent_t> marks;
   uint32_t profile_capacity = 0, profile_frames = 0;
+  CullState cull;
 };
 
 namespace {
@@ -401,6 +419,13 @@ extern "C" void gs_destroy(gs_ctx* ctx) {
   for (DeviceBuffer* b : all) release(*b);
   set_release(ctx->single);
   set_release(ctx->batch);
+  {
+    CullState& c = ctx->cull;
+    DeviceBuffer* cb[] = {&c.net,  &c.params,   &c.m,          &c.v,        &c.grad,
+                          &c.loss, &c.moments,  &c.feats,      &c.record,   &c.ds_feats,
+                          &c.ds_targets, &c.ds_count, &c.tmp_in, &c.tmp_target, &c.tmp_out};
+    for (DeviceBuffer* b : cb) release(*b);
+  }
   for (Lane& l : ctx->lanes) lane_destroy(ctx, l);
   for (int i = 0; i < gs_ctx::kRing; i++) {
     if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
@@ -717,6 +742,9 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   p.output_depth = ctx->cfg.output_depth ? 1u : 0u;
   p.raster_stats = ctx->cfg.raster_stats ? 1u : 0u;
   p.tile_size = ts;
+  // Training-set frames: chunking off, so every recorded alpha * T uses the
+  // true transmittance in front of it.
+  if (ctx->cull.collecting) p.chunk_size = 0;
   b.state = (DeviceState*)set.state.ptr + v;
   b.scene.means = (const float4*)ctx->means.ptr;
   b.scene.scales_opacity = (const uint2*)ctx->scales_opacity.ptr;
@@ -756,6 +784,12 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.depth_passes = div_up(key_bits, b.radix_bits);
   b.tile_passes = div_up(bits_for(tiles), b.radix_bits);
   if (b.tile_passes == 0) b.tile_passes = 1;
+  if (ctx->cull.collecting) {
+    b.cull_feats = (float*)ctx->cull.feats.ptr;
+    b.record = (uint32_t*)ctx->cull.record.ptr;
+  } else if (ctx->cull.enabled) {
+    b.cull_net = (const gs::CullNet*)ctx->cull.net.ptr;
+  }
 }
 
 // Stage the per-view camera uniforms and object frames through the next
@@ -1168,6 +1202,8 @@ extern "C" int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t by
   else if (n == "intersection_offsets") { src = b.intersection_offsets; len = (size_t)st.visible * 4; }
   else if (n == "sorted_records") { src = gs::sorted_tile_records(b); len = (size_t)st.intersections * 8; }
   else if (n == "tile_offsets") { src = b.tile_offsets; len = ((size_t)b.params.tile_count + 1) * 4; }
+  else if (n == "cull_features" && b.cull_feats) { src = b.cull_feats; len = a * 64; }
+  else if (n == "contribution_record" && b.record) { src = b.record; len = a * 4; }
   else return fail(ctx, GS_E_RANGE, "unknown debug buffer: " + n);
   if (size) *size = len;
   if (!dst) return GS_OK;
@@ -1283,3 +1319,392 @@ extern "C" int gs_autotune(gs_ctx* ctx, uint32_t n, const gs_frame* frames, uint
   return GS_OK;
 }
 
+
+// ------------------------------------------------------------ learned cull ----
+
+namespace {
+
+int cull_require(gs_ctx* ctx) {
+  return ctx->cull.ready ? GS_OK : fail(ctx, GS_E_STATE, "learned cull not initialised");
+}
+
+// Host image of the network from fp32 parameters (bf16 weights).
+void pack_net(const float* p, gs::CullNet& n) {
+  for (int k = 0; k < gs::kCullHidden * gs::kCullIn; k++) n.w1[k] = gs::gs_f2bf(p[gs::kOffW1 + k]);
+  for (int k = 0; k < gs::kCullHidden * gs::kCullHidden; k++)
+    n.w2[k] = gs::gs_f2bf(p[gs::kOffW2 + k]);
+  for (int k = 0; k < gs::kCullHidden; k++) {
+    n.w3[k] = gs::gs_f2bf(p[gs::kOffW3 + k]);
+    n.b1[k] = p[gs::kOffB1 + k];
+    n.b2[k] = p[gs::kOffB2 + k];
+  }
+  n.b3 = p[gs::kOffB3];
+}
+
+int cull_sync_copy(gs_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind,
+                   const char* what) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GS_OK : hip_fail(ctx, e, what);
+}
+
+// Replace the weights and biases of the device network, keeping its
+// threshold and normalisation.
+int cull_upload_params(gs_ctx* ctx, const float* params) {
+  CullState& c = ctx->cull;
+  gs::CullNet n;
+  int rc = cull_sync_copy(ctx, &n, c.net.ptr, sizeof(n), hipMemcpyDeviceToHost, "cull network");
+  if (rc) return rc;
+  pack_net(params, n);
+  if ((rc = cull_sync_copy(ctx, c.net.ptr, &n, sizeof(n), hipMemcpyHostToDevice,
+                           "cull network")))
+    return rc;
+  return cull_sync_copy(ctx, c.params.ptr, params, gs::kCullParams * sizeof(float),
+                        hipMemcpyHostToDevice, "cull parameters");
+}
+
+}  // namespace
+
+extern "C" int gs_cull_init(gs_ctx* ctx, uint64_t seed) {
+  if (!ctx) return GS_E_RANGE;
+  (void)hipSetDevice(ctx->cfg.device);
+  CullState& c = ctx->cull;
+  size_t pb = gs::kCullParams * sizeof(float);
+  int rc;
+  if ((rc = ensure(ctx, c.net, sizeof(gs::CullNet), "cull network")) ||
+      (rc = ensure(ctx, c.params, pb, "cull parameters")) ||
+      (rc = ensure(ctx, c.m, pb, "cull adam m")) || (rc = ensure(ctx, c.v, pb, "cull adam v")) ||
+      (rc = ensure(ctx, c.grad, pb, "cull gradient")) ||
+      (rc = ensure(ctx, c.loss, sizeof(float), "cull loss")) ||
+      (rc = ensure(ctx, c.moments, 2 * gs::kCullIn * sizeof(double), "cull moments")))
+    return rc;
+  // He-uniform weights from splitmix64, zero biases, output bias at the
+  // middle of the target range.
+  std::vector<float> p(gs::kCullParams, 0.0f);
+  uint64_t st = seed;
+  auto uniform = [&]() {
+    uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+  };
+  double a1 = sqrt(6.0 / gs::kCullIn), a2 = sqrt(6.0 / gs::kCullHidden);
+  for (int k = 0; k < gs::kCullHidden * gs::kCullIn; k++) p[gs::kOffW1 + k] = (float)(a1 * uniform());
+  for (int k = 0; k < gs::kCullHidden * gs::kCullHidden; k++)
+    p[gs::kOffW2 + k] = (float)(a2 * uniform());
+  for (int k = 0; k < gs::kCullHidden; k++) p[gs::kOffW3 + k] = (float)(0.1 * a2 * uniform());
+  p[gs::kOffB3] = -8.0f;
+  gs::CullNet n;
+  memset(&n, 0, sizeof(n));
+  pack_net(p.data(), n);
+  n.log_threshold = logf(1e-4f);
+  for (int k = 0; k < gs::kCullIn; k++) {
+    n.mu[k] = 0.0f;
+    n.inv_sigma[k] = 1.0f;
+  }
+  if ((rc = cull_sync_copy(ctx, c.net.ptr, &n, sizeof(n), hipMemcpyHostToDevice, "cull network")) ||
+      (rc = cull_sync_copy(ctx, c.params.ptr, p.data(), pb, hipMemcpyHostToDevice,
+                           "cull parameters")))
+    return rc;
+  hipError_t e = hipMemsetAsync(c.m.ptr, 0, pb, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c.v.ptr, 0, pb, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c.grad.ptr, 0, pb, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull state clear");
+  c.step = 0;
+  c.ready = true;
+  return GS_OK;
+}
+
+extern "C" int gs_cull_set_params(gs_ctx* ctx, const float* params) {
+  if (!ctx || !params) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  for (int k = 0; k < gs::kCullParams; k++)
+    if (!finite(params[k])) return fail(ctx, GS_E_RANGE, "cull parameters must be finite");
+  return cull_upload_params(ctx, params);
+}
+
+extern "C" int gs_cull_get_params(gs_ctx* ctx, float* params) {
+  if (!ctx || !params) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  return cull_sync_copy(ctx, params, ctx->cull.params.ptr, gs::kCullParams * sizeof(float),
+                        hipMemcpyDeviceToHost, "cull parameters");
+}
+
+extern "C" int gs_cull_set_normalization(gs_ctx* ctx, const float mu[16],
+                                         const float inv_sigma[16]) {
+  if (!ctx || !mu || !inv_sigma) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  for (int k = 0; k < gs::kCullIn; k++)
+    if (!finite(mu[k]) || !finite(inv_sigma[k]))
+      return fail(ctx, GS_E_RANGE, "normalisation must be finite");
+  char* base = (char*)ctx->cull.net.ptr;
+  if ((rc = cull_sync_copy(ctx, base + offsetof(gs::CullNet, mu), mu, 16 * sizeof(float),
+                           hipMemcpyHostToDevice, "cull normalisation")))
+    return rc;
+  return cull_sync_copy(ctx, base + offsetof(gs::CullNet, inv_sigma), inv_sigma,
+                        16 * sizeof(float), hipMemcpyHostToDevice, "cull normalisation");
+}
+
+extern "C" int gs_cull_get_normalization(gs_ctx* ctx, float mu[16], float inv_sigma[16]) {
+  if (!ctx || !mu || !inv_sigma) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  gs::CullNet n;
+  if ((rc = cull_sync_copy(ctx, &n, ctx->cull.net.ptr, sizeof(n), hipMemcpyDeviceToHost,
+                           "cull network")))
+    return rc;
+  memcpy(mu, n.mu, sizeof(n.mu));
+  memcpy(inv_sigma, n.inv_sigma, sizeof(n.inv_sigma));
+  return GS_OK;
+}
+
+extern "C" int gs_cull_enable(gs_ctx* ctx, int enabled, float threshold) {
+  if (!ctx) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  if (!(threshold > 0.0f && threshold < 1.0f))
+    return fail(ctx, GS_E_RANGE, "cull threshold must be in (0, 1)");
+  float lt = logf(threshold);
+  char* base = (char*)ctx->cull.net.ptr;
+  if ((rc = cull_sync_copy(ctx, base + offsetof(gs::CullNet, log_threshold), &lt, sizeof(lt),
+                           hipMemcpyHostToDevice, "cull threshold")))
+    return rc;
+  ctx->cull.enabled = enabled != 0;
+  return GS_OK;
+}
+
+extern "C" int gs_cull_reserve(gs_ctx* ctx, uint32_t capacity) {
+  if (!ctx || capacity == 0) return GS_E_RANGE;
+  (void)hipSetDevice(ctx->cfg.device);
+  CullState& c = ctx->cull;
+  int rc;
+  if ((rc = ensure(ctx, c.ds_feats, (size_t)capacity * gs::kCullIn * sizeof(float),
+                   "cull training features")) ||
+      (rc = ensure(ctx, c.ds_targets, (size_t)capacity * sizeof(float), "cull training targets")) ||
+      (rc = ensure(ctx, c.ds_count, sizeof(uint32_t), "cull training count")))
+    return rc;
+  hipError_t e = hipMemsetAsync(c.ds_count.ptr, 0, sizeof(uint32_t), ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull training count");
+  c.ds_capacity = capacity;
+  return GS_OK;
+}
+
+extern "C" int gs_cull_dataset_size(gs_ctx* ctx, uint32_t* n) {
+  if (!ctx || !n) return GS_E_RANGE;
+  if (!ctx->cull.ds_capacity) {
+    *n = 0;
+    return GS_OK;
+  }
+  uint32_t v = 0;
+  int rc = cull_sync_copy(ctx, &v, ctx->cull.ds_count.ptr, sizeof(v), hipMemcpyDeviceToHost,
+                          "cull training count");
+  if (rc) return rc;
+  *n = v < ctx->cull.ds_capacity ? v : ctx->cull.ds_capacity;
+  return GS_OK;
+}
+
+// An instrumented plain frame of the single-camera path, then a hashed
+// sample (fraction `rate`) of its visible Gaussians appended to the set.
+// An overflowing frame appends nothing; *state reports it.
+extern "C" int gs_cull_collect(gs_ctx* ctx, const gs_frame* f, float rate, uint32_t seed,
+                               gs_state* state) {
+  if (!ctx || !f) return GS_E_RANGE;
+  if (!ctx->cull.ds_capacity) return fail(ctx, GS_E_STATE, "no training set reserved");
+  if (!(rate > 0.0f && rate <= 1.0f)) return fail(ctx, GS_E_RANGE, "rate must be in (0, 1]");
+  if (!ctx->has_data) return fail(ctx, GS_E_STATE, "render before data");
+  (void)hipSetDevice(ctx->cfg.device);
+  CullState& c = ctx->cull;
+  size_t a = ctx->active_count ? ctx->active_count : 1;
+  int rc;
+  if ((rc = ensure(ctx, c.feats, a * gs::kCullIn * sizeof(float), "cull features")) ||
+      (rc = ensure(ctx, c.record, a * sizeof(uint32_t), "contribution record")))
+    return rc;
+  c.collecting = true;
+  rc = render_set(ctx, ctx->single, 1, f, nullptr, true);
+  c.collecting = false;
+  if (rc) return rc;
+  gs_state st;
+  if ((rc = gs_read_state(ctx, &st))) return rc;
+  if (state) *state = st;
+  if (st.overflow) return GS_OK;
+  double keep = (double)rate * 4294967296.0;
+  uint32_t k = keep >= 4294967295.0 ? 0xffffffffu : (uint32_t)keep;
+  hipError_t e = gs::cull_collect((const float4*)ctx->single.frames[0].proj.mean,
+                                  (const float*)c.feats.ptr, (const uint32_t*)c.record.ptr,
+                                  ctx->active_count, k, seed, (float*)c.ds_feats.ptr,
+                                  (float*)c.ds_targets.ptr, (uint32_t*)c.ds_count.ptr,
+                                  c.ds_capacity, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull collect");
+  return GS_OK;
+}
+
+extern "C" int gs_cull_read_dataset(gs_ctx* ctx, uint32_t first, uint32_t n, float* features,
+                                    float* targets) {
+  if (!ctx) return GS_E_RANGE;
+  uint32_t size = 0;
+  int rc = gs_cull_dataset_size(ctx, &size);
+  if (rc) return rc;
+  if ((uint64_t)first + n > size) return fail(ctx, GS_E_RANGE, "range exceeds the training set");
+  if (n == 0) return GS_OK;
+  const CullState& c = ctx->cull;
+  if (features &&
+      (rc = cull_sync_copy(ctx, features, (const float*)c.ds_feats.ptr + (size_t)first * gs::kCullIn,
+                           (size_t)n * gs::kCullIn * sizeof(float), hipMemcpyDeviceToHost,
+                           "cull training features")))
+    return rc;
+  if (targets &&
+      (rc = cull_sync_copy(ctx, targets, (const float*)c.ds_targets.ptr + first,
+                           (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                           "cull training targets")))
+    return rc;
+  return GS_OK;
+}
+
+// Feature standardisation from the training set: mean and 1 / std (std
+// floored at 1e-3).
+extern "C" int gs_cull_fit_normalization(gs_ctx* ctx) {
+  if (!ctx) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  uint32_t n = 0;
+  if ((rc = gs_cull_dataset_size(ctx, &n))) return rc;
+  if (n < 2) return fail(ctx, GS_E_STATE, "training set too small");
+  CullState& c = ctx->cull;
+  hipError_t e = hipMemsetAsync(c.moments.ptr, 0, 2 * gs::kCullIn * sizeof(double), ctx->stream);
+  if (e == hipSuccess)
+    e = gs::cull_moments((const float*)c.ds_feats.ptr, n, (double*)c.moments.ptr, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull moments");
+  double mom[2 * gs::kCullIn];
+  if ((rc = cull_sync_copy(ctx, mom, c.moments.ptr, sizeof(mom), hipMemcpyDeviceToHost,
+                           "cull moments")))
+    return rc;
+  float mu[gs::kCullIn], inv[gs::kCullIn];
+  for (int k = 0; k < gs::kCullIn; k++) {
+    double m = mom[k] / n;
+    double var = mom[gs::kCullIn + k] / n - m * m;
+    double sd = sqrt(var > 0.0 ? var : 0.0);
+    mu[k] = (float)m;
+    inv[k] = (float)(1.0 / (sd > 1e-3 ? sd : 1e-3));
+  }
+  return gs_cull_set_normalization(ctx, mu, inv);
+}
+
+namespace {
+
+uint64_t gcd64(uint64_t a, uint64_t b) {
+  while (b) {
+    uint64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+}  // namespace
+
+// `steps` Adam steps on minibatches of `batch` samples drawn by a fresh
+// affine permutation of the training set each step.
+extern "C" int gs_cull_train(gs_ctx* ctx, uint32_t steps, uint32_t batch, float lr,
+                             float under_weight, uint64_t seed, float* last_loss) {
+  if (!ctx || batch == 0) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  if (!(lr > 0.0f && finite(lr)) || !(under_weight >= 1.0f && finite(under_weight)))
+    return fail(ctx, GS_E_RANGE, "learning rate must be positive, under-weight at least 1");
+  uint32_t n = 0;
+  if ((rc = gs_cull_dataset_size(ctx, &n))) return rc;
+  if (n == 0) return fail(ctx, GS_E_STATE, "empty training set");
+  CullState& c = ctx->cull;
+  const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  uint64_t st = seed;
+  auto next = [&]() {
+    uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  hipError_t e = hipSuccess;
+  for (uint32_t s = 0; s < steps && e == hipSuccess; s++) {
+    uint64_t a = 1;
+    if (n > 1)
+      do a = next() % n; while (a == 0 || gcd64(a, n) != 1);
+    uint32_t b = (uint32_t)(next() % n);
+    e = hipMemsetAsync(c.loss.ptr, 0, sizeof(float), ctx->stream);
+    if (e == hipSuccess)
+      e = gs::cull_train_step((const float*)c.ds_feats.ptr, (const float*)c.ds_targets.ptr, n,
+                              (uint32_t)a, b, 0u, batch, (const gs::CullNet*)c.net.ptr,
+                              under_weight, (float*)c.grad.ptr, (float*)c.loss.ptr, ctx->stream);
+    c.step++;
+    float corr1 = 1.0f - powf(beta1, (float)c.step), corr2 = 1.0f - powf(beta2, (float)c.step);
+    if (e == hipSuccess)
+      e = gs::cull_adam((float*)c.params.ptr, (float*)c.m.ptr, (float*)c.v.ptr,
+                        (float*)c.grad.ptr, lr, beta1, beta2, eps, corr1, corr2,
+                        (gs::CullNet*)c.net.ptr, ctx->stream);
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull training");
+  float loss = 0.0f;
+  if ((rc = cull_sync_copy(ctx, &loss, c.loss.ptr, sizeof(loss), hipMemcpyDeviceToHost,
+                           "cull loss")))
+    return rc;
+  if (last_loss) *last_loss = loss / (float)batch;
+  return GS_OK;
+}
+
+// The network's predictions for n host feature rows (raw features).
+extern "C" int gs_cull_eval(gs_ctx* ctx, uint32_t n, const float* features, float* out) {
+  if (!ctx || (n > 0 && (!features || !out))) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc || n == 0) return rc;
+  CullState& c = ctx->cull;
+  if ((rc = ensure(ctx, c.tmp_in, (size_t)n * gs::kCullIn * sizeof(float), "cull eval input")) ||
+      (rc = ensure(ctx, c.tmp_out, (size_t)n * sizeof(float), "cull eval output")) ||
+      (rc = cull_sync_copy(ctx, c.tmp_in.ptr, features, (size_t)n * gs::kCullIn * sizeof(float),
+                           hipMemcpyHostToDevice, "cull eval input")))
+    return rc;
+  hipError_t e = gs::cull_eval((const float*)c.tmp_in.ptr, n, (const gs::CullNet*)c.net.ptr,
+                               (float*)c.tmp_out.ptr, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull eval");
+  return cull_sync_copy(ctx, out, c.tmp_out.ptr, (size_t)n * sizeof(float),
+                        hipMemcpyDeviceToHost, "cull eval output");
+}
+
+// Gradient of the training loss over n host samples, in parameter order
+// (no parameter update), and the loss.
+extern "C" int gs_cull_gradient(gs_ctx* ctx, uint32_t n, const float* features,
+                                const float* targets, float under_weight, float* grad,
+                                float* loss) {
+  if (!ctx || n == 0 || !features || !targets || !grad) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  CullState& c = ctx->cull;
+  size_t pb = gs::kCullParams * sizeof(float);
+  if ((rc = ensure(ctx, c.tmp_in, (size_t)n * gs::kCullIn * sizeof(float), "cull grad input")) ||
+      (rc = ensure(ctx, c.tmp_target, (size_t)n * sizeof(float), "cull grad target")) ||
+      (rc = cull_sync_copy(ctx, c.tmp_in.ptr, features, (size_t)n * gs::kCullIn * sizeof(float),
+                           hipMemcpyHostToDevice, "cull grad input")) ||
+      (rc = cull_sync_copy(ctx, c.tmp_target.ptr, targets, (size_t)n * sizeof(float),
+                           hipMemcpyHostToDevice, "cull grad target")))
+    return rc;
+  hipError_t e = hipMemsetAsync(c.grad.ptr, 0, pb, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c.loss.ptr, 0, sizeof(float), ctx->stream);
+  if (e == hipSuccess)
+    e = gs::cull_train_step((const float*)c.tmp_in.ptr, (const float*)c.tmp_target.ptr, n, 1u, 0u,
+                            0u, n, (const gs::CullNet*)c.net.ptr, under_weight,
+                            (float*)c.grad.ptr, (float*)c.loss.ptr, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "cull gradient");
+  if ((rc = cull_sync_copy(ctx, grad, c.grad.ptr, pb, hipMemcpyDeviceToHost, "cull gradient")))
+    return rc;
+  float l = 0.0f;
+  if ((rc = cull_sync_copy(ctx, &l, c.loss.ptr, sizeof(l), hipMemcpyDeviceToHost, "cull loss")))
+    return rc;
+  if (loss) *loss = l / (float)n;
+  e = hipMemsetAsync(c.grad.ptr, 0, pb, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? GS_OK : hip_fail(ctx, e, "cull gradient clear");
+}

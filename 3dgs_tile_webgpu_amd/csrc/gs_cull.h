@@ -1,0 +1,239 @@
+// gs_cull.h - the learned cull: a 16-64-64-1 MLP that predicts, from a
+// projected Gaussian's features, the log of the largest alpha * T it
+// contributes to any pixel; Gaussians predicted below a threshold are
+// dropped before duplication and sort. No counterpart in the reference (its
+// nearest mechanisms are the opacity and subpixel culls of
+// ProjectionStage.ts:533, 572-587).
+//
+// Weights are bf16, products run on the matrix cores
+// (v_mfma_f32_16x16x16bf16_1k) with fp32 accumulation, one wave per 64
+// Gaussians. Operand layout of that instruction, lane l = 16 g + i
+// (g = l / 16, i = l % 16): A (16x16, M x K) lane holds A[i][4g .. 4g+3];
+// B (16x16, K x N) lane holds B[4g .. 4g+3][i]; D (16x16) lane holds
+// D[4g + r][i], r = 0..3.
+#ifndef GS_CULL_H
+#define GS_CULL_H
+
+#include <stdint.h>
+
+#include "gs_math.h"
+
+namespace gs {
+
+constexpr int kCullIn = 16;
+constexpr int kCullHidden = 64;
+// fp32 parameter vector: W1 [64][16], b1 [64], W2 [64][64], b2 [64], w3 [64], b3.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kCullHidden * kCullIn;
+constexpr int kOffW2 = kOffB1 + kCullHidden;
+constexpr int kOffB2 = kOffW2 + kCullHidden * kCullHidden;
+constexpr int kOffW3 = kOffB2 + kCullHidden;
+constexpr int kOffB3 = kOffW3 + kCullHidden;
+constexpr int kCullParams = kOffB3 + 1;  // 5313
+// Padded row length (bf16) of the 64-wide LDS matrices.
+constexpr int kRow = 72;
+
+// The network as the kernels read it.
+struct CullNet {
+  uint16_t w1[kCullHidden * kCullIn];      // bf16 [hidden][input]
+  uint16_t w2[kCullHidden * kCullHidden];  // bf16 [out][in]
+  uint16_t w3[kCullHidden];                // bf16
+  float b1[kCullHidden], b2[kCullHidden];
+  float b3;
+  float log_threshold;  // cull when the prediction is below this
+  float mu[kCullIn], inv_sigma[kCullIn];  // feature standardisation
+};
+
+// binary32 -> bfloat16, round to nearest even (NaN -> quiet NaN).
+GS_HD inline uint16_t gs_f2bf(float f) {
+  uint32_t u = gs_f2b(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+GS_HD inline float gs_bf2f(uint16_t h) { return gs_b2f((uint32_t)h << 16); }
+
+// The 16 features of a projected Gaussian: footprint (log pixel radius,
+// log conic diagonal, conic correlation), opacity (linear and log), log
+// view depth, the object-space view direction, the object-space position,
+// log SH energy (sum of squared higher-band coefficients) and the
+// normalised screen position.
+GS_HD inline void cull_features(float f[kCullIn], float rx, float ry, const float conic[3],
+                                float opacity, float depth, const float dir[3],
+                                const float pos[3], float sh_energy, float cx, float cy,
+                                float vw, float vh) {
+  f[0] = gs_logf(rx > ry ? rx : ry);
+  f[1] = gs_logf(conic[0]);
+  f[2] = conic[1] / gs_sqrtf(conic[0] * conic[2]);
+  f[3] = gs_logf(conic[2]);
+  f[4] = opacity;
+  f[5] = gs_logf(opacity);
+  f[6] = gs_logf(depth);
+  f[7] = dir[0];
+  f[8] = dir[1];
+  f[9] = dir[2];
+  f[10] = pos[0];
+  f[11] = pos[1];
+  f[12] = pos[2];
+  f[13] = gs_logf(sh_energy + 1e-6f);
+  f[14] = cx / vw;
+  f[15] = cy / vh;
+}
+
+#ifdef __HIPCC__
+
+typedef short gs_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float gs_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ gs_f32x4 mfma16(gs_bf16x4 a, gs_bf16x4 b, gs_f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Wave-scope ordering of LDS writes before reads by other lanes of the
+// same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Four consecutive bf16 of an LDS row.
+__device__ __forceinline__ gs_bf16x4 lds4(const uint16_t* p) {
+  return *reinterpret_cast<const gs_bf16x4*>(p);
+}
+// Four bf16 a row apart (a column slice).
+__device__ __forceinline__ gs_bf16x4 lds4_col(const uint16_t* p, int stride) {
+  gs_bf16x4 v;
+  v.x = (short)p[0];
+  v.y = (short)p[stride];
+  v.z = (short)p[2 * stride];
+  v.w = (short)p[3 * stride];
+  return v;
+}
+
+// The network's weights staged in LDS for a block.
+struct CullLdsNet {
+  uint16_t w1[kCullHidden * kCullIn];
+  uint16_t w2[kCullHidden * kRow];  // [out][in], rows padded
+  float w3[kCullHidden];
+  float b1[kCullHidden], b2[kCullHidden];
+  float b3, log_threshold;
+  float mu[kCullIn], inv_sigma[kCullIn];
+};
+
+// Per-wave scratch of the inference forward pass: standardised inputs,
+// then (over the same bytes) the hidden activations, and the outputs.
+struct CullLdsWave {
+  union {
+    uint16_t x[64 * kCullIn];
+    uint16_t a1[64 * kRow];
+  };
+  float y[64];
+};
+
+// Cooperative copy of the network into LDS (all threads of the block).
+__device__ inline void cull_stage_net(const CullNet* __restrict__ net, CullLdsNet& L) {
+  for (int t = threadIdx.x; t < kCullHidden * kCullIn; t += blockDim.x) L.w1[t] = net->w1[t];
+  for (int t = threadIdx.x; t < kCullHidden * kCullHidden; t += blockDim.x)
+    L.w2[(t / kCullHidden) * kRow + (t % kCullHidden)] = net->w2[t];
+  for (int t = threadIdx.x; t < kCullHidden; t += blockDim.x) {
+    L.w3[t] = gs_bf2f(net->w3[t]);
+    L.b1[t] = net->b1[t];
+    L.b2[t] = net->b2[t];
+  }
+  for (int t = threadIdx.x; t < kCullIn; t += blockDim.x) {
+    L.mu[t] = net->mu[t];
+    L.inv_sigma[t] = net->inv_sigma[t];
+  }
+  if (threadIdx.x == 0) {
+    L.b3 = net->b3;
+    L.log_threshold = net->log_threshold;
+  }
+}
+
+// Forward pass of one wave: lane l holds the raw features of sample l
+// (zeros for idle lanes); returns that sample's prediction. x (64 x 16) and
+// a1 (64 x kRow) are the wave's LDS rows for the bf16 inputs and hidden
+// activations (they may overlap: x is dead once layer 1 has run); y holds
+// 64 outputs. With kKeepZ2 the layer-2 pre-activations (bias added) are
+// returned in z2, in the D layout above as [mt][nt] tiles.
+template <bool kKeepZ2>
+__device__ inline float cull_forward_wave(const CullLdsNet& N, uint16_t* x, uint16_t* a1,
+                                          float* y, const float f[kCullIn],
+                                          gs_f32x4 (*z2)[4]) {
+  const int l = __lane_id();
+  const int i = l & 15, g = l >> 4;
+  // Standardised inputs, bf16, row l.
+#pragma unroll
+  for (int k = 0; k < kCullIn; k++) x[l * kCullIn + k] = gs_f2bf((f[k] - N.mu[k]) * N.inv_sigma[k]);
+  wave_sync();
+  // Layer 1: Z1 = X W1^T (64 x 64), K = 16.
+  gs_f32x4 z1[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    gs_bf16x4 a = lds4(&x[(16 * mt + i) * kCullIn + 4 * g]);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      gs_bf16x4 b = lds4(&N.w1[(16 * nt + i) * kCullIn + 4 * g]);
+      z1[mt][nt] = mfma16(a, b, gs_f32x4{0.f, 0.f, 0.f, 0.f});
+    }
+  }
+  wave_sync();  // the x rows are overwritten by a1 below
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      float bias = N.b1[16 * nt + i];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float v = z1[mt][nt][r] + bias;
+        a1[(16 * mt + 4 * g + r) * kRow + 16 * nt + i] = gs_f2bf(v > 0.0f ? v : 0.0f);
+      }
+    }
+  wave_sync();
+  // Layer 2: Z2 = A1 W2^T (64 x 64), K = 64; layer 3 folded in.
+  float part[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) part[mt][r] = 0.0f;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      gs_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < 4; kt++) {
+        gs_bf16x4 a = lds4(&a1[(16 * mt + i) * kRow + 16 * kt + 4 * g]);
+        gs_bf16x4 b = lds4(&N.w2[(16 * nt + i) * kRow + 16 * kt + 4 * g]);
+        acc = mfma16(a, b, acc);
+      }
+      float bias = N.b2[16 * nt + i], w3 = N.w3[16 * nt + i];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float v = acc[r] + bias;
+        if (kKeepZ2) z2[mt][nt][r] = v;
+        part[mt][r] += (v > 0.0f ? v : 0.0f) * w3;
+      }
+    }
+  // Sum over the 16 hidden units each group of 16 lanes holds.
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float v = part[mt][r];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      v += __shfl_xor(v, 8, 64);
+      if (i == 0) y[16 * mt + 4 * g + r] = v + N.b3;
+    }
+  wave_sync();
+  return y[l];
+}
+
+#endif  // __HIPCC__
+
+}  // namespace gs
+
+#endif

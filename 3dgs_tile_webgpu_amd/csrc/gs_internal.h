@@ -7,6 +7,8 @@
 
 namespace gs {
 
+struct CullNet;
+
 constexpr int kBlock = 256;          // threads per block (4 wave64)
 constexpr int kItems = 16;           // keys per thread in scans and sorts
 constexpr int kPartition = kBlock * kItems;  // 4096 keys per block
@@ -127,6 +129,12 @@ struct FrameBuffers {
   uint32_t tile_passes;
   uint32_t radix_bits;   // digit width: 4, 6 or 8
   uint32_t raster_batch; // records staged per light-tile round: 128 or 256
+  // Learned cull: the network run inside projection (or null), and for
+  // training-set collection the per-Gaussian features written by projection
+  // and the per-Gaussian max alpha * T recorded by the raster (or null).
+  const CullNet* cull_net;
+  float* cull_feats;
+  uint32_t* record;
 };
 
 // Frame stages, in launch order; launch_frame can record an event before
@@ -152,6 +160,20 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, hipStream_t a
 // Location of the sorted depth / tile records after launch_frame.
 const uint2* sorted_depth_records(const FrameBuffers& b);
 const uint2* sorted_tile_records(const FrameBuffers& b);
+
+// Learned-cull training and evaluation launches (gs_cull.hip).
+hipError_t cull_collect(const float4* pmean, const float* feats, const uint32_t* record,
+                        uint32_t active, uint32_t keep, uint32_t seed, float* ds_feats,
+                        float* ds_targets, uint32_t* count, uint32_t capacity, hipStream_t s);
+hipError_t cull_moments(const float* feats, uint32_t n, double* out, hipStream_t s);
+hipError_t cull_train_step(const float* feats, const float* targets, uint32_t n, uint32_t perm_a,
+                           uint32_t perm_b, uint32_t start, uint32_t batch, const CullNet* net,
+                           float under_weight, float* grad, float* loss, hipStream_t s);
+hipError_t cull_adam(float* params, float* m, float* v, float* grad, float lr, float beta1,
+                     float beta2, float eps, float corr1, float corr2, CullNet* net,
+                     hipStream_t s);
+hipError_t cull_eval(const float* feats, uint32_t n, const CullNet* net, float* out,
+                     hipStream_t s);
 
 }  // namespace gs
 

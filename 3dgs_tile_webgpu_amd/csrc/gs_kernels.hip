@@ -18,6 +18,7 @@
 // Every size that the reference reads through indirect dispatch (V, K) is
 // read from DeviceState on the device, so the frame can be replayed as a
 // captured graph; grids are sized by capacity and idle blocks exit at once.
+#include "gs_cull.h"
 #include "gs_internal.h"
 #include "gs_project.h"
 
@@ -70,24 +71,35 @@ GS_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds4
 
 // -------------------------------------------------------------- project ----
 
-__global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
-                                                    DeviceState* __restrict__ state,
-                                                    SceneView sc, ProjectedView out) {
-  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  if (gid == 0) {
-    state->checked = 0ull;
-    state->blended = 0ull;
-  }
-  if (gid >= fp.active_count) return;
-  out.tile_counts[gid] = 0u;
-  out.mean[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// Outcome of projecting one Gaussian: nothing to draw, culled by the
+// subpixel sample test (its centre and depth are still published), or
+// visible with `count` tiles.
+enum ProjKind : int { kProjNone = 0, kProjSubpixel = 1, kProjVisible = 2 };
+struct ProjOut {
+  int kind;
+  float4 mean, conic, color;
+  uint32_t count;
+};
+
+// Projection modes: the plain path, the learned cull, and feature
+// collection for its training set.
+enum ProjMode : int { kModePlain = 0, kModeCull = 1, kModeCollect = 2 };
+
+// ProjectionStage.ts:385-649 for active ordinal gid. With kMode != plain it
+// also fills the 16 cull features of a visible Gaussian.
+template <int kMode>
+GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint32_t gid,
+                                float feat[kCullIn]) {
+  ProjOut o;
+  o.kind = kProjNone;
+  o.count = 0u;
   uint32_t slot = fp.has_active_slots ? sc.active_slots[gid] : gid;
   float4 mean = sc.means[slot];
-  if (mean.w < 0.0f) return;
+  if (mean.w < 0.0f) return o;
   uint32_t object = gs_f2u(mean.w);
-  if (object >= fp.object_count) return;
+  if (object >= fp.object_count) return o;
   const float* frame = sc.objects + (size_t)object * 40;
-  if (!(frame[36] > 0.0f)) return;
+  if (!(frame[36] > 0.0f)) return o;
   uint2 so = sc.scales_opacity[slot];
   float scale[3] = {gs_expf(gs_half_to_float(so.x & 0xffffu)),
                     gs_expf(gs_half_to_float(so.x >> 16)),
@@ -122,27 +134,28 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   view_covariance(scale, rot, mv, cv);
   Projected p;
   perspective_projection(view, cv, fp, *fp.camera, p);
-  if (!p.valid) return;
+  if (!p.valid) return o;
   float opacity = gs_clampf(base_opacity, 0.0f, 1.0f);
   if (fp.antialias_compensated)
     opacity = opacity * gs_sqrtf(gs_clampf(p.original_det / p.det, 0.0f, 1.0f));
-  if (opacity < (float)(1.0 / 255.0) || opacity != opacity) return;
+  if (opacity < (float)(1.0 / 255.0) || opacity != opacity) return o;
   float thr = gs_logf(opacity * 255.0f);
   float ext_x = gs_sqrtf(thr * 2.0f * gs_clampf(p.cov[0], 1e-12f, 1e4f));
   float ext_y = gs_sqrtf(thr * 2.0f * gs_clampf(p.cov[2], 1e-12f, 1e4f));
   float rx = gs_ceilf(ext_x), ry = gs_ceilf(ext_y);
-  if (rx <= 0.0f || ry <= 0.0f) return;
+  if (rx <= 0.0f || ry <= 0.0f) return o;
   float bminx = p.cx - rx, bminy = p.cy - ry, bmaxx = p.cx + rx, bmaxy = p.cy + ry;
   float vw = fp.viewport[0], vh = fp.viewport[1];
   uint32_t axis = fp.boundary_axis;
   if ((axis != 1 && (bmaxx < 0.0f || bminx >= vw)) ||
       (axis != 2 && (bmaxy < 0.0f || bminy >= vh)))
-    return;
+    return o;
   if (fp.subpixel_culling &&
       !subpixel_has_sample(p.cx, p.cy, p.conic, thr, ext_x, ext_y, gs_f2u(vw), gs_f2u(vh),
                            axis)) {
-    out.mean[gid] = make_float4(p.cx, p.cy, p.depth, -1.0f);
-    return;
+    o.kind = kProjSubpixel;
+    o.mean = make_float4(p.cx, p.cy, p.depth, -1.0f);
+    return o;
   }
   float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
   float ts = (float)fp.tile_size;
@@ -162,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   for (int ty = tmin_y; ty <= tmax_y; ty++)
     for (int tx = tmin_x; tx <= tmax_x; tx++)
       if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) count++;
-  if (count == 0) return;
+  if (count == 0) return o;
   // SH colour (projectionHelpers.ts:118-176), direction = normalized
   // object-space offset from the camera.
   uint32_t degree = fp.sh_degree;
@@ -173,6 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   decode_sh(shc[0], k);
   float w0 = 0.28209479177387814f;
   float col[3] = {w0 * k[0], w0 * k[1], w0 * k[2]};
+  float sh_energy = 0.0f;
 #define GS_SH_ADD(wexpr, idx)                       \
   {                                                 \
     float w_ = (wexpr);                             \
@@ -180,6 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
     col[0] = col[0] + w_ * k[0];                    \
     col[1] = col[1] + w_ * k[1];                    \
     col[2] = col[2] + w_ * k[2];                    \
+    if (kMode != kModePlain)                        \
+      sh_energy += k[0] * k[0] + k[1] * k[1] + k[2] * k[2]; \
   }
   if (degree >= 1) {
     GS_SH_ADD(-0.4886025119029199f * y, 1);
@@ -208,10 +224,67 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   float r = gs_clampf(col[0] + 0.5f, 0.0f, 1.0f);
   float g = gs_clampf(col[1] + 0.5f, 0.0f, 1.0f);
   float b = gs_clampf(col[2] + 0.5f, 0.0f, 1.0f);
-  out.mean[gid] = make_float4(p.cx, p.cy, p.depth, opacity);
-  out.conic[gid] = make_float4(p.conic[0], p.conic[1], p.conic[2], rx + ry * 512.0f);
-  out.color[gid] = make_float4(r, g, b, gs_b2f(slot ^ 0x3f800000u));
-  out.tile_counts[gid] = count;
+  o.kind = kProjVisible;
+  o.mean = make_float4(p.cx, p.cy, p.depth, opacity);
+  o.conic = make_float4(p.conic[0], p.conic[1], p.conic[2], rx + ry * 512.0f);
+  o.color = make_float4(r, g, b, gs_b2f(slot ^ 0x3f800000u));
+  o.count = count;
+  if (kMode != kModePlain)
+    cull_features(feat, rx, ry, p.conic, opacity, p.depth, dir, local, sh_energy, p.cx, p.cy,
+                  vw, vh);
+  return o;
+}
+
+// Projection of every active Gaussian. The learned-cull variant runs the
+// MLP over each wave's visible Gaussians (matrix cores, weights in LDS) and
+// drops those predicted below the threshold as if they touched no tile;
+// the collect variant stores the features and clears the per-Gaussian
+// contribution record of the instrumented raster.
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
+                                                    DeviceState* __restrict__ state,
+                                                    SceneView sc, ProjectedView out,
+                                                    const CullNet* __restrict__ net,
+                                                    float* __restrict__ feats,
+                                                    uint32_t* __restrict__ record) {
+  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  if (gid == 0) {
+    state->checked = 0ull;
+    state->blended = 0ull;
+  }
+  float feat[kCullIn];
+  ProjOut o;
+  o.kind = kProjNone;
+  if (gid < fp.active_count) o = project_gaussian<kMode>(fp, sc, gid, feat);
+  if constexpr (kMode == kModeCull) {
+    __shared__ CullLdsNet lnet;
+    __shared__ CullLdsWave lwave[kBlock / 64];
+    cull_stage_net(net, lnet);
+    __syncthreads();
+    bool visible = o.kind == kProjVisible;
+    if (__ballot(visible) != 0ull) {  // wave-uniform
+      if (!visible)
+        for (int k = 0; k < kCullIn; k++) feat[k] = 0.0f;
+      CullLdsWave& w = lwave[threadIdx.x >> 6];
+      float y = cull_forward_wave<false>(lnet, w.x, w.a1, w.y, feat, nullptr);
+      if (visible && y < lnet.log_threshold) o.kind = kProjNone;
+    }
+  }
+  if (gid >= fp.active_count) return;
+  if constexpr (kMode == kModeCollect) {
+    record[gid] = 0u;
+    if (o.kind == kProjVisible)
+      for (int k = 0; k < kCullIn; k++) feats[(size_t)gid * kCullIn + k] = feat[k];
+  }
+  if (o.kind == kProjVisible) {
+    out.mean[gid] = o.mean;
+    out.conic[gid] = o.conic;
+    out.color[gid] = o.color;
+    out.tile_counts[gid] = o.count;
+  } else {
+    out.tile_counts[gid] = 0u;
+    out.mean[gid] = o.kind == kProjSubpixel ? o.mean : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
 // ----------------------------------------------------------------- scans ----
@@ -624,6 +697,7 @@ struct RasterLds {
   float4 mean[kBatch];   // cx, cy, depth, opacity
   float4 conic[kBatch];  // a, b, c, ln(255 opacity)
   float4 color[kBatch];
+  uint32_t gid[kBatch];  // active ordinals (instrumented raster only)
   uint32_t mask[kWords][kPixels];  // [word][pixel] candidate bits of the round
 };
 
@@ -648,15 +722,17 @@ GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, 
 
 // Blend records [cb, ce) of `tile` into one pixel per thread (threads
 // 0..E*E-1), from T = 1. All kThreads threads of the block must call it;
-// threads below kBatch stage one record per round.
-template <int kEdge, int kThreads, int kBatch>
+// threads below kBatch stage one record per round. The instrumented
+// variant (kRecord) also raises record[id] to every alpha * T that
+// Gaussian id contributes (binary32 bits; non-negative, so unsigned order).
+template <int kEdge, int kThreads, int kBatch, bool kRecord>
 GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
                                  const uint2* __restrict__ sorted,
                                  const float4* __restrict__ pmean,
                                  const float4* __restrict__ pconic,
                                  const float4* __restrict__ pcolor,
                                  RasterLds<kEdge * kEdge, kBatch>& L, uint32_t& x, uint32_t& y,
-                                 bool& active) {
+                                 bool& active, uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr int kMaskWords = kBatch / 32;
   static_assert(kThreads >= kPixels && kThreads >= kBatch, "block too small");
@@ -676,11 +752,12 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   // Prefetch the first round.
   uint32_t idx = cb + t;
   float4 m = make_float4(0.f, 0.f, 0.f, 0.f), c = m, k = m;
+  uint32_t cg = 0u;
   if (stager && idx < ce) {
-    uint32_t gid = sorted[idx].y;
-    m = pmean[gid];
-    c = pconic[gid];
-    k = pcolor[gid];
+    cg = sorted[idx].y;
+    m = pmean[cg];
+    c = pconic[cg];
+    k = pcolor[cg];
   }
   __syncthreads();
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
@@ -694,14 +771,15 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
       L.mean[t] = m;
       L.conic[t] = make_float4(c.x, c.y, c.z, thr);
       L.color[t] = k;
+      if (kRecord) L.gid[t] = cg;
     }
     float4 cm = m, cc = make_float4(c.x, c.y, c.z, thr);
     uint32_t nidx = batch + kBatch + t;
     if (stager && nidx < ce) {
-      uint32_t gid = sorted[nidx].y;
-      m = pmean[gid];
-      c = pconic[gid];
-      k = pcolor[gid];
+      cg = sorted[nidx].y;
+      m = pmean[cg];
+      c = pconic[cg];
+      k = pcolor[cg];
     }
     // Candidate pixels of record t.
     if (mine) {
@@ -757,6 +835,7 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
           float alpha = cd[q].alpha;
           if (alpha < (float)(1.0 / 255.0)) continue;
           float contribution = s.t * alpha;
+          if (kRecord) atomicMax(&record[L.gid[j[q]]], gs_f2b(contribution));
           s.r = s.r + cd[q].color.x * contribution;
           s.g = s.g + cd[q].color.y * contribution;
           s.b = s.b + cd[q].color.z * contribution;
@@ -823,12 +902,13 @@ GS_DEV void store_pixel(const FrameParams& fp, uint32_t x, uint32_t y, float r, 
 }
 
 // Light tiles (up to kLightMax records): one block per tile.
-template <int kEdge, int kBatch>
+template <int kEdge, int kBatch, bool kRecord>
 __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
     const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
     const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
-    uint16_t* __restrict__ color_out, float* __restrict__ depth_out) {
+    uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
+    uint32_t* __restrict__ record) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
   if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
   uint32_t tile = blockIdx.x;
@@ -837,8 +917,8 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
   if (end - begin > kLightMax) return;  // heavy unit
   uint32_t x, y;
   bool active;
-  PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch>(
-      fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active);
+  PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
+      fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
   if (fp.raster_stats) add_stats(state, s.checked, s.blended);
   if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
 }
@@ -914,14 +994,14 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
 }
 
 // Heavy units: one 1024-thread block per chunk task or heavy tile.
-template <int kEdge>
+template <int kEdge, bool kRecord>
 __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
     const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
     const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
     const uint2* __restrict__ units, float4* __restrict__ partials,
     float* __restrict__ partial_depth, uint16_t* __restrict__ color_out,
-    float* __restrict__ depth_out) {
+    float* __restrict__ depth_out, uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
   __shared__ RasterLds<kPixels, heavy_batch(kEdge)> lds;
   uint32_t u = blockIdx.x;
@@ -937,8 +1017,8 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
   }
   uint32_t x, y;
   bool active;
-  PixelSum s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge)>(
-      fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active);
+  PixelSum s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge), kRecord>(
+      fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
   if (fp.raster_stats) add_stats(state, s.checked, s.blended);
   if (threadIdx.x >= (uint32_t)kPixels) return;
   if (chunk) {
@@ -1033,7 +1113,7 @@ void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_coun
 
 // The raster stage: the heavy work list on s, then heavy units on aux
 // concurrently with light tiles on s, then the chunk composite.
-template <int kEdge, int kBatch>
+template <int kEdge, int kBatch, bool kRecord>
 hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
                          hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
   const FrameParams& fp = b.params;
@@ -1043,14 +1123,14 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   hipError_t e = hipEventRecord(fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
   if (e != hipSuccess) return e;
-  k_raster_heavy<kEdge><<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
+  k_raster_heavy<kEdge, kRecord><<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
       fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
-      b.chunk_partials, b.partial_depth, b.color_out, b.depth_out);
+      b.chunk_partials, b.partial_depth, b.color_out, b.depth_out, b.record);
   e = hipEventRecord(join, aux);
   if (e != hipSuccess) return e;
-  k_raster_light<kEdge, kBatch><<<b.max_tiles, light_threads(kEdge, kBatch), 0, s>>>(
+  k_raster_light<kEdge, kBatch, kRecord><<<b.max_tiles, light_threads(kEdge, kBatch), 0, s>>>(
       fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.color_out,
-      b.depth_out);
+      b.depth_out, b.record);
   e = hipStreamWaitEvent(s, join, 0);
   if (e != hipSuccess) return e;
   if (fp.chunk_size != 0)
@@ -1060,11 +1140,14 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   return hipGetLastError();
 }
 
+// The instrumented raster (contribution record) uses the 256-record batch.
 template <int kEdge>
 hipError_t launch_raster_edge(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
                               hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
-  if (b.raster_batch == 128) return launch_raster<kEdge, 128>(b, sorted, s, aux, fork, join);
-  return launch_raster<kEdge, 256>(b, sorted, s, aux, fork, join);
+  if (b.record) return launch_raster<kEdge, 256, true>(b, sorted, s, aux, fork, join);
+  if (b.raster_batch == 128)
+    return launch_raster<kEdge, 128, false>(b, sorted, s, aux, fork, join);
+  return launch_raster<kEdge, 256, false>(b, sorted, s, aux, fork, join);
 }
 
 }  // namespace
@@ -1081,7 +1164,15 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   uint32_t n_blocks = div_up(b.max_n, kBlock);
   if (n_blocks == 0) n_blocks = 1;
   mark(kStageProject);
-  k_project<<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj);
+  if (b.cull_net)
+    k_project<kModeCull><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, b.cull_net,
+                                                     nullptr, nullptr);
+  else if (b.record)
+    k_project<kModeCollect><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, nullptr,
+                                                        b.cull_feats, b.record);
+  else
+    k_project<kModePlain><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, nullptr,
+                                                      nullptr, nullptr);
   mark(kStageVisible);
 
   // Visible flags -> exclusive offsets -> compacted (depth key, ordinal).

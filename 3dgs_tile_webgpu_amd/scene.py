@@ -49,6 +49,8 @@ class Workload:
     anisotropic: bool = False
     near_fraction: float = 0.0
     description: str = ""
+    orbit: tuple[float, float] | None = None  # camera orbit radius range (multi-view)
+    views: int = 1
 
     def camera(self) -> PerspectiveCamera:
         return PerspectiveCamera(fov=60.0, aspect=self.width / self.height, near=0.1, far=1000.0,
@@ -66,7 +68,18 @@ WORKLOADS = {
     "6m_4k": Workload("6m_4k", 6_000_000, 3840, 2160, 60.0, 20.0, seed=20243, anisotropic=True,
                       near_fraction=0.02,
                       description="6M anisotropic Gaussians with near-plane clusters, 3840x2160"),
+    # Multi-view workloads: the 10-unit cube of the small scene seen from
+    # seeded orbits of radius 12-25 around the origin.
+    "2m_batch": Workload("2m_batch", 2_000_000, 1920, 1080, 18.0, 5.0, seed=20244,
+                         orbit=(12.0, 25.0), views=64,
+                         description="2M Gaussians, 64 orbit cameras of mixed resolution"),
+    "3m_cull": Workload("3m_cull", 3_000_000, 1920, 1080, 18.0, 5.0, seed=20245,
+                        orbit=(12.0, 25.0), views=64,
+                        description="3M Gaussians, 1920x1080 orbit views (512 train, 64 held out)"),
 }
+
+# Resolutions of the batched multi-camera workload.
+BATCH_RESOLUTIONS = [(854, 480), (1280, 720), (1920, 1080), (2560, 1440)]
 
 
 def spec_for(w: Workload, count: int | None = None) -> N.GsSceneSpec:

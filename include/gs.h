@@ -262,6 +262,55 @@ int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], double* frame_m
 uint32_t gs_capacity_next(uint32_t current, uint32_t demand, uint32_t maximum,
                           double growth, double headroom);
 
+/* Learned cull (no reference counterpart; the reference's nearest
+ * mechanisms are the opacity and subpixel culls, ProjectionStage.ts:533,
+ * 572-587). A 16-64-64-1 ReLU MLP with bf16 weights, run on the matrix
+ * cores inside projection, predicts the log of each visible Gaussian's
+ * largest alpha * T; with the cull enabled, Gaussians predicted below
+ * log(threshold) are dropped before duplication and sort, in gs_render and
+ * gs_render_batch alike. Parameters (GS_CULL_PARAMS f32): W1[64][16],
+ * b1[64], W2[64][64], b2[64], w3[64], b3. Features (GS_CULL_FEATURES f32):
+ * log pixel radius, log conic a, conic b / sqrt(a c), log conic c, opacity,
+ * log opacity, log view depth, object-space view direction (3), object-space
+ * position (3), log SH energy, x / width, y / height. */
+#define GS_CULL_PARAMS 5313
+#define GS_CULL_FEATURES 16
+/* He-uniform weights from seed, identity normalisation, threshold 1e-4,
+ * Adam state cleared. Required before the other gs_cull_* calls. */
+int gs_cull_init(gs_ctx* ctx, uint64_t seed);
+int gs_cull_set_params(gs_ctx* ctx, const float* params);
+int gs_cull_get_params(gs_ctx* ctx, float* params);
+/* Inputs are standardised as (f - mu) * inv_sigma before the network. */
+int gs_cull_set_normalization(gs_ctx* ctx, const float mu[16], const float inv_sigma[16]);
+int gs_cull_get_normalization(gs_ctx* ctx, float mu[16], float inv_sigma[16]);
+/* Switch the cull in projection on or off; threshold in (0, 1) on alpha*T
+ * (also the threshold the training loss weighs under-predictions around). */
+int gs_cull_enable(gs_ctx* ctx, int enabled, float threshold);
+/* Training set of up to `capacity` samples (cleared). */
+int gs_cull_reserve(gs_ctx* ctx, uint32_t capacity);
+/* Render `frame` on the single-camera path with the plain pipeline,
+ * chunking off and every blended alpha * T recorded per Gaussian, then
+ * append a hashed fraction `rate` of its visible Gaussians (features, log
+ * max alpha * T clamped to [-16, 0]). *state receives the frame's state;
+ * an overflowing frame appends nothing. */
+int gs_cull_collect(gs_ctx* ctx, const gs_frame* frame, float rate, uint32_t seed,
+                    gs_state* state);
+int gs_cull_dataset_size(gs_ctx* ctx, uint32_t* n);
+int gs_cull_read_dataset(gs_ctx* ctx, uint32_t first, uint32_t n, float* features,
+                         float* targets);
+/* Standardisation from the training set's feature means and deviations. */
+int gs_cull_fit_normalization(gs_ctx* ctx);
+/* `steps` Adam steps (beta 0.9 / 0.999) on minibatches of `batch` samples;
+ * loss mean(w (y - t)^2) with w = under_weight where the target is above
+ * the threshold and the prediction below it. *last_loss: the last step's. */
+int gs_cull_train(gs_ctx* ctx, uint32_t steps, uint32_t batch, float lr, float under_weight,
+                  uint64_t seed, float* last_loss);
+/* Predictions for n host feature rows (raw features), and the training
+ * loss gradient over n host samples (parameter order, no update). */
+int gs_cull_eval(gs_ctx* ctx, uint32_t n, const float* features, float* out);
+int gs_cull_gradient(gs_ctx* ctx, uint32_t n, const float* features, const float* targets,
+                     float under_weight, float* grad, float* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,9 +24,11 @@
 #include "gs_oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -241,7 +243,8 @@ bool projection_is_valid(const M4& p) {
 }
 
 // kernels/projectionHelpers.ts:118-176 (evaluate_gaussian_sh_rgb8e8)
-V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir) {
+V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir,
+               float* energy = nullptr) {
   float x = dir.x, y = dir.y, z = dir.z;
   uint32_t n = (degree + 1) * (degree + 1);
   const uint32_t* c = sh + (size_t)slot * n;
@@ -251,11 +254,15 @@ V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir) {
     float w = 0.28209479177387814f;
     color = {w * k.x, w * k.y, w * k.z};
   }
+  // energy: sum over the higher-band coefficients of |k|^2, in evaluation
+  // order (a learned-cull feature; no reference counterpart).
+  float e = 0.0f;
   auto add = [&](float w, uint32_t i) {
     V3 k = decode_sh(c[i]);
     color.x = color.x + w * k.x;
     color.y = color.y + w * k.y;
     color.z = color.z + w * k.z;
+    e = e + (k.x * k.x + k.y * k.y + k.z * k.z);
   };
   if (degree >= 1) {
     add(-0.4886025119029199f * y, 1);
@@ -280,6 +287,7 @@ V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir) {
     add(1.445305721320277f * z * (xx - yy), 14);
     add(-0.5900435899266435f * x * (xx - 3.0f * yy), 15);
   }
+  if (energy) *energy = e;
   return {gs_clampf(color.x + 0.5f, 0.0f, 1.0f), gs_clampf(color.y + 0.5f, 0.0f, 1.0f),
           gs_clampf(color.z + 0.5f, 0.0f, 1.0f)};
 }
@@ -468,6 +476,9 @@ struct gs_oracle_result {
   std::vector<uint64_t> depth_records, emitted, sorted;
   std::vector<float> image, depth_image, final_t;
   std::vector<uint16_t> image_half;
+  // Learned-cull training data (scene.cull_outputs).
+  std::vector<float> cull_features;
+  std::unique_ptr<std::atomic<uint32_t>[]> max_contribution;
 };
 
 namespace {
@@ -541,11 +552,34 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
   int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmax.y / ts), 0.0f, max_ty));
   if (s.boundary_axis == 1) { tmin_x = 0; tmax_x = (int)tiles_x - 1; }
   if (s.boundary_axis == 2) { tmin_y = 0; tmax_y = (int)tiles_y - 1; }
-  V3 color = evaluate_sh(s.sh, slot, s.sh_degree, view_dir);
+  float sh_energy = 0.0f;
+  V3 color = evaluate_sh(s.sh, slot, s.sh_degree, view_dir, &sh_energy);
   uint32_t count = count_tiles(center, conic, thr, V2{radius_x, radius_y}, tmin_x,
                                tmin_y, tmax_x, tmax_y, V2{s.width, s.height},
                                s.boundary_axis, ts);
   if (count == 0) return;
+  if (s.cull_outputs) {
+    // Learned-cull features: log pixel radius, log conic diagonal, conic
+    // correlation, opacity, log opacity, log depth, view direction, object-
+    // space position, log SH energy, normalised screen position.
+    float* f = &r.cull_features[(size_t)gid * 16];
+    f[0] = gs_logf(radius_x > radius_y ? radius_x : radius_y);
+    f[1] = gs_logf(conic.x);
+    f[2] = conic.y / gs_sqrtf(conic.x * conic.z);
+    f[3] = gs_logf(conic.z);
+    f[4] = opacity;
+    f[5] = gs_logf(opacity);
+    f[6] = gs_logf(depth);
+    f[7] = view_dir.x;
+    f[8] = view_dir.y;
+    f[9] = view_dir.z;
+    f[10] = local.x;
+    f[11] = local.y;
+    f[12] = local.z;
+    f[13] = gs_logf(sh_energy + 1e-6f);
+    f[14] = center.x / s.width;
+    f[15] = center.y / s.height;
+  }
   pm[0] = center.x;
   pm[1] = center.y;
   pm[2] = depth;
@@ -631,6 +665,12 @@ PixelAccum blend_range(const gs_oracle_result& r, uint32_t begin, uint32_t end,
     float alpha = gs_clampf(mean[3] * gs_expf(power), 0.0f, 0.99f);
     if (alpha < (float)(1.0 / 255.0)) continue;
     float contribution = a.t * alpha;
+    if (s.cull_outputs) {
+      std::atomic<uint32_t>& m = r.max_contribution[gid];
+      uint32_t bits = gs_f2b(contribution), old = m.load(std::memory_order_relaxed);
+      while (old < bits && !m.compare_exchange_weak(old, bits, std::memory_order_relaxed)) {
+      }
+    }
     a.c[0] = a.c[0] + color[0] * contribution;
     a.c[1] = a.c[1] + color[1] * contribution;
     a.c[2] = a.c[2] + color[2] * contribution;
@@ -728,6 +768,11 @@ extern "C" gs_oracle_result* gs_oracle_run(const gs_oracle_scene* sp) {
   r.projected_conic.assign((size_t)n * 4, 0.0f);
   r.projected_color.assign((size_t)n * 4, 0.0f);
   r.tile_counts.assign(n, 0);
+  if (s.cull_outputs) {
+    r.cull_features.assign((size_t)n * 16, 0.0f);
+    r.max_contribution.reset(new std::atomic<uint32_t>[n ? n : 1]);
+    for (uint32_t i = 0; i < n; i++) r.max_contribution[i].store(0u);
+  }
   parallel_for(threads, n, [&](size_t b, size_t e, uint32_t) {
     for (size_t g = b; g < e; g++) project_one(s, tiles_x, tiles_y, (uint32_t)g, r);
   });
@@ -874,6 +919,13 @@ extern "C" size_t gs_oracle_get(const gs_oracle_result* r, const char* name, voi
   if (n == "image_f16") return GS_VEC(image_half);
   if (n == "depth_f32") return GS_VEC(depth_image);
   if (n == "final_t") return GS_VEC(final_t);
+  if (n == "cull_features") return GS_VEC(cull_features);
+  if (n == "max_contribution") {
+    size_t cnt = r->scene.cull_outputs ? r->scene.active_count : 0;
+    std::vector<uint32_t> v(cnt);
+    for (size_t i = 0; i < cnt; i++) v[i] = r->max_contribution[i].load();
+    return copy_out(v.data(), v.size() * 4, dst, bytes);
+  }
   if (n == "depth_records") return copy_pairs(r->depth_records, dst, bytes);
   if (n == "emitted_records") return copy_pairs(r->emitted, dst, bytes);
   if (n == "sorted_records") return copy_pairs(r->sorted, dst, bytes);

@@ -46,6 +46,9 @@ typedef struct gs_oracle_scene {
   float background[4];
   uint32_t threads;               /* worker threads (0 = all) */
   uint32_t tile_size;             /* tile edge 8, 16 or 32; 0 = 16 (constants.ts:1) */
+  uint32_t cull_outputs;          /* also produce cull_features (n * 16 f32) and
+                                     max_contribution (n u32: binary32 bits of each
+                                     Gaussian's largest alpha * T) */
 } gs_oracle_scene;
 
 typedef struct gs_oracle_counts {

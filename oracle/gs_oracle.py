@@ -28,7 +28,7 @@ class OracleScene(C.Structure):
         ("filter_variance", f32), ("boundary_axis", u32), ("subpixel_culling", u32),
         ("antialias_compensated", u32), ("depth_mode", u32), ("raster_chunk_size", u32),
         ("capacity", u32), ("transmittance_threshold", f32), ("background", f32 * 4),
-        ("threads", u32), ("tile_size", u32),
+        ("threads", u32), ("tile_size", u32), ("cull_outputs", u32),
     ]
 
 
@@ -97,6 +97,7 @@ class OracleResult:
         "depth_records": (np.uint32, 2), "emitted_records": (np.uint32, 2),
         "sorted_records": (np.uint32, 2), "image_f32": (np.float32, 4),
         "image_f16": (np.uint16, 4), "depth_f32": (np.float32, 0), "final_t": (np.float32, 0),
+        "cull_features": (np.float32, 16), "max_contribution": (np.uint32, 0),
     }
 
     def __init__(self, handle, width, height, keep):
@@ -134,7 +135,7 @@ class OracleResult:
 
 def run_frame(scene, frame, objects, *, capacity, subpixel=True, compensated=True,
               depth_mode=0, chunk_size=8192, transmittance=1e-4, background=(0, 0, 0, 0),
-              threads=0, tile_size=16) -> OracleResult:
+              threads=0, tile_size=16, cull_outputs=False) -> OracleResult:
     """Run the oracle on a PackedScene with the exact f32 uniforms of `frame`
     (a GsFrame) and `objects` ((k, 40) f32 object table)."""
     keep = [np.ascontiguousarray(scene.means, np.float32),
@@ -168,6 +169,7 @@ def run_frame(scene, frame, objects, *, capacity, subpixel=True, compensated=Tru
     s.background = (f32 * 4)(*background)
     s.threads = threads
     s.tile_size = tile_size
+    s.cull_outputs = 1 if cull_outputs else 0
     h = lib().gs_oracle_run(C.byref(s))
     if not h:
         raise ValueError("oracle rejected the scene")

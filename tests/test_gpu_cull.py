@@ -1,0 +1,172 @@
+"""GPU checks of the learned cull.
+
+- Training data: the instrumented frame's features and per-Gaussian max
+  alpha * T equal the oracle's bit for bit (chunking off, as collected).
+- Network: the matrix-core forward pass and the backward kernels agree with
+  the CPU reference (oracle/gs_cull_ref.py, itself pinned by finite
+  differences) at the same bf16 rounding points: predictions within 2e-3,
+  gradients within 1% (relative L2 per parameter block).
+- Pipeline: a frame with the cull on is bit-identical to the plain oracle
+  frame of the surviving Gaussians -- the cull only removes Gaussians.
+- Training: the trained network removes a large share of the Gaussians and
+  keys on held-out views at PSNR >= 45 dB against the plain frame.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import gs_cull_ref as R
+from test_gpu_pipeline import oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(gs):
+    return gs.workload_scene("small", count=60_000)
+
+
+def test_collected_features_and_record_match_oracle(gs, oracle, gpu_available, small):
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    p = gs.gaussian_pass(cam, small, {"intersectionCapacity": 1 << 21})
+    cull = gs.LearnedCull(p, seed=1)
+    cull.reserve(1 << 17)
+    st = cull.collect(cam, 640, 360, rate=1.0, seed=7)
+    assert st.overflow == 0
+    feats_gpu = p.read_debug("cull_features").view(np.float32).reshape(-1, 16)
+    rec_gpu = p.read_debug("contribution_record")
+    f = gs.frame_for(cam, 640, 360, p.options)
+    ref = oracle.run_frame(small, f, p.object_frames(), capacity=p.capacity, chunk_size=0,
+                           cull_outputs=True)
+    vis = ref.get("projected_mean")[:, 3] > 0
+    np.testing.assert_array_equal(feats_gpu[vis].view(np.uint32),
+                                  ref.get("cull_features")[vis].view(np.uint32))
+    np.testing.assert_array_equal(rec_gpu, ref.get("max_contribution"))
+    # The training set holds every visible Gaussian once at rate 1.
+    assert cull.dataset_size() == int(vis.sum())
+    df, dt = cull.read_dataset()
+    order = np.lexsort(df.T[::-1])
+    want = ref.get("cull_features")[vis]
+    np.testing.assert_array_equal(df[order], want[np.lexsort(want.T[::-1])])
+    rec = ref.get("max_contribution")[vis].view(np.float32).astype(np.float64)
+    t_ref = np.clip(np.log(np.maximum(rec, 1e-7)), -16, 0)
+    np.testing.assert_allclose(np.sort(dt), np.sort(t_ref), atol=2e-6)
+    p.dispose()
+
+
+def reference_setup(rng, n):
+    params = R.random_params(rng)
+    mu = rng.normal(0, 1, 16).astype(np.float32)
+    inv = rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    feats = (mu + rng.normal(0, 1, (n, 16)) / inv).astype(np.float32)
+    return params, mu, inv, feats
+
+
+def test_forward_matches_reference(gs, gpu_available, small):
+    rng = np.random.default_rng(11)
+    params, mu, inv, feats = reference_setup(rng, 5000)
+    p = gs.gaussian_pass(gs.WORKLOADS["small"].camera(), small)
+    cull = gs.LearnedCull(p)
+    cull.set_params(params)
+    cull.set_normalization(mu, inv)
+    got = cull.predict(feats)
+    want = R.forward(params, mu, inv, feats)["y"]
+    err = np.abs(got - want)
+    print(f"forward: max|d|={err.max():.2e} over |y| up to {np.abs(want).max():.1f}")
+    assert err.max() <= 2e-3 * max(1.0, np.abs(want).max() / 10)
+    p.dispose()
+
+
+def test_gradient_matches_reference(gs, gpu_available, small):
+    rng = np.random.default_rng(12)
+    params, mu, inv, feats = reference_setup(rng, 1000)
+    targets = rng.uniform(-16, 0, 1000).astype(np.float32)
+    p = gs.gaussian_pass(gs.WORKLOADS["small"].camera(), small)
+    cull = gs.LearnedCull(p)
+    cull.set_params(params)
+    cull.set_normalization(mu, inv)
+    g, loss = cull.gradient(feats, targets, under_weight=4.0)
+    lw, gw = R.loss_and_grad(params, mu, inv, feats, targets, 4.0, math.log(1e-4))
+    assert loss == pytest.approx(lw, rel=1e-3)
+    blocks = {"W1": (R.OFF_W1, R.OFF_B1), "b1": (R.OFF_B1, R.OFF_W2), "W2": (R.OFF_W2, R.OFF_B2),
+              "b2": (R.OFF_B2, R.OFF_W3), "w3": (R.OFF_W3, R.OFF_B3), "b3": (R.OFF_B3, R.PARAMS)}
+    for name, (a, b) in blocks.items():
+        rel = np.linalg.norm(g[a:b] - gw[a:b]) / max(np.linalg.norm(gw[a:b]), 1e-12)
+        print(f"gradient {name}: relative L2 error {rel:.2e}")
+        assert rel <= 1e-2, name
+    p.dispose()
+
+
+def test_cull_equals_plain_on_survivors(gs, oracle, gpu_available, small):
+    """With the cull on, the frame is the plain frame of the survivors."""
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    p = gs.gaussian_pass(cam, small, {"intersectionCapacity": 1 << 21})
+    p.render(800, 450)
+    plain_mean = p.read_debug("projected_mean")
+    cull = gs.LearnedCull(p, seed=5)
+    rng = np.random.default_rng(5)
+    params = R.random_params(rng)
+    params[R.OFF_B3] = 0.0
+    cull.set_params(params)
+    # Standardise on the visible Gaussians' oracle features so the random net
+    # spreads its predictions, then shift its output so half fall below.
+    f = gs.frame_for(cam, 800, 450, p.options)
+    ref_feats = oracle.run_frame(small, f, p.object_frames(), capacity=p.capacity, chunk_size=0,
+                                 cull_outputs=True)
+    vis = ref_feats.get("projected_mean")[:, 3] > 0
+    fv = ref_feats.get("cull_features")[vis]
+    cull.set_normalization(fv.mean(0), 1.0 / np.maximum(fv.std(0), 1e-3))
+    pred = cull.predict(fv)
+    params[R.OFF_B3] = math.log(1e-3) - float(np.median(pred))  # median lands on the threshold
+    cull.set_params(params)
+    cull.enable(threshold=1e-3)
+    p.render(800, 450)
+    mean = p.read_debug("projected_mean")
+    culled = (plain_mean[:, 3] > 0) & (mean[:, 3] == 0)
+    frac = culled.sum() / max(1, (plain_mean[:, 3] > 0).sum())
+    print(f"culled {culled.sum()} of {(plain_mean[:, 3] > 0).sum()} visible ({frac:.1%})")
+    assert 0.2 < frac < 0.8
+    survivors = np.flatnonzero(~culled).astype(np.uint32)
+    sub = gs.PackedScene(small.means, small.scales_opacity, small.rotations, small.sh, 3, survivors)
+    ref = oracle.run_frame(sub, f, p.object_frames(), capacity=p.capacity, chunk_size=8192)
+    np.testing.assert_array_equal(p.read_image().view(np.uint16), ref.get("image_f16"))
+    assert p.read_state().intersections == ref.counts.count
+    p.dispose()
+
+
+def test_training_culls_on_held_out_views(gs, gpu_available):
+    """Train on 24 orbit views of a dense 300k scene, evaluate on 4 others:
+    the trained cull removes at least a fifth of the keys at >= 45 dB."""
+    scene = gs.generate(gs.spec_for(gs.WORKLOADS["3m_cull"], count=300_000))
+    views = gs.orbit_cameras(28, seed=99, radius=(12.0, 25.0), resolutions=[(640, 360)])
+    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 21})
+    cull = gs.LearnedCull(b, seed=3)
+    cull.reserve(4_000_000)
+    for k, (cam, (w, h)) in enumerate(views[:24]):
+        cull.collect(cam, w, h, rate=0.5, seed=k)
+    n = cull.dataset_size()
+    assert n > 100_000
+    cull.fit_normalization()
+    loss0 = cull.train(1, batch=32768, seed=1)
+    loss = cull.train(400, batch=32768, lr=2e-3, seed=2)
+    print(f"training set {n}, loss {loss0:.3f} -> {loss:.3f}")
+    assert loss < 0.5 * loss0
+    cams = [c for c, _ in views[24:]]
+    sizes = [s for _, s in views[24:]]
+    b.render(cams, sizes)
+    plain = [b.read_image(v).copy() for v in range(4)]
+    keys_plain = sum(s.intersections for s in b.read_states())
+    cull.enable(1e-4)
+    b.render(cams, sizes)
+    keys_cull = sum(s.intersections for s in b.read_states())
+    ps = [gs.psnr(b.read_image(v), plain[v]) for v in range(4)]
+    print(f"keys {keys_plain} -> {keys_cull} ({1 - keys_cull / keys_plain:.1%} removed), "
+          f"PSNR min {min(ps):.1f} mean {np.mean(ps):.1f} dB")
+    assert keys_cull < 0.8 * keys_plain
+    assert min(ps) >= 45.0
+    b.dispose()
