@@ -1615,8 +1615,8 @@ extern "C" int gs_cull_train(gs_ctx* ctx, uint32_t steps, uint32_t batch, float 
   if (!ctx || batch == 0) return GS_E_RANGE;
   int rc = cull_require(ctx);
   if (rc) return rc;
-  if (!(lr > 0.0f && finite(lr)) || !(under_weight >= 1.0f && finite(under_weight)))
-    return fail(ctx, GS_E_RANGE, "learning rate must be positive, under-weight at least 1");
+  if (!(lr > 0.0f && finite(lr)) || !(under_weight >= 0.0f && finite(under_weight)))
+    return fail(ctx, GS_E_RANGE, "learning rate must be positive, under-weight nonnegative");
   uint32_t n = 0;
   if ((rc = gs_cull_dataset_size(ctx, &n))) return rc;
   if (n == 0) return fail(ctx, GS_E_STATE, "empty training set");

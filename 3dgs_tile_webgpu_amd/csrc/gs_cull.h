@@ -14,6 +14,7 @@
 #ifndef GS_CULL_H
 #define GS_CULL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "gs_math.h"
@@ -43,6 +44,8 @@ struct CullNet {
   float log_threshold;  // cull when the prediction is below this
   float mu[kCullIn], inv_sigma[kCullIn];  // feature standardisation
 };
+static_assert(offsetof(CullNet, w2) % 16 == 0 && offsetof(CullNet, w1) % 16 == 0,
+              "the weight matrices are staged 16 bytes at a time");
 
 // binary32 -> bfloat16, round to nearest even (NaN -> quiet NaN).
 GS_HD inline uint16_t gs_f2bf(float f) {
@@ -54,10 +57,11 @@ GS_HD inline uint16_t gs_f2bf(float f) {
 GS_HD inline float gs_bf2f(uint16_t h) { return gs_b2f((uint32_t)h << 16); }
 
 // The 16 features of a projected Gaussian: footprint (log pixel radius,
-// log conic diagonal, conic correlation), opacity (linear and log), log
-// view depth, the object-space view direction, the object-space position,
-// log SH energy (sum of squared higher-band coefficients) and the
-// normalised screen position.
+// log conic diagonal, conic correlation), opacity, the position along the
+// viewing ray (object-space position . view direction, a proxy for how
+// deep in the scene the Gaussian sits), log view depth, the object-space
+// view direction, the object-space position, log SH energy (sum of squared
+// higher-band coefficients) and the normalised screen position.
 GS_HD inline void cull_features(float f[kCullIn], float rx, float ry, const float conic[3],
                                 float opacity, float depth, const float dir[3],
                                 const float pos[3], float sh_energy, float cx, float cy,
@@ -67,7 +71,7 @@ GS_HD inline void cull_features(float f[kCullIn], float rx, float ry, const floa
   f[2] = conic[1] / gs_sqrtf(conic[0] * conic[2]);
   f[3] = gs_logf(conic[2]);
   f[4] = opacity;
-  f[5] = gs_logf(opacity);
+  f[5] = pos[0] * dir[0] + pos[1] * dir[1] + pos[2] * dir[2];
   f[6] = gs_logf(depth);
   f[7] = dir[0];
   f[8] = dir[1];
@@ -111,8 +115,9 @@ __device__ __forceinline__ gs_bf16x4 lds4_col(const uint16_t* p, int stride) {
   return v;
 }
 
-// The network's weights staged in LDS for a block.
-struct CullLdsNet {
+// The network's weights staged in LDS for a block (16-byte aligned: the
+// matrix fragments are read 8 bytes at a time).
+struct alignas(16) CullLdsNet {
   uint16_t w1[kCullHidden * kCullIn];
   uint16_t w2[kCullHidden * kRow];  // [out][in], rows padded
   float w3[kCullHidden];
@@ -121,21 +126,17 @@ struct CullLdsNet {
   float mu[kCullIn], inv_sigma[kCullIn];
 };
 
-// Per-wave scratch of the inference forward pass: standardised inputs,
-// then (over the same bytes) the hidden activations, and the outputs.
-struct CullLdsWave {
-  union {
-    uint16_t x[64 * kCullIn];
-    uint16_t a1[64 * kRow];
-  };
-  float y[64];
-};
-
-// Cooperative copy of the network into LDS (all threads of the block).
+// Cooperative copy of the network into LDS (all threads of the block), the
+// weight matrices 16 bytes at a time (the CullNet comes from a 256-byte
+// aligned allocation; w1 and w2 sit at 16-byte multiples).
 __device__ inline void cull_stage_net(const CullNet* __restrict__ net, CullLdsNet& L) {
-  for (int t = threadIdx.x; t < kCullHidden * kCullIn; t += blockDim.x) L.w1[t] = net->w1[t];
-  for (int t = threadIdx.x; t < kCullHidden * kCullHidden; t += blockDim.x)
-    L.w2[(t / kCullHidden) * kRow + (t % kCullHidden)] = net->w2[t];
+  const uint4* w1 = reinterpret_cast<const uint4*>(net->w1);
+  const uint4* w2 = reinterpret_cast<const uint4*>(net->w2);
+  constexpr int kW1Chunks = kCullHidden * kCullIn / 8, kW2Chunks = kCullHidden * kCullHidden / 8;
+  for (int t = threadIdx.x; t < kW1Chunks; t += blockDim.x)
+    reinterpret_cast<uint4*>(L.w1)[t] = w1[t];
+  for (int t = threadIdx.x; t < kW2Chunks; t += blockDim.x)  // 8 chunks per row
+    *reinterpret_cast<uint4*>(&L.w2[(t >> 3) * kRow + (t & 7) * 8]) = w2[t];
   for (int t = threadIdx.x; t < kCullHidden; t += blockDim.x) {
     L.w3[t] = gs_bf2f(net->w3[t]);
     L.b1[t] = net->b1[t];
@@ -167,26 +168,23 @@ __device__ inline float cull_forward_wave(const CullLdsNet& N, uint16_t* x, uint
 #pragma unroll
   for (int k = 0; k < kCullIn; k++) x[l * kCullIn + k] = gs_f2bf((f[k] - N.mu[k]) * N.inv_sigma[k]);
   wave_sync();
-  // Layer 1: Z1 = X W1^T (64 x 64), K = 16.
-  gs_f32x4 z1[4][4];
+  // Layer 1: Z1 = X W1^T (64 x 64), K = 16. The A fragments of all four
+  // row tiles are taken into registers first (the x rows may share bytes
+  // with a1), then each row tile's products go straight to a1.
+  gs_bf16x4 xa[4];
 #pragma unroll
-  for (int mt = 0; mt < 4; mt++) {
-    gs_bf16x4 a = lds4(&x[(16 * mt + i) * kCullIn + 4 * g]);
-#pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-      gs_bf16x4 b = lds4(&N.w1[(16 * nt + i) * kCullIn + 4 * g]);
-      z1[mt][nt] = mfma16(a, b, gs_f32x4{0.f, 0.f, 0.f, 0.f});
-    }
-  }
-  wave_sync();  // the x rows are overwritten by a1 below
+  for (int mt = 0; mt < 4; mt++) xa[mt] = lds4(&x[(16 * mt + i) * kCullIn + 4 * g]);
+  wave_sync();
 #pragma unroll
   for (int mt = 0; mt < 4; mt++)
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
+      gs_bf16x4 b = lds4(&N.w1[(16 * nt + i) * kCullIn + 4 * g]);
+      gs_f32x4 z = mfma16(xa[mt], b, gs_f32x4{0.f, 0.f, 0.f, 0.f});
       float bias = N.b1[16 * nt + i];
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        float v = z1[mt][nt][r] + bias;
+        float v = z[r] + bias;
         a1[(16 * mt + 4 * g + r) * kRow + 16 * nt + i] = gs_f2bf(v > 0.0f ? v : 0.0f);
       }
     }
@@ -230,6 +228,82 @@ __device__ inline float cull_forward_wave(const CullLdsNet& N, uint16_t* x, uint
     }
   wave_sync();
   return y[l];
+}
+
+// Inference forward pass of one wave, computed transposed so that no
+// activation goes through LDS: Z1^T = W1 X^T (M = hidden, N = samples, K =
+// inputs) leaves each lane holding hidden units 16mt + 4g + r of sample
+// 16nt + i, which is exactly the B operand of Z2^T = W2 A1^T for K tile
+// kt = mt. Layer 3 sums over the hidden units in registers and across the
+// four lane groups. Lane l holds the raw features of sample l (zeros for
+// idle lanes) and gets that sample's prediction; xs is the wave's 64 x 16
+// bf16 LDS rows for the standardised inputs. Same bf16 operands and fp32
+// accumulation as cull_forward_wave.
+__device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
+                                        const float f[kCullIn]) {
+  const int l = __lane_id();
+  const int i = l & 15, g = l >> 4;
+  uint32_t packed[kCullIn / 2];
+#pragma unroll
+  for (int k = 0; k < kCullIn / 2; k++)
+    packed[k] = (uint32_t)gs_f2bf((f[2 * k] - N.mu[2 * k]) * N.inv_sigma[2 * k]) |
+                ((uint32_t)gs_f2bf((f[2 * k + 1] - N.mu[2 * k + 1]) * N.inv_sigma[2 * k + 1])
+                 << 16);
+  uint4* row = reinterpret_cast<uint4*>(&xs[l * kCullIn]);
+  row[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+  row[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+  wave_sync();
+  gs_bf16x4 xb[4];  // B operand: X[16 nt + i][4g .. 4g + 3]
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) xb[nt] = lds4(&xs[(16 * nt + i) * kCullIn + 4 * g]);
+  wave_sync();  // the rows may be rewritten by the next call
+  // Layer 1 (transposed), bias, ReLU, bf16: the B operands of layer 2.
+  gs_bf16x4 a1t[4][4];  // [kt][nt]
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    gs_bf16x4 a = lds4(&N.w1[(16 * mt + i) * kCullIn + 4 * g]);
+    float4 b1 = *reinterpret_cast<const float4*>(&N.b1[16 * mt + 4 * g]);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      gs_f32x4 z = mfma16(a, xb[nt], gs_f32x4{0.f, 0.f, 0.f, 0.f});
+      float h0 = z[0] + b1.x, h1 = z[1] + b1.y, h2 = z[2] + b1.z, h3 = z[3] + b1.w;
+      gs_bf16x4 v;
+      v[0] = (short)gs_f2bf(h0 > 0.0f ? h0 : 0.0f);
+      v[1] = (short)gs_f2bf(h1 > 0.0f ? h1 : 0.0f);
+      v[2] = (short)gs_f2bf(h2 > 0.0f ? h2 : 0.0f);
+      v[3] = (short)gs_f2bf(h3 > 0.0f ? h3 : 0.0f);
+      a1t[mt][nt] = v;
+    }
+  }
+  // Layer 2 (transposed) with layer 3 folded in.
+  float yp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    gs_bf16x4 w[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) w[kt] = lds4(&N.w2[(16 * mt + i) * kRow + 16 * kt + 4 * g]);
+    float4 b2 = *reinterpret_cast<const float4*>(&N.b2[16 * mt + 4 * g]);
+    float4 w3 = *reinterpret_cast<const float4*>(&N.w3[16 * mt + 4 * g]);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      gs_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < 4; kt++) acc = mfma16(w[kt], a1t[kt][nt], acc);
+      float v0 = acc[0] + b2.x, v1 = acc[1] + b2.y, v2 = acc[2] + b2.z, v3 = acc[3] + b2.w;
+      yp[nt] += (v0 > 0.0f ? v0 : 0.0f) * w3.x;
+      yp[nt] += (v1 > 0.0f ? v1 : 0.0f) * w3.y;
+      yp[nt] += (v2 > 0.0f ? v2 : 0.0f) * w3.z;
+      yp[nt] += (v3 > 0.0f ? v3 : 0.0f) * w3.w;
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    yp[nt] += __shfl_xor(yp[nt], 16, 64);
+    yp[nt] += __shfl_xor(yp[nt], 32, 64);
+  }
+  // Sample l = 16 g + i is column i of N tile nt = g.
+  float y = g == 0 ? yp[0] : (g == 1 ? yp[1] : (g == 2 ? yp[2] : yp[3]));
+  return y + N.b3;
 }
 
 #endif  // __HIPCC__

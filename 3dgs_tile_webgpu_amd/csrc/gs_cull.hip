@@ -4,10 +4,11 @@
 // writes each visible Gaussian's 16 features, the raster raises a
 // per-Gaussian record to every alpha * T it blends, and k_cull_collect
 // appends a hashed sample of (features, log max alpha * T) pairs. Training
-// minimises a weighted squared error of the log prediction (under-
-// predictions of Gaussians above the cull threshold weigh more, since
-// culling those costs image quality) with Adam; the forward and backward
-// products run on the matrix cores, one wave per 64 samples.
+// minimises a weighted squared error of the log prediction with Adam: an
+// under-prediction of a Gaussian above the cull threshold weighs
+// 1 + lambda * (its max alpha * T), since culling it costs image quality in
+// proportion to that contribution. The forward and backward products run
+// on the matrix cores, one wave per 64 samples.
 #include "gs_cull.h"
 #include "gs_internal.h"
 
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void k_cull_moments(const float* __restrict__ 
   if (threadIdx.x < 2 * kCullIn) atomicAdd(&out[threadIdx.x], part[threadIdx.x]);
 }
 
-struct TrainWave {
+struct alignas(16) TrainWave {
   uint16_t x[64 * kCullIn];
   uint16_t a1[64 * kRow];
   uint16_t dz[64 * kRow];  // dZ2, later dZ1 (bf16)
@@ -98,7 +99,7 @@ struct TrainWave {
   float dy[64];
 };
 
-struct TrainLds {
+struct alignas(16) TrainLds {
   CullLdsNet net;
   uint16_t w2t[kCullHidden * kRow];  // W2 transposed: [in][out]
   union {
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_cull_train(
     gs_f32x4 z2[4][4];
     float y = cull_forward_wave<true>(L.net, W.x, W.a1, W.y, f, z2);
     float d = valid ? y - t : 0.0f;
-    float wgt = (t > log_thr && y < t) ? under_weight : 1.0f;
+    float wgt = (t > log_thr && y < t) ? 1.0f + under_weight * gs_expf(t) : 1.0f;
     loss_acc += wgt * d * d;
     float dy = 2.0f * wgt * d * inv_batch;
     gb3 += dy;
@@ -302,15 +303,14 @@ __global__ __launch_bounds__(256) void k_cull_eval(const float* __restrict__ fea
                                                    const CullNet* __restrict__ net,
                                                    float* __restrict__ out) {
   __shared__ CullLdsNet lnet;
-  __shared__ CullLdsWave lwave[4];
+  __shared__ __align__(16) uint16_t lx[4][64 * kCullIn];
   cull_stage_net(net, lnet);
   __syncthreads();
   uint32_t j = blockIdx.x * 256 + threadIdx.x;
   float f[kCullIn];
 #pragma unroll
   for (int k = 0; k < kCullIn; k++) f[k] = j < n ? feats[(size_t)j * kCullIn + k] : 0.0f;
-  CullLdsWave& w = lwave[threadIdx.x >> 6];
-  float y = cull_forward_wave<false>(lnet, w.x, w.a1, w.y, f, nullptr);
+  float y = cull_infer_wave(lnet, lx[threadIdx.x >> 6], f);  // the projection's forward pass
   if (j < n) out[j] = y;
 }
 

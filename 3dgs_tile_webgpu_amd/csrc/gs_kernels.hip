@@ -235,47 +235,7 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   return o;
 }
 
-// Projection of every active Gaussian. The learned-cull variant runs the
-// MLP over each wave's visible Gaussians (matrix cores, weights in LDS) and
-// drops those predicted below the threshold as if they touched no tile;
-// the collect variant stores the features and clears the per-Gaussian
-// contribution record of the instrumented raster.
-template <int kMode>
-__global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
-                                                    DeviceState* __restrict__ state,
-                                                    SceneView sc, ProjectedView out,
-                                                    const CullNet* __restrict__ net,
-                                                    float* __restrict__ feats,
-                                                    uint32_t* __restrict__ record) {
-  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  if (gid == 0) {
-    state->checked = 0ull;
-    state->blended = 0ull;
-  }
-  float feat[kCullIn];
-  ProjOut o;
-  o.kind = kProjNone;
-  if (gid < fp.active_count) o = project_gaussian<kMode>(fp, sc, gid, feat);
-  if constexpr (kMode == kModeCull) {
-    __shared__ CullLdsNet lnet;
-    __shared__ CullLdsWave lwave[kBlock / 64];
-    cull_stage_net(net, lnet);
-    __syncthreads();
-    bool visible = o.kind == kProjVisible;
-    if (__ballot(visible) != 0ull) {  // wave-uniform
-      if (!visible)
-        for (int k = 0; k < kCullIn; k++) feat[k] = 0.0f;
-      CullLdsWave& w = lwave[threadIdx.x >> 6];
-      float y = cull_forward_wave<false>(lnet, w.x, w.a1, w.y, feat, nullptr);
-      if (visible && y < lnet.log_threshold) o.kind = kProjNone;
-    }
-  }
-  if (gid >= fp.active_count) return;
-  if constexpr (kMode == kModeCollect) {
-    record[gid] = 0u;
-    if (o.kind == kProjVisible)
-      for (int k = 0; k < kCullIn; k++) feats[(size_t)gid * kCullIn + k] = feat[k];
-  }
+GS_DEV void store_projection(const ProjectedView& out, uint32_t gid, const ProjOut& o) {
   if (o.kind == kProjVisible) {
     out.mean[gid] = o.mean;
     out.conic[gid] = o.conic;
@@ -285,6 +245,62 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
     out.tile_counts[gid] = 0u;
     out.mean[gid] = o.kind == kProjSubpixel ? o.mean : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
+}
+
+// Projection of every active Gaussian; the collect variant also stores the
+// features and clears the per-Gaussian contribution record of the
+// instrumented raster.
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
+                                                    DeviceState* __restrict__ state,
+                                                    SceneView sc, ProjectedView out,
+                                                    float* __restrict__ feats,
+                                                    uint32_t* __restrict__ record) {
+  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  if (gid == 0) {
+    state->checked = 0ull;
+    state->blended = 0ull;
+  }
+  if (gid >= fp.active_count) return;
+  float feat[kCullIn];
+  ProjOut o = project_gaussian<kMode>(fp, sc, gid, feat);
+  if constexpr (kMode == kModeCollect) {
+    record[gid] = 0u;
+    if (o.kind == kProjVisible)
+      for (int k = 0; k < kCullIn; k++) feats[(size_t)gid * kCullIn + k] = feat[k];
+  }
+  store_projection(out, gid, o);
+}
+
+// The learned-cull projection: after projecting, every wave with a visible
+// Gaussian runs the MLP over its 64 lanes (matrix cores, weights staged in
+// LDS per block) and drops those predicted below the threshold as if they
+// touched no tile.
+__global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
+                                                         DeviceState* __restrict__ state,
+                                                         SceneView sc, ProjectedView out,
+                                                         const CullNet* __restrict__ net) {
+  __shared__ CullLdsNet lnet;
+  __shared__ __align__(16) uint16_t lx[kBlock / 64][64 * kCullIn];
+  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  if (gid == 0) {
+    state->checked = 0ull;
+    state->blended = 0ull;
+  }
+  float feat[kCullIn];
+  ProjOut o;
+  o.kind = kProjNone;
+  if (gid < fp.active_count) o = project_gaussian<kModeCull>(fp, sc, gid, feat);
+  cull_stage_net(net, lnet);
+  __syncthreads();
+  bool visible = o.kind == kProjVisible;
+  if (__ballot(visible) != 0ull) {  // wave-uniform
+    if (!visible)
+      for (int k = 0; k < kCullIn; k++) feat[k] = 0.0f;
+    float y = cull_infer_wave(lnet, lx[threadIdx.x >> 6], feat);
+    if (visible && y < lnet.log_threshold) o.kind = kProjNone;
+  }
+  if (gid < fp.active_count) store_projection(out, gid, o);
 }
 
 // ----------------------------------------------------------------- scans ----
@@ -1165,14 +1181,13 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   if (n_blocks == 0) n_blocks = 1;
   mark(kStageProject);
   if (b.cull_net)
-    k_project<kModeCull><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, b.cull_net,
-                                                     nullptr, nullptr);
+    k_project_cull<<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, b.cull_net);
   else if (b.record)
-    k_project<kModeCollect><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, nullptr,
-                                                        b.cull_feats, b.record);
+    k_project<kModeCollect><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, b.cull_feats,
+                                                        b.record);
   else
     k_project<kModePlain><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, nullptr,
-                                                      nullptr, nullptr);
+                                                      nullptr);
   mark(kStageVisible);
 
   // Visible flags -> exclusive offsets -> compacted (depth key, ordinal).

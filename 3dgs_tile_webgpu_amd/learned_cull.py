@@ -127,6 +127,42 @@ class LearnedCull:
         self.threshold = threshold
         N.check(N.lib().gs_cull_enable(self._ctx, 1 if self.enabled else 0, threshold), self._ctx)
 
+    def calibrate(self, cameras: list[PerspectiveCamera], sizes: list[tuple[int, int]],
+                  target_psnr: float = 46.0,
+                  shifts=(-3.0, -2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0,
+                          8.0)) -> tuple[float, list]:
+        """Move the network's output bias by the lowest shift (most culling)
+        whose culled frames of these cameras (a BatchedPass owner) keep a
+        mean PSNR of at least target_psnr against their plain frames. Returns
+        the shift and (shift, key fraction kept, mean PSNR, min PSNR) rows."""
+        b = self.owner
+        was = self.enabled
+        self.disable()
+        b.render(cameras, sizes)
+        plain = [b.read_image(v).copy() for v in range(len(cameras))]
+        keys = sum(s.intersections for s in b.read_states())
+        p0 = self.params()
+        rows = []
+        chosen = shifts[-1]
+        for shift in shifts:
+            p = p0.copy()
+            p[PARAMS - 1] += shift
+            self.set_params(p)
+            self.enable()
+            b.render(cameras, sizes)
+            kept = sum(s.intersections for s in b.read_states()) / max(1, keys)
+            ps = [psnr(b.read_image(v), plain[v]) for v in range(len(cameras))]
+            rows.append((shift, kept, float(np.mean(ps)), float(min(ps))))
+            if np.mean(ps) >= target_psnr:
+                chosen = shift
+                break
+        p = p0.copy()
+        p[PARAMS - 1] += chosen
+        self.set_params(p)
+        if not was:
+            self.disable()
+        return chosen, rows
+
     def predict(self, features: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(features, np.float32).reshape(-1, FEATURES)
         out = np.empty(f.shape[0], np.float32)

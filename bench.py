@@ -4,7 +4,14 @@
 A step is one full frame of the plain path (projection, visible compaction,
 depth sort, ordered scan, emission, tile sort, tile ranges, raster) over the
 1M-Gaussian synthetic scene, inputs resident in HBM, replayed as a captured
-graph. Prints one JSON line.
+graph. Beside it the same JSON line carries
+  "batched":      64 orbit cameras of mixed resolution over 2M Gaussians in
+                  one captured launch, launch shape picked by the autotuner;
+  "learned_cull": the MLP cull trained on 512 orbit views of 3M Gaussians at
+                  1920x1080 and evaluated on 64 held-out views: batched
+                  frames/sec with and without it, and PSNR against the plain
+                  frames.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -68,6 +75,116 @@ def cpu_baseline(gs, scene, camera, workload, threads):
     }
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batched(gs, reps=10):
+    """64 orbit cameras (radius 12-25, resolutions drawn from 854x480 ..
+    2560x1440) over the 2M-Gaussian scene, one gs_render_batch per step."""
+    w = gs.WORKLOADS["2m_batch"]
+    scene = gs.workload_scene("2m_batch")
+    views = gs.orbit_cameras(w.views, seed=777, radius=w.orbit, resolutions=gs.BATCH_RESOLUTIONS)
+    cams, sizes = [c for c, _ in views], [s for _, s in views]
+    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22})
+    progress("batched: autotune")
+    t0 = time.perf_counter()
+    chosen, rows = b.autotune(cams, sizes, reps=2)
+    tune_s = time.perf_counter() - t0
+    b.render(cams, sizes)
+    for _ in range(2):
+        b.launch(b._frames)
+    b.read_states()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.launch(b._frames)
+    st = b.read_states()
+    elapsed = time.perf_counter() - t0
+    pixels = sum(x * y for x, y in sizes)
+    out = {
+        "workload": w.name, "gaussians": w.count, "views": len(cams),
+        "frames_per_s": round(len(cams) * reps / elapsed, 1),
+        "ms_per_batch": round(elapsed * 1e3 / reps, 3),
+        "megapixels_per_s": round(pixels * reps / elapsed / 1e6, 1),
+        "keys_per_batch": int(sum(x.intersections for x in st)),
+        "tuning": chosen,
+        "autotune_ms_per_batch": {f"t{r['tileSize']}-d{r['radixBits']}-b{r['rasterBatch']}":
+                                  round(r["ms"], 3) for r in rows},
+        "autotune_seconds": round(tune_s, 2),
+    }
+    b.dispose()
+    return out
+
+
+def learned_cull(gs, train_views=512, eval_views=64, reps=3):
+    """Train the cull on orbit views of the 3M-Gaussian scene, calibrate its
+    output bias on 16 training views (mean PSNR >= 47.5 dB against plain),
+    then time held-out views batched with and without it."""
+    w = gs.WORKLOADS["3m_cull"]
+    scene = gs.workload_scene("3m_cull")
+    res = [(w.width, w.height)]
+    train = gs.orbit_cameras(train_views, seed=5001, radius=w.orbit, resolutions=res)
+    held = gs.orbit_cameras(eval_views, seed=5002, radius=w.orbit, resolutions=res)
+    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 24})
+    cull = gs.LearnedCull(b, seed=3)
+    cull.reserve(100_000_000)
+    progress("learned cull: collecting training frames")
+    t0 = time.perf_counter()
+    for k, (cam, (x, y)) in enumerate(train):
+        cull.collect(cam, x, y, rate=0.05, seed=k)
+    collect_s = time.perf_counter() - t0
+    n = cull.dataset_size()
+    progress(f"learned cull: training on {n} samples")
+    t0 = time.perf_counter()
+    cull.fit_normalization()
+    cull.train(10000, batch=65536, lr=2e-3, under_weight=300.0, seed=1)
+    cull.train(5000, batch=65536, lr=5e-4, under_weight=300.0, seed=2)
+    loss = cull.train(5000, batch=65536, lr=1e-4, under_weight=300.0, seed=3)
+    train_s = time.perf_counter() - t0
+    progress("learned cull: calibrating and evaluating")
+    shift, rows = cull.calibrate([c for c, _ in train[:16]], [s for _, s in train[:16]],
+                                 target_psnr=47.5)
+    cams, sizes = [c for c, _ in held], [s for _, s in held]
+
+    def timed():
+        b.render(cams, sizes)
+        for _ in range(2):
+            b.launch(b._frames)
+        b.read_states()
+        t = time.perf_counter()
+        for _ in range(reps):
+            b.launch(b._frames)
+        st = b.read_states()
+        return (time.perf_counter() - t) / reps, st
+
+    cull.disable()
+    t_plain, st_plain = timed()
+    plain = [b.read_image(v).copy() for v in range(len(cams))]
+    cull.enable(1e-4)
+    t_cull, st_cull = timed()
+    ps = [gs.psnr(b.read_image(v), plain[v]) for v in range(len(cams))]
+    k_plain = sum(x.intersections for x in st_plain)
+    k_cull = sum(x.intersections for x in st_cull)
+    v_plain = sum(x.visible for x in st_plain)
+    v_cull = sum(x.visible for x in st_cull)
+    b.dispose()
+    return {
+        "workload": w.name, "gaussians": w.count, "resolution": f"{w.width}x{w.height}",
+        "train_views": train_views, "eval_views": eval_views, "threshold": 1e-4,
+        "fps_plain_batched": round(len(cams) / t_plain, 1),
+        "fps_cull_batched": round(len(cams) / t_cull, 1),
+        "speedup": round(t_plain / t_cull, 3),
+        "psnr_mean_db": round(float(np.mean(ps)), 2), "psnr_min_db": round(float(min(ps)), 2),
+        "gaussians_culled_fraction": round(1 - v_cull / max(1, v_plain), 4),
+        "keys_removed_fraction": round(1 - k_cull / max(1, k_plain), 4),
+        "training_samples": n, "collect_seconds": round(collect_s, 2),
+        "train_seconds": round(train_s, 2), "final_loss": round(loss, 4),
+        "bias_shift": shift,
+        "calibration": [{"shift": r[0], "keys_kept": round(r[1], 4), "psnr_mean": round(r[2], 2),
+                         "psnr_min": round(r[3], 2)} for r in rows],
+    }
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -87,6 +204,8 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched and learned-cull measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -198,8 +317,13 @@ def main():
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
+        progress("cpu baseline")
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(gs, scene, camera, w, threads)
+    if not args.no_extras:
+        del scene
+        result["batched"] = batched(gs)
+        result["learned_cull"] = learned_cull(gs)
     print(json.dumps(result), flush=True)
 
 

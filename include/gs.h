@@ -271,7 +271,7 @@ uint32_t gs_capacity_next(uint32_t current, uint32_t demand, uint32_t maximum,
  * gs_render_batch alike. Parameters (GS_CULL_PARAMS f32): W1[64][16],
  * b1[64], W2[64][64], b2[64], w3[64], b3. Features (GS_CULL_FEATURES f32):
  * log pixel radius, log conic a, conic b / sqrt(a c), log conic c, opacity,
- * log opacity, log view depth, object-space view direction (3), object-space
+ * object-space position . view direction, log view depth, object-space view direction (3), object-space
  * position (3), log SH energy, x / width, y / height. */
 #define GS_CULL_PARAMS 5313
 #define GS_CULL_FEATURES 16
@@ -301,8 +301,9 @@ int gs_cull_read_dataset(gs_ctx* ctx, uint32_t first, uint32_t n, float* feature
 /* Standardisation from the training set's feature means and deviations. */
 int gs_cull_fit_normalization(gs_ctx* ctx);
 /* `steps` Adam steps (beta 0.9 / 0.999) on minibatches of `batch` samples;
- * loss mean(w (y - t)^2) with w = under_weight where the target is above
- * the threshold and the prediction below it. *last_loss: the last step's. */
+ * loss mean(w (y - t)^2) with w = 1 + under_weight * exp(t) where the
+ * target t is above log(threshold) and the prediction y below t, else 1.
+ * *last_loss: the last step's. */
 int gs_cull_train(gs_ctx* ctx, uint32_t steps, uint32_t batch, float lr, float under_weight,
                   uint64_t seed, float* last_loss);
 /* Predictions for n host feature rows (raw features), and the training

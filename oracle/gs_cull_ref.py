@@ -11,7 +11,7 @@ gfx950 kernels feed the matrix cores:
   a1  = bf16(relu(x W1b^T + b1))            W1b = bf16(W1)
   z2  = a1 W2b^T + b2,  a2 = relu(z2)       W2b = bf16(W2)
   y   = a2 . w3b + b3                       w3b = bf16(w3)
-  loss = mean(w (y - t)^2), w = under_weight if (t > log_thr and y < t) else 1
+  loss = mean(w (y - t)^2), w = 1 + under_weight e^t if (t > log_thr and y < t) else 1
 
 Backward: dy = 2 w (y - t) / n; dZ2 = dy w3b [z2 > 0]; dW2 = bf16(dZ2)^T a1;
 dA1 = bf16(dZ2) W2b; dZ1 = dA1 [a1 > 0]; dW1 = bf16(dZ1)^T x; db1, db2 sum
@@ -72,7 +72,7 @@ def loss_and_grad(p, mu, inv_sigma, feats, targets, under_weight, log_thr, round
     y = fw["y"]
     n = y.shape[0]
     d = y - t
-    w = np.where((t > log_thr) & (y < t), under_weight, 1.0)
+    w = np.where((t > log_thr) & (y < t), 1.0 + under_weight * np.exp(t), 1.0)
     loss = float(np.mean(w * d * d))
     dy = 2.0 * w * d / n
     z2, a1, a2, x = fw["z2"], fw["a1"], fw["a2"], fw["x"]

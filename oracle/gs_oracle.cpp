@@ -560,15 +560,16 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
   if (count == 0) return;
   if (s.cull_outputs) {
     // Learned-cull features: log pixel radius, log conic diagonal, conic
-    // correlation, opacity, log opacity, log depth, view direction, object-
-    // space position, log SH energy, normalised screen position.
+    // correlation, opacity, position . view direction, log depth, view
+    // direction, object-space position, log SH energy, normalised screen
+    // position.
     float* f = &r.cull_features[(size_t)gid * 16];
     f[0] = gs_logf(radius_x > radius_y ? radius_x : radius_y);
     f[1] = gs_logf(conic.x);
     f[2] = conic.y / gs_sqrtf(conic.x * conic.z);
     f[3] = gs_logf(conic.z);
     f[4] = opacity;
-    f[5] = gs_logf(opacity);
+    f[5] = local.x * view_dir.x + local.y * view_dir.y + local.z * view_dir.z;
     f[6] = gs_logf(depth);
     f[7] = view_dir.x;
     f[8] = view_dir.y;

@@ -109,6 +109,8 @@ def test_oracle_features_consistent_with_projection(gs, oracle):
     assert vis.sum() > 1000
     fv = feats[vis]
     np.testing.assert_array_equal(fv[:, 4], pm[vis, 3])                 # opacity
+    np.testing.assert_allclose(fv[:, 5], np.sum(fv[:, 10:13] * fv[:, 7:10], axis=1),
+                               rtol=1e-5, atol=1e-5)                     # position . direction
     np.testing.assert_allclose(np.exp(fv[:, 6]), pm[vis, 2], rtol=1e-6)  # log depth
     np.testing.assert_allclose(fv[:, 14] * 320, pm[vis, 0], rtol=1e-6)
     np.testing.assert_allclose(fv[:, 15] * 180, pm[vis, 1], rtol=1e-6)
