@@ -103,6 +103,7 @@ class GsState(C.Structure):
         ("tiles_y", u32),
         ("raster_checked", C.c_uint64),
         ("raster_blended", C.c_uint64),
+        ("raster_checked_heavy", C.c_uint64),
     ]
 
 
@@ -189,7 +190,8 @@ SIGNATURES = {
     "gs_device_image": (C.c_int, [P, C.POINTER(P), C.POINTER(P)]),
     "gs_capacity_next": (u32, [u32, u32, u32, C.c_double, C.c_double]),
     "gs_profile_begin": (C.c_int, [P, u32]),
-    "gs_profile_end": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(u32)]),
+    "gs_profile_end": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(u32)]),
     "gs_encode_compact": (None, [C.c_size_t, P, P, P, P, P]),
     "gs_encode_quat_oct101012": (u32, [P]),
     "gs_pack_sh_rgb8e8": (u32, [P]),

@@ -875,7 +875,7 @@ int launch_set(gs_ctx* ctx, ViewSet& set, bool profile) {
   if (profile) {
     if (ctx->profile_frames >= ctx->profile_capacity)
       return fail(ctx, GS_E_RANGE, "profiling capacity exhausted");
-    hipEvent_t* marks = &ctx->marks[(size_t)ctx->profile_frames * (gs::kStageCount + 1)];
+    hipEvent_t* marks = &ctx->marks[(size_t)ctx->profile_frames * gs::kMarks];
     hipError_t e = enqueue_set(ctx, set, marks);
     if (e != hipSuccess) return hip_fail(ctx, e, "frame launch");
     ctx->profile_frames++;
@@ -1065,6 +1065,7 @@ int read_states(gs_ctx* ctx, const ViewSet& set, uint32_t first, uint32_t n, gs_
     o.tiles_y = p.tiles_y;
     o.raster_checked = s[i].checked;
     o.raster_blended = s[i].blended;
+    o.raster_checked_heavy = s[i].checked_heavy;
   }
   return GS_OK;
 }
@@ -1100,7 +1101,7 @@ extern "C" int gs_render_batch(gs_ctx* ctx, uint32_t n, const gs_frame* frames,
 extern "C" int gs_profile_begin(gs_ctx* ctx, uint32_t max_frames) {
   if (!ctx || max_frames == 0) return GS_E_RANGE;
   (void)hipSetDevice(ctx->cfg.device);
-  size_t need = (size_t)max_frames * (gs::kStageCount + 1);
+  size_t need = (size_t)max_frames * gs::kMarks;
   while (ctx->marks.size() < need) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return fail(ctx, GS_E_DEVICE, "hipEventCreate");
@@ -1112,28 +1113,31 @@ extern "C" int gs_profile_begin(gs_ctx* ctx, uint32_t max_frames) {
 }
 
 extern "C" int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], double* frame_ms,
-                              uint32_t* frames) {
+                              double* heavy_ms, uint32_t* frames) {
   if (!ctx) return GS_E_RANGE;
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
   double sums[gs::kStageCount] = {0};
-  double total = 0.0;
+  double total = 0.0, heavy_total = 0.0;
   for (uint32_t f = 0; f < ctx->profile_frames; f++) {
-    hipEvent_t* m = &ctx->marks[(size_t)f * (gs::kStageCount + 1)];
+    hipEvent_t* m = &ctx->marks[(size_t)f * gs::kMarks];
     for (int st = 0; st < gs::kStageCount; st++) {
       float ms = 0.0f;
       if (hipEventElapsedTime(&ms, m[st], m[st + 1]) != hipSuccess)
         return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
       sums[st] += ms;
     }
-    float whole = 0.0f;
-    if (hipEventElapsedTime(&whole, m[0], m[gs::kStageCount]) != hipSuccess)
+    float whole = 0.0f, heavy = 0.0f;
+    if (hipEventElapsedTime(&whole, m[0], m[gs::kStageCount]) != hipSuccess ||
+        hipEventElapsedTime(&heavy, m[gs::kMarkHeavyBegin], m[gs::kMarkHeavyEnd]) != hipSuccess)
       return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
     total += whole;
+    heavy_total += heavy;
   }
   if (stage_ms)
     for (int st = 0; st < gs::kStageCount; st++) stage_ms[st] = sums[st];
   if (frame_ms) *frame_ms = total;
+  if (heavy_ms) *heavy_ms = heavy_total;
   if (frames) *frames = ctx->profile_frames;
   ctx->profile_capacity = 0;
   ctx->profile_frames = 0;

@@ -66,6 +66,7 @@ struct DeviceState {
   uint32_t chunk_tasks;  // raster chunk tasks of this frame
   uint32_t heavy_units;  // raster heavy work-list length of this frame
   unsigned long long checked, blended;
+  unsigned long long checked_heavy;  // the part of `checked` in heavy units
 };
 
 struct SceneView {
@@ -138,7 +139,8 @@ struct FrameBuffers {
 };
 
 // Frame stages, in launch order; launch_frame can record an event before
-// each stage and one after the last (kStageCount + 1 events).
+// each stage and one after the last (kStageCount + 1 events), then two on
+// the raster's second stream around the heavy-unit kernel (kMarks in all).
 enum Stage : int {
   kStageProject = 0,
   kStageVisible,
@@ -150,9 +152,12 @@ enum Stage : int {
   kStageRaster,
   kStageCount
 };
+constexpr int kMarkHeavyBegin = kStageCount + 1;
+constexpr int kMarkHeavyEnd = kStageCount + 2;
+constexpr int kMarks = kStageCount + 3;
 
 // Enqueue one frame on `stream` (all kernels; sizes read on the device).
-// `marks` is null or kStageCount + 1 events recorded at stage boundaries.
+// `marks` is null or kMarks events (stage boundaries, heavy-unit kernel).
 // The raster's light and heavy kernels run concurrently: `aux` is forked
 // from and joined back into `stream` through the two events.
 hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, hipStream_t aux,

@@ -260,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   if (gid == 0) {
     state->checked = 0ull;
     state->blended = 0ull;
+    state->checked_heavy = 0ull;
   }
   if (gid >= fp.active_count) return;
   float feat[kCullIn];
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
   if (gid == 0) {
     state->checked = 0ull;
     state->blended = 0ull;
+    state->checked_heavy = 0ull;
   }
   float feat[kCullIn];
   ProjOut o;
@@ -872,7 +874,7 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   return s;
 }
 
-GS_DEV void add_stats(DeviceState* state, uint32_t checked, uint32_t blended) {
+GS_DEV void add_stats(DeviceState* state, uint32_t checked, uint32_t blended, bool heavy) {
   unsigned long long c64 = checked, b64 = blended;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
@@ -882,6 +884,7 @@ GS_DEV void add_stats(DeviceState* state, uint32_t checked, uint32_t blended) {
   if (lane_id() == 0) {
     atomicAdd(&state->checked, c64);
     atomicAdd(&state->blended, b64);
+    if (heavy) atomicAdd(&state->checked_heavy, c64);
   }
 }
 
@@ -935,7 +938,7 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
   bool active;
   PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
       fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
-  if (fp.raster_stats) add_stats(state, s.checked, s.blended);
+  if (fp.raster_stats) add_stats(state, s.checked, s.blended, false);
   if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
 }
 
@@ -1035,7 +1038,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
   bool active;
   PixelSum s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge), kRecord>(
       fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
-  if (fp.raster_stats) add_stats(state, s.checked, s.blended);
+  if (fp.raster_stats) add_stats(state, s.checked, s.blended, true);
   if (threadIdx.x >= (uint32_t)kPixels) return;
   if (chunk) {
     size_t p = (size_t)u * kPixels + threadIdx.x;  // chunk tasks lead the list
@@ -1131,7 +1134,7 @@ void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_coun
 // concurrently with light tiles on s, then the chunk composite.
 template <int kEdge, int kBatch, bool kRecord>
 hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
-                         hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
+                         hipStream_t aux, hipEvent_t fork, hipEvent_t join, hipEvent_t* marks) {
   const FrameParams& fp = b.params;
   DeviceState* st = b.state;
   k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
@@ -1139,9 +1142,11 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   hipError_t e = hipEventRecord(fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
   if (e != hipSuccess) return e;
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
   k_raster_heavy<kEdge, kRecord><<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
       fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
       b.chunk_partials, b.partial_depth, b.color_out, b.depth_out, b.record);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], aux);
   e = hipEventRecord(join, aux);
   if (e != hipSuccess) return e;
   k_raster_light<kEdge, kBatch, kRecord><<<b.max_tiles, light_threads(kEdge, kBatch), 0, s>>>(
@@ -1159,11 +1164,12 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
 // The instrumented raster (contribution record) uses the 256-record batch.
 template <int kEdge>
 hipError_t launch_raster_edge(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
-                              hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
-  if (b.record) return launch_raster<kEdge, 256, true>(b, sorted, s, aux, fork, join);
+                              hipStream_t aux, hipEvent_t fork, hipEvent_t join,
+                              hipEvent_t* marks) {
+  if (b.record) return launch_raster<kEdge, 256, true>(b, sorted, s, aux, fork, join, marks);
   if (b.raster_batch == 128)
-    return launch_raster<kEdge, 128, false>(b, sorted, s, aux, fork, join);
-  return launch_raster<kEdge, 256, false>(b, sorted, s, aux, fork, join);
+    return launch_raster<kEdge, 128, false>(b, sorted, s, aux, fork, join, marks);
+  return launch_raster<kEdge, 256, false>(b, sorted, s, aux, fork, join, marks);
 }
 
 }  // namespace
@@ -1233,11 +1239,11 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   mark(kStageRaster);
   hipError_t e;
   if (fp.tile_size == 8)
-    e = launch_raster_edge<8>(b, sorted, s, aux, fork, join);
+    e = launch_raster_edge<8>(b, sorted, s, aux, fork, join, marks);
   else if (fp.tile_size == 32)
-    e = launch_raster_edge<32>(b, sorted, s, aux, fork, join);
+    e = launch_raster_edge<32>(b, sorted, s, aux, fork, join, marks);
   else
-    e = launch_raster_edge<16>(b, sorted, s, aux, fork, join);
+    e = launch_raster_edge<16>(b, sorted, s, aux, fork, join, marks);
   if (e != hipSuccess) return e;
   mark(kStageCount);
   return hipGetLastError();

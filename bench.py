@@ -206,6 +206,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched and learned-cull measurements")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="end with a device-to-device copy of this many MiB, a kernel of "
+                         "known traffic for calibrating hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -258,10 +261,13 @@ def main():
         render()
     sums = (C.c_double * 8)()
     frame_ms = C.c_double()
+    heavy_ms = C.c_double()
     frames = C.c_uint32()
-    gs._native.check(lib.gs_profile_end(ctx, sums, C.byref(frame_ms), C.byref(frames)), ctx)
+    gs._native.check(lib.gs_profile_end(ctx, sums, C.byref(frame_ms), C.byref(heavy_ms),
+                                        C.byref(frames)), ctx)
     stage_ms = {s: sums[i] / frames.value for i, s in enumerate(STAGES)}
     eager_frame_ms = frame_ms.value / frames.value
+    heavy_kernel_ms = heavy_ms.value / frames.value
 
     tiles = st.tiles_x * st.tiles_y
     tile_passes = max(1, -(-int(tiles).bit_length() // 8))
@@ -271,18 +277,25 @@ def main():
     sort_ms = stage_ms["depth_sort"] + stage_ms["tile_sort"]
     dominant = max(STAGES, key=lambda s: stage_ms[s])
     if dominant == "raster":
-        # FP32 roofline: checked pairs x (20 FP32 ops + 1 exp) per launch.
+        # The raster's dominant kernel is k_raster_heavy (tiles above 1024
+        # records and chunk tasks), timed with events on its own stream.
+        # FP32 roofline: its checked pairs x (20 FP32 ops + 1 exp) per launch.
         p.dispose()
         p2 = gs.gaussian_pass(camera, scene, {"intersectionCapacity": p.capacity, "rasterStats": True})
         p2.render(w.width, w.height, retry_overflow=False)
-        checked = p2.read_state().raster_checked
+        st2 = p2.read_state()
+        checked, checked_heavy = st2.raster_checked, st2.raster_checked_heavy
         p2.dispose()
-        flops = checked * 21
-        achieved = flops / (stage_ms["raster"] * 1e-3) / 1e12
+        flops = checked_heavy * 21
+        achieved = flops / (heavy_kernel_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                    "traffic": None, "kernel": "k_raster", "stage_ms": round(stage_ms["raster"], 4),
-                    "algorithmic_flops": int(flops), "checked_pairs": int(checked)}
+                    "traffic": None, "kernel": "k_raster_heavy",
+                    "kernel_ms": round(heavy_kernel_ms, 4), "algorithmic_flops": int(flops),
+                    "checked_pairs_kernel": int(checked_heavy), "checked_pairs_frame": int(checked),
+                    "raster_stage_ms": round(stage_ms["raster"], 4),
+                    "raster_stage_tflops": round(checked * 21 / (stage_ms["raster"] * 1e-3) / 1e12,
+                                                 3)}
     else:
         achieved = algo[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -324,6 +337,13 @@ def main():
         del scene
         result["batched"] = batched(gs)
         result["learned_cull"] = learned_cull(gs)
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        result["copy_probe_bytes"] = n
     print(json.dumps(result), flush=True)
 
 

@@ -141,6 +141,8 @@ typedef struct gs_state {
   uint32_t tiles_x, tiles_y;
   uint64_t raster_checked;     /* with raster_stats */
   uint64_t raster_blended;
+  uint64_t raster_checked_heavy; /* the part of raster_checked in heavy units
+                                    (tiles above 1024 records, chunk tasks) */
 } gs_state;
 
 void gs_config_default(gs_config* config);
@@ -253,9 +255,11 @@ int gs_device_image(gs_ctx* ctx, void** color, void** depth);
 #define GS_STAGE_COUNT 8
 int gs_profile_begin(gs_ctx* ctx, uint32_t max_frames);
 /* Synchronises; stage_ms receives the per-stage sums over the recorded
- * frames, frame_ms the sum of whole-frame times, *frames the count. */
+ * frames, frame_ms the sum of whole-frame times, heavy_ms the sum of the
+ * heavy-unit raster kernel's times (events on its own stream), *frames the
+ * count. */
 int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], double* frame_ms,
-                   uint32_t* frames);
+                   double* heavy_ms, uint32_t* frames);
 
 /* Capacity growth rule (IntersectionCapacityPolicy.ts:90-101): returns the
  * next capacity, or 0 when demand exceeds maximum. */
