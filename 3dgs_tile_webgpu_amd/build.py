@@ -1,10 +1,11 @@
 """Build recipe for the native library (libgs.so) of this package.
 
 Compiles the gfx950 kernels, the C ABI and the host-side producer with
-hipcc into ``_lib/libgs.so`` next to this file. Floating-point contraction
-is disabled everywhere (the kernels must round exactly like the CPU
-restatement they are checked against), and packed FP32 instructions are
-disabled for the device code.
+hipcc into ``_lib/libgs.so`` next to this file, and, where Node's N-API
+headers are installed, the Node addon ``_lib/gs_napi.node`` over it.
+Floating-point contraction is disabled everywhere (the kernels must round
+exactly like the CPU restatement they are checked against), and packed FP32
+instructions are disabled for the device code.
 """
 from __future__ import annotations
 
@@ -46,6 +47,27 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libgs")
 
 
+NODE_INCLUDE = Path(os.environ.get("GS_NODE_INCLUDE", "/usr/include/node"))
+NAPI_PATH = LIB_DIR / "gs_napi.node"
+
+
+def build_napi(verbose: bool = False) -> Path | None:
+    """Compile node/gs_napi.cpp against libgs.so; None when N-API is absent."""
+    if not (NODE_INCLUDE / "node_api.h").exists():
+        return None
+    tmp = LIB_DIR / "gs_napi.node.tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{NODE_INCLUDE}",
+           str(HERE / "node" / "gs_napi.cpp"), "-o", str(tmp), f"-L{LIB_DIR}", "-lgs",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"g++ failed on the N-API addon ({proc.returncode}):\n{proc.stderr[-8000:]}")
+    os.replace(tmp, NAPI_PATH)
+    return NAPI_PATH
+
+
 def build(verbose: bool = False) -> Path:
     """Compile every source into _lib/libgs.so (always recompiles)."""
     LIB_DIR.mkdir(exist_ok=True)
@@ -60,6 +82,7 @@ def build(verbose: bool = False) -> Path:
     if verbose and warnings:
         print("\n".join(warnings), file=sys.stderr)
     os.replace(tmp, LIB_PATH)
+    build_napi(verbose)
     return LIB_PATH
 
 
