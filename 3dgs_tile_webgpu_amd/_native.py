@@ -104,6 +104,7 @@ class GsState(C.Structure):
         ("raster_checked", C.c_uint64),
         ("raster_blended", C.c_uint64),
         ("raster_checked_heavy", C.c_uint64),
+        ("error_flags", u32),
     ]
 
 

@@ -51,8 +51,8 @@ struct Lane {
   // Intersection records, sized for `capacity`.
   uint32_t capacity = 0;
   DeviceBuffer records_a, records_b;
-  // Radix histograms and scan partials, sized for max(slots, capacity).
-  uint32_t sort_capacity = 0;
+  // Radix sort workspace and scan partials, sized for max(slots, capacity).
+  uint32_t sort_capacity = 0, sort_parts = 0;
   DeviceBuffer hist, partials;
   // Tile ranges, sized for tile_capacity + 1.
   uint32_t tile_capacity = 0;
@@ -216,15 +216,23 @@ int lane_records(gs_ctx* ctx, Lane& l, uint32_t capacity) {
 int lane_sort(gs_ctx* ctx, Lane& l) {
   uint32_t m = std::max(l.slot_capacity, l.capacity);
   if (m <= l.sort_capacity && l.hist.ptr) return GS_OK;
-  uint32_t parts = div_up(m, gs::kPartition);
+  uint32_t parts = div_up(m, gs::kSortPartition);
   if (parts == 0) parts = 1;
+  uint32_t scan_parts = div_up(m, gs::kScanPartition);
   int rc;
-  if ((rc = ensure(ctx, l.hist, (size_t)parts * gs::kMaxRadix * 4, "radix-histograms")) ||
-      (rc = ensure(ctx, l.partials, ((size_t)parts * gs::kMaxRadix / gs::kPartition + parts +
-                                     64) * 4,
-                   "scan-partials")))
+  // Sort workspace: digit totals of up to 8 passes, then two sets of
+  // partition status rows (passes alternate between them).
+  size_t sort_words = 8 * 256 + (size_t)2 * parts * 256;
+  // Scan workspace: a 64-bit status word per partition. It starts at zero;
+  // the radix sort's clearing kernel zeroes it again after each scan.
+  size_t scan_bytes = (size_t)8 * (scan_parts + 1);
+  if ((rc = ensure(ctx, l.hist, sort_words * 4, "radix-sort-workspace")) ||
+      (rc = ensure(ctx, l.partials, scan_bytes, "scan-workspace")))
     return rc;
+  hipError_t e = hipMemsetAsync(l.partials.ptr, 0, l.partials.bytes, l.stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync");
   l.sort_capacity = m;
+  l.sort_parts = parts;
   return GS_OK;
 }
 
@@ -344,7 +352,7 @@ static int validate_config(const gs_config* c, std::string& msg) {
     msg = "initialIntersectionCapacity exceeds maximum capacity";
     return GS_E_RANGE;
   }
-  if (c->max_intersection_capacity > 0x7fffffffu) {
+  if (c->max_intersection_capacity > 0x3fffffffu) {
     msg = "Intersection capacity exceeds device buffer or dispatch limits";
     return GS_E_RANGE;
   }
@@ -771,6 +779,7 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.task_capacity = task_capacity_for(ctx->cfg, capacity);
   b.unit_capacity = unit_capacity_for(b.task_capacity, capacity);
   b.hist = (uint32_t*)l.hist.ptr;
+  b.sort_parts = l.sort_parts;
   b.partials = (uint32_t*)l.partials.ptr;
   b.color_out = (uint16_t*)(color_dst ? color_dst : set.color[v].ptr);
   b.depth_out = ctx->cfg.output_depth ? (float*)(depth_dst ? depth_dst : set.depth[v].ptr)
@@ -1066,6 +1075,7 @@ int read_states(gs_ctx* ctx, const ViewSet& set, uint32_t first, uint32_t n, gs_
     o.raster_checked = s[i].checked;
     o.raster_blended = s[i].blended;
     o.raster_checked_heavy = s[i].checked_heavy;
+    o.error_flags = s[i].sort_error;
   }
   return GS_OK;
 }

@@ -10,8 +10,10 @@ namespace gs {
 struct CullNet;
 
 constexpr int kBlock = 256;          // threads per block (4 wave64)
-constexpr int kItems = 16;           // keys per thread in scans and sorts
-constexpr int kPartition = kBlock * kItems;  // 4096 keys per block
+constexpr int kScanItems = 8;        // elements per thread in a scan
+constexpr int kScanPartition = kBlock * kScanItems;  // 2048 elements per scan block
+constexpr int kSortItems = 4;        // keys per thread in a radix pass
+constexpr int kSortPartition = kBlock * kSortItems;  // 1024 keys per sort block
 constexpr int kMaxRadixBits = 8;     // widest digit of the LSD sorts
 constexpr int kMaxRadix = 1 << kMaxRadixBits;
 constexpr int kMaxTile = 32;         // widest tile edge (pixels)
@@ -67,6 +69,8 @@ struct DeviceState {
   uint32_t heavy_units;  // raster heavy work-list length of this frame
   unsigned long long checked, blended;
   unsigned long long checked_heavy;  // the part of `checked` in heavy units
+  uint32_t sort_error;  // a look-back exceeded its wait bound: bit 0 in a
+                        // radix pass, bit 1 in a scan
 };
 
 struct SceneView {
@@ -113,8 +117,11 @@ struct FrameBuffers {
   uint2* records_a;
   uint2* records_b;
   uint32_t* tile_offsets;
-  uint32_t* hist;       // 2^radix_bits * ceil(max(N, capacity) / kPartition)
-  uint32_t* partials;   // scan partials
+  uint32_t* hist;       // sort workspace: 8 x 256 digit totals, then
+                        // 2 x sort_parts x 256 partition status words
+  uint32_t sort_parts;  // partitions the status rows are sized for
+  uint32_t* partials;   // scan workspace: one 64-bit status word per scan
+                        // partition (zero between scans)
   uint32_t* chunk_first;  // per tile: first chunk task (tile capacity)
   uint2* tasks;           // raster heavy units: (tile, chunk or ~0u)
   float4* chunk_partials; // per task and pixel: (C.rgb, T)

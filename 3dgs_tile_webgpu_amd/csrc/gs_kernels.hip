@@ -261,6 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
     state->checked = 0ull;
     state->blended = 0ull;
     state->checked_heavy = 0ull;
+    state->sort_error = 0u;
   }
   if (gid >= fp.active_count) return;
   float feat[kCullIn];
@@ -288,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
     state->checked = 0ull;
     state->blended = 0ull;
     state->checked_heavy = 0ull;
+    state->sort_error = 0u;
   }
   float feat[kCullIn];
   ProjOut o;
@@ -307,12 +309,33 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
 
 // ----------------------------------------------------------------- scans ----
 //
-// Reduce-then-scan over partitions of kPartition elements: k_scan_reduce
-// writes one saturating sum per partition, k_scan_partials scans those in a
-// single block, k_scan_downsweep rescans each partition with its offset and
-// hands (index, exclusive prefix, value) to a store functor. Saturating
-// addition is associative, so the result equals the reference's serial
-// Blelloch scan (scan.ts:27-62) bit for bit.
+// Single-pass exclusive scan with decoupled look-back (one kernel per scan).
+// Block b scans partition b of kScanPartition elements; wave w of the block
+// owns the partition's w-th run of 64 * kScanItems elements and loads it in
+// coalesced rounds of 64 (element k * 64 + lane of the run), scanning each
+// round across the wave with a running carry. The block's total is
+// published at once (aggregate), wave 0 then walks back over the preceding
+// partitions 64 at a time until it meets an inclusive prefix, and thread 0
+// publishes the partition's inclusive prefix. The last partition's prefix
+// is the scan total, which feeds DeviceState by mode. All additions
+// saturate; saturating addition of unsigned values is associative and
+// commutative, so the result equals the reference's serial Blelloch scan
+// (scan.ts:27-62) bit for bit.
+// Partitions are taken in block order: workgroups are dispatched in
+// increasing order on each XCD, so the lowest unfinished partition is
+// always resident and every wait ends. No per-block atomic is taken (on
+// this GPU same-address atomics from many blocks serialize at about 12 ns
+// each). The status words start at zero: the next radix sort's clearing
+// kernel zeroes them after each scan (k_sort_clear). A look-back wait is
+// bounded; exceeding it raises bit 1 of DeviceState::sort_error.
+
+constexpr uint32_t kLookbackSpinLimit = 1u << 20;
+constexpr uint64_t kScanAggregate = 1ull << 32;
+constexpr uint64_t kScanPrefix = 2ull << 32;
+
+struct ScanWorkspace {
+  unsigned long long* status;  // per partition: flag << 32 | saturating sum
+};
 
 struct LoadVisible {  // scan.ts:78-83: 1 if projectedMean.w > 0
   const float4* mean;
@@ -323,11 +346,6 @@ struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
   const uint32_t* counts;
   GS_DEV uint32_t operator()(uint32_t r) const { return counts[sorted[r].y]; }
 };
-struct LoadPlain {
-  const uint32_t* v;
-  GS_DEV uint32_t operator()(uint32_t i) const { return v[i]; }
-};
-
 struct StoreCompact {  // visibility.ts:37-61 (+ visibleOffsets)
   const float4* mean;
   uint32_t depth_mode;
@@ -357,63 +375,10 @@ struct StoreOrdered {
     offsets[r] = prefix;
   }
 };
-struct StorePlain {
-  uint32_t* v;
-  GS_DEV void operator()(uint32_t i, uint32_t prefix, uint32_t) const { v[i] = prefix; }
-};
 
-template <class Load>
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(Load load, LenSpec len_spec,
-                                                        uint32_t* __restrict__ partials) {
-  __shared__ uint32_t lds4[4];
-  uint32_t len = eval_len(len_spec);
-  uint32_t base = blockIdx.x * (uint32_t)kPartition;
-  if (base >= len) return;
-  uint32_t first = base + threadIdx.x * kItems;
-  uint32_t sum = 0;
-#pragma unroll
-  for (int k = 0; k < kItems; k++)
-    if (first + k < len) sum = sat_add(sum, load(first + k));
-  uint32_t total;
-  block_exclusive_scan(sum, &total, lds4);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
-
-// One block of 1024 threads scans ceil(len / kPartition) partition sums in
-// chunks of 1024 with a running carry. Mode selects what the total feeds.
-__global__ __launch_bounds__(1024) void k_scan_partials(LenSpec len_spec,
-                                                        uint32_t* __restrict__ partials,
-                                                        ScanMode mode, uint32_t capacity,
-                                                        const FrameCamera* __restrict__ camera,
-                                                        DeviceState* __restrict__ state) {
-  __shared__ uint32_t wave_sums[16];
-  __shared__ uint32_t carry_s;
-  uint32_t len = eval_len(len_spec);
-  uint32_t nparts = (uint32_t)(((uint64_t)len + kPartition - 1) / kPartition);
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < nparts; base += 1024) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < nparts ? partials[i] : 0u;
-    uint32_t incl = wave_inclusive_scan(v);
-    uint32_t wave = threadIdx.x >> 6;
-    if (lane_id() == 63) wave_sums[wave] = incl;
-    __syncthreads();
-    uint32_t before = carry_s;
-    for (uint32_t w = 0; w < wave; w++) before = sat_add(before, wave_sums[w]);
-    uint32_t prev = __shfl_up(incl, 1, 64);
-    uint32_t excl = lane_id() == 0 ? 0u : prev;
-    if (i < nparts) partials[i] = sat_add(before, excl);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t c = carry_s;
-      for (uint32_t w = 0; w < 16; w++) c = sat_add(c, wave_sums[w]);
-      carry_s = c;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  uint32_t total = carry_s;
+// What a scan's total feeds.
+GS_DEV void scan_finish(ScanMode mode, uint32_t total, uint32_t capacity,
+                        const FrameCamera* camera, DeviceState* state) {
   if (mode == kScanVisible) {
     state->visible = total;
   } else if (mode == kScanIntersections) {
@@ -431,112 +396,303 @@ __global__ __launch_bounds__(1024) void k_scan_partials(LenSpec len_spec,
   }
 }
 
+GS_DEV uint32_t wave_sum_sat(uint32_t v) { return __shfl(wave_inclusive_scan(v), 63, 64); }
+
 template <class Load, class Store>
-__global__ __launch_bounds__(kBlock) void k_scan_downsweep(Load load, Store store,
-                                                           LenSpec len_spec,
-                                                           const uint32_t* __restrict__ partials) {
-  __shared__ uint32_t lds4[4];
-  uint32_t len = eval_len(len_spec);
-  uint32_t base = blockIdx.x * (uint32_t)kPartition;
-  if (base >= len) return;
-  uint32_t first = base + threadIdx.x * kItems;
-  uint32_t vals[kItems];
-  uint32_t sum = 0;
+__global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec len_spec,
+                                                 ScanWorkspace ws, ScanMode mode,
+                                                 uint32_t capacity,
+                                                 const FrameCamera* __restrict__ camera,
+                                                 DeviceState* __restrict__ state) {
+  __shared__ uint32_t s_prefix;
+  __shared__ uint32_t wave_tot[kBlock / 64];
+  const uint32_t t = threadIdx.x, wave = t >> 6, lane = lane_id();
+  const uint32_t len = eval_len(len_spec);
+  const uint32_t nparts = (uint32_t)(((uint64_t)len + kScanPartition - 1) / kScanPartition);
+  const uint32_t part = blockIdx.x;
+  if (part < nparts) {
+    const uint32_t run = part * (uint32_t)kScanPartition + wave * 64u * kScanItems;
+    uint32_t val[kScanItems], ex[kScanItems];
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    vals[k] = first + k < len ? load(first + k) : 0u;
-    sum = sat_add(sum, vals[k]);
-  }
-  uint32_t total;
-  uint32_t prefix = block_exclusive_scan(sum, &total, lds4);
-  prefix = sat_add(partials[blockIdx.x], prefix);
+    for (int k = 0; k < kScanItems; k++) {
+      uint32_t i = run + k * 64u + lane;
+      val[k] = i < len ? load(i) : 0u;
+    }
+    uint32_t carry = 0u;
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    if (first + k < len) store(first + k, prefix, vals[k]);
-    prefix = sat_add(prefix, vals[k]);
+    for (int k = 0; k < kScanItems; k++) {
+      uint32_t incl = wave_inclusive_scan(val[k]);
+      uint32_t prev = __shfl_up(incl, 1, 64);
+      ex[k] = sat_add(carry, lane == 0 ? 0u : prev);
+      carry = sat_add(carry, __shfl(incl, 63, 64));
+    }
+    if (lane == 0) wave_tot[wave] = carry;
+    __syncthreads();
+    uint32_t before = 0u, block_total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; w++) {
+      if (w < wave) before = sat_add(before, wave_tot[w]);
+      block_total = sat_add(block_total, wave_tot[w]);
+    }
+    if (t == 0)
+      __hip_atomic_store(&ws.status[part],
+                         (part == 0 ? kScanPrefix : kScanAggregate) | block_total,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wave == 0) {
+      uint32_t excl = 0u;
+      int32_t q = (int32_t)part - 1;
+      uint32_t spins = 0u;
+      while (q >= 0) {
+        int32_t j = q - (int32_t)lane;
+        unsigned long long v =
+            j >= 0 ? __hip_atomic_load(&ws.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : kScanPrefix;  // before the first partition: an empty prefix
+        uint32_t flag = (uint32_t)(v >> 32);
+        uint64_t prefix_lanes = __ballot(flag == 2u);
+        uint32_t stop = prefix_lanes ? (uint32_t)__builtin_ctzll(prefix_lanes) : 63u;
+        uint64_t needed = stop == 63u ? ~0ull : ((2ull << stop) - 1ull);
+        if (__ballot(flag == 0u) & needed) {
+          if (++spins >= kLookbackSpinLimit) {
+            if (lane == 0) atomicOr(&state->sort_error, 2u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl = sat_add(excl, wave_sum_sat(lane <= stop ? (uint32_t)v : 0u));
+        if (prefix_lanes) break;
+        q -= 64;
+      }
+      if (t == 0) {
+        s_prefix = excl;
+        uint32_t incl = sat_add(excl, block_total);
+        if (part > 0)
+          __hip_atomic_store(&ws.status[part], kScanPrefix | incl, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        if (part == nparts - 1) scan_finish(mode, incl, capacity, camera, state);
+      }
+    }
+    __syncthreads();
+    uint32_t prefix = sat_add(s_prefix, before);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      uint32_t i = run + k * 64u + lane;
+      if (i < len) store(i, sat_add(prefix, ex[k]), val[k]);
+    }
+  } else if (part == 0 && t == 0) {
+    scan_finish(mode, 0u, capacity, camera, state);  // empty sequence
   }
 }
 
 // ----------------------------------------------------------- radix sort ----
 //
-// Stable LSD sort of (key, value) pairs with kBits-bit digits (4, 6 or 8).
-// A pass is a per-partition digit histogram laid out [digit][partition],
-// its exclusive scan (the plain scan kernels), then a scatter that ranks
-// keys stably inside the partition: 16 rounds of 256 consecutive keys, each
-// wave finds its same-digit peers with kBits + 1 ballots (the extra bit
-// marks keys past the end), waves are ordered through LDS.
+// Stable LSD sort of (key, value) pairs with kBits-bit digits (4, 6 or 8),
+// one kernel per digit pass ("onesweep" with decoupled look-back):
+//  - k_sort_histograms counts the digits of every pass in one read of the
+//    keys (global digit totals, hence each digit's start in the output);
+//  - block b of k_sort_pass takes partition b of kSortPartition keys.
+//    Each thread holds kSortItems keys in registers (key r*kBlock + t of the
+//    partition). Per key, the wave finds its same-digit peers with kBits + 1
+//    ballots (the extra bit marks keys past the end): the popcount of the
+//    lower peers is the key's rank in its wave, and the highest peer records
+//    the group size for its (round, wave) slot. One thread per digit then
+//    turns the slot counts into exclusive offsets in (round, wave) order,
+//    which is input order, publishes the partition's digit count, walks
+//    back over the preceding partitions' published counts (kLookbackWindow
+//    partitions per step) until it meets an inclusive prefix, and publishes
+//    its own inclusive prefix. The keys are scattered straight from
+//    registers. Keys keep their input order within a digit.
+// Passes alternate between two sets of status rows: each block of pass p
+// clears its own row in the other set for pass p + 1 (pass p - 1, its last
+// user, has finished; every pass has the same partitions). A block waits
+// only on lower partitions, which in-order dispatch has already made
+// resident (as for the scans); the wait is bounded anyway, and a block that
+// exceeds the bound raises DeviceState::sort_error instead of waiting on.
 
+constexpr int kMaxPasses = 8;  // 32-bit keys with 4-bit digits
+constexpr int kLookbackWindow = 16;  // partitions read per look-back step
+constexpr uint32_t kHistBlocks = 128;  // blocks of the digit-total kernel
+constexpr int kSortWaves = kBlock / 64;
+constexpr int kSortSlots = kSortItems * kSortWaves;
+constexpr uint32_t kStatusAggregate = 1u << 30;
+constexpr uint32_t kStatusPrefix = 2u << 30;
+constexpr uint32_t kStatusCount = (1u << 30) - 1u;
+
+// Views into a lane's sort workspace: per-pass digit totals [pass][256],
+// two sets of partition status rows [set][parts][256], and the scans'
+// status words, which the clearing kernel also zeroes.
+struct SortWorkspace {
+  uint32_t* ghist;
+  uint32_t* status;
+  uint32_t parts;
+  unsigned long long* scan_status;
+  uint32_t scan_words;
+};
+
+GS_DEV uint32_t* status_row(const SortWorkspace& ws, uint32_t set, uint32_t part) {
+  return ws.status + ((size_t)set * ws.parts + part) * 256u;
+}
+
+// Clears the digit totals, the status rows of the first pass and the scan
+// status words (a kernel rather than a memset, so the frame can be
+// captured).
+__global__ __launch_bounds__(kBlock) void k_sort_clear(SortWorkspace ws,
+                                                       const uint32_t* __restrict__ count_ptr) {
+  uint32_t nparts = (*count_ptr + kSortPartition - 1) / kSortPartition;
+  uint32_t stride = gridDim.x * kBlock;
+  uint32_t first = blockIdx.x * kBlock + threadIdx.x;
+  for (uint32_t i = first; i < (uint32_t)(kMaxPasses * 256); i += stride) ws.ghist[i] = 0u;
+  for (uint32_t i = first; i < nparts * 256u; i += stride) ws.status[i] = 0u;
+  for (uint32_t i = first; i < ws.scan_words; i += stride) ws.scan_status[i] = 0ull;
+}
+
+// Same-digit peers of this lane within the wave (kBits + 1 ballots).
 template <int kBits>
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint2* __restrict__ in,
-                                                       const uint32_t* __restrict__ count_ptr,
-                                                       uint32_t shift,
-                                                       uint32_t* __restrict__ hist) {
-  constexpr uint32_t kRadix = 1u << kBits;
-  __shared__ uint32_t bins[kRadix];
-  uint32_t count = *count_ptr;
-  uint32_t nparts = (count + kPartition - 1) / kPartition;
-  if (blockIdx.x >= nparts) return;
-  if (threadIdx.x < kRadix) bins[threadIdx.x] = 0;
-  __syncthreads();
-  uint32_t base = blockIdx.x * (uint32_t)kPartition;
-#pragma unroll 4
-  for (int r = 0; r < kItems; r++) {
-    uint32_t i = base + r * kBlock + threadIdx.x;
-    if (i < count) atomicAdd(&bins[(in[i].x >> shift) & (kRadix - 1)], 1u);
+GS_DEV uint64_t digit_peers(uint32_t d) {
+  uint64_t peers = ~0ull;
+#pragma unroll
+  for (int bit = 0; bit <= kBits; bit++) {
+    uint64_t bal = __ballot((d >> bit) & 1u);
+    peers &= ((d >> bit) & 1u) ? bal : ~bal;
   }
-  __syncthreads();
-  if (threadIdx.x < kRadix) hist[threadIdx.x * nparts + blockIdx.x] = bins[threadIdx.x];
+  return peers;
 }
 
 template <int kBits>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint2* __restrict__ in,
-                                                          uint2* __restrict__ out,
-                                                          const uint32_t* __restrict__ count_ptr,
-                                                          uint32_t shift,
-                                                          const uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restrict__ in,
+                                                            const uint32_t* __restrict__ count_ptr,
+                                                            uint32_t passes,
+                                                            uint32_t* __restrict__ ghist) {
   constexpr uint32_t kRadix = 1u << kBits;
-  __shared__ uint32_t running[kRadix];
-  __shared__ uint32_t wave_cnt[4][kRadix];
-  uint32_t count = *count_ptr;
-  uint32_t nparts = (count + kPartition - 1) / kPartition;
-  if (blockIdx.x >= nparts) return;
-  uint32_t t = threadIdx.x;
-  if (t < kRadix) {
-    running[t] = hist[t * nparts + blockIdx.x];
+  __shared__ uint32_t h[kMaxPasses * kRadix];
+  for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0u;
+  __syncthreads();
+  // Wave-aggregated counting: one LDS add per distinct digit per wave (the
+  // high digits of depth keys are nearly constant, and per-key LDS atomics
+  // on one bin serialize).
+  const uint32_t count = *count_ptr, lane = lane_id();
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < count;
+       base += gridDim.x * kBlock) {
+    uint32_t i = base + lane;
+    bool valid = i < count;
+    uint32_t key = valid ? in[i].x : 0u;
 #pragma unroll
-    for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+    for (int p = 0; p < kMaxPasses; p++) {
+      if ((uint32_t)p >= passes) break;  // uniform
+      uint32_t d = valid ? ((key >> (p * kBits)) & (kRadix - 1)) : kRadix;
+      uint64_t peers = digit_peers<kBits>(d);
+      if (d < kRadix && (peers >> lane) == 1ull)
+        atomicAdd(&h[p * kRadix + d], (uint32_t)__popcll(peers));
+    }
   }
   __syncthreads();
-  uint32_t wave = t >> 6, lane = lane_id();
-  uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  uint32_t base = blockIdx.x * (uint32_t)kPartition;
-  for (int r = 0; r < kItems; r++) {
+  for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock)
+    if (h[i]) atomicAdd(&ghist[(i / kRadix) * 256u + i % kRadix], h[i]);
+}
+
+template <int kBits>
+__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint2* __restrict__ in,
+                                                      uint2* __restrict__ out,
+                                                      const uint32_t* __restrict__ count_ptr,
+                                                      uint32_t pass, SortWorkspace ws,
+                                                      DeviceState* __restrict__ state) {
+  constexpr uint32_t kRadix = 1u << kBits;
+  __shared__ uint32_t slot[kSortSlots][kRadix];  // counts, then exclusive offsets
+  __shared__ uint32_t digit_base[kRadix];
+  __shared__ uint32_t lds4[4];
+  const uint32_t t = threadIdx.x;
+  const uint32_t count = *count_ptr;
+  const uint32_t nparts = (count + kSortPartition - 1) / kSortPartition;
+  const uint32_t part = blockIdx.x;
+  if (part >= nparts) return;  // block-uniform
+  for (uint32_t i = t; i < kSortSlots * kRadix; i += kBlock) (&slot[0][0])[i] = 0u;
+  __syncthreads();
+  const uint32_t set = pass & 1u;
+  const uint32_t shift = pass * kBits;
+  const uint32_t base = part * (uint32_t)kSortPartition;
+  const uint32_t wave = t >> 6, lane = lane_id();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  // The row this partition uses in the next pass.
+  status_row(ws, set ^ 1u, part)[t] = 0u;
+
+  uint2 rec[kSortItems];
+  uint32_t dig[kSortItems], rank[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
     uint32_t i = base + r * kBlock + t;
-    bool valid = i < count;
-    uint2 rec = valid ? in[i] : make_uint2(0u, 0u);
-    uint32_t d = valid ? ((rec.x >> shift) & (kRadix - 1)) : (uint32_t)kRadix;
-    uint64_t peers = ~0ull;
+    rec[r] = i < count ? in[i] : make_uint2(0u, 0u);
+    dig[r] = i < count ? ((rec[r].x >> shift) & (kRadix - 1)) : kRadix;
+  }
 #pragma unroll
-    for (int bit = 0; bit <= kBits; bit++) {
-      uint64_t bal = __ballot((d >> bit) & 1u);
-      peers &= ((d >> bit) & 1u) ? bal : ~bal;
-    }
-    uint32_t rank = __popcll(peers & lt_mask);
-    bool leader = (peers >> lane) == 1ull;  // highest lane among its peers
-    if (valid && leader) wave_cnt[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = running[d] + rank;
-      for (uint32_t w = 0; w < wave; w++) pos += wave_cnt[w][d];
-      out[pos] = rec;
-    }
-    __syncthreads();
-    if (t < kRadix) {
-      running[t] += wave_cnt[0][t] + wave_cnt[1][t] + wave_cnt[2][t] + wave_cnt[3][t];
+  for (int r = 0; r < kSortItems; r++) {
+    uint64_t peers = digit_peers<kBits>(dig[r]);
+    rank[r] = __popcll(peers & lt_mask);
+    if (dig[r] < kRadix && (peers >> lane) == 1ull)
+      slot[r * kSortWaves + wave][dig[r]] = (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  uint32_t local = 0u;
+  if (t < kRadix) {
 #pragma unroll
-      for (int w = 0; w < 4; w++) wave_cnt[w][t] = 0;
+    for (int k = 0; k < kSortSlots; k++) {
+      uint32_t c = slot[k][t];
+      slot[k][t] = local;
+      local += c;
     }
-    __syncthreads();
+    __hip_atomic_store(&status_row(ws, set, part)[t],
+                       (part == 0 ? kStatusPrefix : kStatusAggregate) | local, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // Each digit's start in the output: exclusive scan of the digit totals.
+  uint32_t total;
+  uint32_t digit_start =
+      block_exclusive_scan(t < kRadix ? ws.ghist[pass * 256u + t] : 0u, &total, lds4);
+  if (t < kRadix) {
+    uint32_t excl = 0u;
+    if (part > 0) {
+      // Look back kLookbackWindow partitions per step (independent loads),
+      // nearest first, up to the first one not yet published.
+      int32_t q = (int32_t)part - 1;
+      uint32_t spins = 0;
+      for (;;) {
+        uint32_t v[kLookbackWindow];
+#pragma unroll
+        for (int j = 0; j < kLookbackWindow; j++)
+          v[j] = q - j >= 0 ? __hip_atomic_load(&status_row(ws, set, q - j)[t], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : kStatusPrefix;  // partition 0 is always a prefix
+        int used = 0;
+        bool found = false, stopped = false;
+#pragma unroll
+        for (int j = 0; j < kLookbackWindow; j++) {
+          uint32_t flag = v[j] & ~kStatusCount;
+          if (!stopped && flag != 0u) {
+            excl += v[j] & kStatusCount;
+            used++;
+            if (flag == kStatusPrefix) found = true;
+          }
+          stopped = stopped || flag == 0u || flag == kStatusPrefix;
+        }
+        if (found) break;
+        q -= used;
+        if (used == 0) {
+          if (++spins >= kLookbackSpinLimit) {
+            atomicOr(&state->sort_error, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __hip_atomic_store(&status_row(ws, set, part)[t], kStatusPrefix | (excl + local),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    digit_base[t] = digit_start + excl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    uint32_t d = dig[r];
+    if (d < kRadix) out[digit_base[d] + slot[r * kSortWaves + wave][d] + rank[r]] = rec[r];
   }
 }
 
@@ -599,17 +755,34 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
 // the first non-empty tile at or after it -- the suffix minimum -- and
 // off[T] ends the last range (K, or the first sentinel). Grid-stride over
 // the device-side count.
+// Record i starts every tile in (tile[i - 1], tile[i]] (tile[-1] = -1,
+// tile[K] = T): off[t] = i there. Runs of empty tiles can be long (a frame
+// with few occupied tiles), so a lane with a run above 8 tiles hands it to
+// its whole wave, which fills it 64 tiles at a time.
 __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
                                                          const DeviceState* __restrict__ state,
                                                          const uint2* __restrict__ sorted,
                                                          uint32_t* __restrict__ off) {
-  uint32_t count = state->count;
-  uint32_t tiles = fp.tile_count;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i <= count; i += gridDim.x * kBlock) {
-    uint32_t lo = i == 0 ? 0u : sorted[i - 1].x + 1u;
-    uint32_t hi = i == count ? tiles : sorted[i].x;
-    if (hi > tiles) hi = tiles;
-    for (uint32_t t = lo; t <= hi; t++) off[t] = i;
+  const uint32_t count = state->count, tiles = fp.tile_count, lane = lane_id();
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base <= count;
+       base += gridDim.x * kBlock) {
+    uint32_t i = base + lane;
+    uint32_t lo = 1u, hi = 0u;  // empty
+    if (i <= count) {
+      lo = i == 0 ? 0u : sorted[i - 1].x + 1u;
+      hi = i == count ? tiles : sorted[i].x;
+      if (hi > tiles) hi = tiles;
+    }
+    bool wide = lo <= hi && hi - lo >= 8u;
+    if (!wide)
+      for (uint32_t t = lo; t <= hi; t++) off[t] = i;
+    uint64_t m = __ballot(wide);
+    while (m) {
+      int src = __builtin_ctzll(m);
+      m &= m - 1ull;
+      uint32_t a = __shfl(lo, src, 64), b = __shfl(hi, src, 64), v = __shfl(i, src, 64);
+      for (uint32_t t = a + lane; t <= b; t += 64u) off[t] = v;
+    }
   }
 }
 
@@ -1089,45 +1262,35 @@ namespace {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-hipError_t plain_scan(uint32_t* data, LenSpec len, uint32_t max_len, uint32_t* partials,
-                      DeviceState* state, hipStream_t s) {
-  uint32_t blocks = div_up(max_len, kPartition);
-  if (blocks == 0) blocks = 1;
-  k_scan_reduce<LoadPlain><<<blocks, kBlock, 0, s>>>(LoadPlain{data}, len, partials);
-  k_scan_partials<<<1, 1024, 0, s>>>(len, partials, kScanPlain, 0u, nullptr, state);
-  k_scan_downsweep<LoadPlain, StorePlain>
-      <<<blocks, kBlock, 0, s>>>(LoadPlain{data}, StorePlain{data}, len, partials);
-  return hipGetLastError();
-}
-
 // Sort pairs in a <-> b for `passes` kBits-bit digits; the result lands in
 // a for an even pass count, in b otherwise.
 template <int kBits>
 void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
-                     uint32_t passes, uint32_t* hist, uint32_t* partials, DeviceState* state,
+                     uint32_t passes, const SortWorkspace& ws, DeviceState* state,
                      hipStream_t s) {
-  uint32_t blocks = div_up(max_count, kPartition);
+  uint32_t blocks = div_up(max_count, kSortPartition);
   if (blocks == 0) blocks = 1;
-  LenSpec hist_len = {count_ptr, 12u /* log2 kPartition */, 1u << kBits, 0u};
+  uint32_t clear_blocks = div_up((uint64_t)blocks * 256u + kMaxPasses * 256u, kBlock);
+  if (clear_blocks > 1024) clear_blocks = 1024;
+  k_sort_clear<<<clear_blocks, kBlock, 0, s>>>(ws, count_ptr);
+  k_sort_histograms<kBits><<<blocks < kHistBlocks ? blocks : kHistBlocks, kBlock, 0, s>>>(
+      a, count_ptr, passes, ws.ghist);
   for (uint32_t p = 0; p < passes; p++) {
     const uint2* in = (p % 2 == 0) ? a : b;
     uint2* out = (p % 2 == 0) ? b : a;
-    uint32_t shift = p * kBits;
-    k_radix_hist<kBits><<<blocks, kBlock, 0, s>>>(in, count_ptr, shift, hist);
-    (void)plain_scan(hist, hist_len, blocks << kBits, partials, state, s);
-    k_radix_scatter<kBits><<<blocks, kBlock, 0, s>>>(in, out, count_ptr, shift, hist);
+    k_sort_pass<kBits><<<blocks, kBlock, 0, s>>>(in, out, count_ptr, p, ws, state);
   }
 }
 
 void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
-                uint32_t passes, uint32_t bits, uint32_t* hist, uint32_t* partials,
-                DeviceState* state, hipStream_t s) {
+                uint32_t passes, uint32_t bits, const SortWorkspace& ws, DeviceState* state,
+                hipStream_t s) {
   if (bits == 4)
-    radix_sort_bits<4>(a, b, count_ptr, max_count, passes, hist, partials, state, s);
+    radix_sort_bits<4>(a, b, count_ptr, max_count, passes, ws, state, s);
   else if (bits == 6)
-    radix_sort_bits<6>(a, b, count_ptr, max_count, passes, hist, partials, state, s);
+    radix_sort_bits<6>(a, b, count_ptr, max_count, passes, ws, state, s);
   else
-    radix_sort_bits<8>(a, b, count_ptr, max_count, passes, hist, partials, state, s);
+    radix_sort_bits<8>(a, b, count_ptr, max_count, passes, ws, state, s);
 }
 
 // The raster stage: the heavy work list on s, then heavy units on aux
@@ -1174,7 +1337,6 @@ hipError_t launch_raster_edge(const FrameBuffers& b, const uint2* sorted, hipStr
 
 }  // namespace
 
-static_assert(kPartition == 4096, "hist LenSpec uses log2(kPartition) = 12");
 
 hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, hipEvent_t fork,
                         hipEvent_t join, hipEvent_t* marks) {
@@ -1198,38 +1360,35 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
 
   // Visible flags -> exclusive offsets -> compacted (depth key, ordinal).
   LenSpec active_len = {nullptr, 0u, 1u, fp.active_count};
-  uint32_t part_blocks = div_up(b.max_n, kPartition);
-  if (part_blocks == 0) part_blocks = 1;
-  LoadVisible lv{b.proj.mean};
-  k_scan_reduce<LoadVisible><<<part_blocks, kBlock, 0, s>>>(lv, active_len, b.partials);
-  k_scan_partials<<<1, 1024, 0, s>>>(active_len, b.partials, kScanVisible, 0u, nullptr, st);
+  uint32_t scan_blocks = div_up(b.max_n, kScanPartition);
+  if (scan_blocks == 0) scan_blocks = 1;
+  ScanWorkspace sws = {reinterpret_cast<unsigned long long*>(b.partials)};
   StoreCompact sc{b.proj.mean, fp.depth_mode, fp.camera, b.visible_offsets, b.depth_a};
-  k_scan_downsweep<LoadVisible, StoreCompact>
-      <<<part_blocks, kBlock, 0, s>>>(lv, sc, active_len, b.partials);
+  k_scan<LoadVisible, StoreCompact><<<scan_blocks, kBlock, 0, s>>>(
+      LoadVisible{b.proj.mean}, sc, active_len, sws, kScanVisible, 0u, fp.camera, st);
 
   // Depth sort over V.
   mark(kStageDepthSort);
-  radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.radix_bits, b.hist,
-             b.partials, st, s);
+  SortWorkspace ws = {b.hist, b.hist + kMaxPasses * 256, b.sort_parts, sws.status, scan_blocks};
+  radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.radix_bits, ws, st,
+             s);
   const uint2* sorted_depth = sorted_depth_records(b);
 
   // Depth-ordered tile counts -> write offsets; K, overflow, output state.
   mark(kStageScan);
   LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
-  LoadOrdered lo{sorted_depth, b.proj.tile_counts};
-  k_scan_reduce<LoadOrdered><<<part_blocks, kBlock, 0, s>>>(lo, vis_len, b.partials);
-  k_scan_partials<<<1, 1024, 0, s>>>(vis_len, b.partials, kScanIntersections, fp.capacity,
-                                      fp.camera, st);
-  k_scan_downsweep<LoadOrdered, StoreOrdered><<<part_blocks, kBlock, 0, s>>>(
-      lo, StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, b.partials);
+  k_scan<LoadOrdered, StoreOrdered><<<scan_blocks, kBlock, 0, s>>>(
+      LoadOrdered{sorted_depth, b.proj.tile_counts},
+      StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, sws, kScanIntersections,
+      fp.capacity, fp.camera, st);
 
   mark(kStageEmit);
   k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.mean, b.proj.conic,
                                      b.ordered_counts, b.intersection_offsets, b.records_a);
 
   mark(kStageTileSort);
-  radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, b.hist,
-             b.partials, st, s);
+  radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, ws, st,
+             s);
   const uint2* sorted = sorted_tile_records(b);
 
   mark(kStageTileOffsets);

@@ -63,7 +63,7 @@ typedef struct gs_config {
   uint32_t subpixel_sample_culling;  /* default 1 */
   uint32_t raster_chunk_size;        /* 0 = null (no chunks), default 8192 */
   uint32_t intersection_capacity;    /* initial record capacity, default 262144 */
-  uint32_t max_intersection_capacity;/* hard limit, default 1 << 28 */
+  uint32_t max_intersection_capacity;/* hard limit, default 1 << 28, at most 2^30 - 1 */
   uint32_t output_depth;             /* also produce the r32f scalar target */
   float transmittance_threshold;     /* (0, 1), default 1e-4 */
   float depth_alpha_threshold;       /* default 0.95 */
@@ -143,6 +143,9 @@ typedef struct gs_state {
   uint64_t raster_blended;
   uint64_t raster_checked_heavy; /* the part of raster_checked in heavy units
                                     (tiles above 1024 records, chunk tasks) */
+  uint32_t error_flags;        /* bit 0: a radix pass, bit 1: a scan exceeded
+                                  its bounded wait for earlier partitions (the
+                                  frame is invalid) */
 } gs_state;
 
 void gs_config_default(gs_config* config);

@@ -42,6 +42,7 @@ def render_both(gs, oracle, scene, camera, width, height, options=None, capacity
 def assert_frame_matches(p, ref, check_f64=True):
     st = p.read_state()
     c = ref.counts
+    assert st.error_flags == 0
     assert (st.visible, st.requested, st.intersections, st.overflow) == (
         c.visible, c.total, c.count, c.overflow)
     assert c.sentinels == 0

@@ -35,7 +35,7 @@ def test_stress_full_frame_properties(gs, gpu_available):
     p = gs.gaussian_pass(w.camera(), scene, {"initialIntersectionCapacity": 1 << 26})
     p.render(w.width, w.height)
     st = p.read_state()
-    assert not st.overflow
+    assert not st.overflow and st.error_flags == 0
     k = st.intersections
     print(f"6M 4K: V={st.visible} K={k}")
     assert 60_000_000 <= k <= 100_000_000
