@@ -46,8 +46,8 @@ struct Lane {
   hipEvent_t fork = nullptr, join = nullptr, done = nullptr;
   // Per-Gaussian frame arrays, sized for slot_capacity Gaussians.
   uint32_t slot_capacity = 0;
-  DeviceBuffer pmean, pconic, pcolor, tile_counts, visible_offsets, depth_a, depth_b,
-      ordered_counts, intersection_offsets;
+  DeviceBuffer pmean, pconic, pcolor, tile_counts, hits, visible_offsets, depth_a, depth_b,
+      ordered_counts, ordered_hits, intersection_offsets;
   // Intersection records, sized for `capacity`.
   uint32_t capacity = 0;
   DeviceBuffer records_a, records_b;
@@ -193,6 +193,8 @@ int lane_slots(gs_ctx* ctx, Lane& l, uint32_t n) {
       (rc = ensure(ctx, l.pconic, c * 16, "projected-conic")) ||
       (rc = ensure(ctx, l.pcolor, c * 16, "projected-color")) ||
       (rc = ensure(ctx, l.tile_counts, c * 4, "tile-counts")) ||
+      (rc = ensure(ctx, l.hits, c * 16, "tile-hits")) ||
+      (rc = ensure(ctx, l.ordered_hits, c * 16, "ordered-tile-hits")) ||
       (rc = ensure(ctx, l.visible_offsets, c * 4, "visible-offsets")) ||
       (rc = ensure(ctx, l.depth_a, c * 8, "depth-records-a")) ||
       (rc = ensure(ctx, l.depth_b, c * 8, "depth-records-b")) ||
@@ -254,6 +256,7 @@ int lane_create(gs_ctx* ctx, Lane& l, bool own_stream) {
 
 void lane_destroy(gs_ctx* ctx, Lane& l) {
   DeviceBuffer* all[] = {&l.pmean,     &l.pconic,         &l.pcolor,       &l.tile_counts,
+                         &l.hits,      &l.ordered_hits,
                          &l.visible_offsets, &l.depth_a,  &l.depth_b,      &l.ordered_counts,
                          &l.intersection_offsets, &l.records_a, &l.records_b, &l.hist,
                          &l.partials,  &l.tile_offsets,   &l.chunk_first,  &l.tasks,
@@ -764,6 +767,8 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.proj.conic = (float4*)l.pconic.ptr;
   b.proj.color = (float4*)l.pcolor.ptr;
   b.proj.tile_counts = (uint32_t*)l.tile_counts.ptr;
+  b.proj.hits = (uint4*)l.hits.ptr;
+  b.ordered_hits = (uint4*)l.ordered_hits.ptr;
   b.visible_offsets = (uint32_t*)l.visible_offsets.ptr;
   b.depth_a = (uint2*)l.depth_a.ptr;
   b.depth_b = (uint2*)l.depth_b.ptr;

@@ -12,8 +12,12 @@ struct CullNet;
 constexpr int kBlock = 256;          // threads per block (4 wave64)
 constexpr int kScanItems = 8;        // elements per thread in a scan
 constexpr int kScanPartition = kBlock * kScanItems;  // 2048 elements per scan block
-constexpr int kSortItems = 4;        // keys per thread in a radix pass
-constexpr int kSortPartition = kBlock * kSortItems;  // 1024 keys per sort block
+#ifndef GS_SORT_THREADS
+#define GS_SORT_THREADS 512
+#endif
+constexpr int kSortThreads = GS_SORT_THREADS;  // threads of a radix-pass block
+constexpr int kSortItems = 8;        // keys per thread in a radix pass
+constexpr int kSortPartition = kSortThreads * kSortItems;  // keys per sort block
 constexpr int kMaxRadixBits = 8;     // widest digit of the LSD sorts
 constexpr int kMaxRadix = 1 << kMaxRadixBits;
 constexpr int kMaxTile = 32;         // widest tile edge (pixels)
@@ -87,6 +91,7 @@ struct ProjectedView {
   float4* conic;
   float4* color;
   uint32_t* tile_counts;
+  uint4* hits;  // per ordinal: tile-hit record (see kHitMask)
 };
 
 // Length of a device-sized sequence: (*ptr rounded up to a multiple of
@@ -113,6 +118,7 @@ struct FrameBuffers {
   uint2* depth_a;
   uint2* depth_b;
   uint32_t* ordered_counts;
+  uint4* ordered_hits;  // tile-hit records in depth order
   uint32_t* intersection_offsets;
   uint2* records_a;
   uint2* records_b;

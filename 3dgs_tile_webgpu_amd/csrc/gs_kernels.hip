@@ -79,7 +79,16 @@ struct ProjOut {
   int kind;
   float4 mean, conic, color;
   uint32_t count;
+  uint4 hits;  // HitRecord
 };
+
+// A visible Gaussian's tile hits, written by the projection and gathered
+// into depth order by the tile-count scan, so that emission reads it in
+// sequence: x = tx0 | ty0 << 16 (bounding-box origin in tiles), y = width |
+// height << 8 | kHitMask (when the box has at most 32 tiles), z = the
+// row-major hit mask over the box, w = the hit count. Without kHitMask the
+// emission tests the tiles again from the projected mean and conic.
+constexpr uint32_t kHitMask = 1u << 16;
 
 // Projection modes: the plain path, the learned cull, and feature
 // collection for its training set.
@@ -93,6 +102,7 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   ProjOut o;
   o.kind = kProjNone;
   o.count = 0u;
+  o.hits = make_uint4(0u, 0u, 0u, 0u);
   uint32_t slot = fp.has_active_slots ? sc.active_slots[gid] : gid;
   float4 mean = sc.means[slot];
   if (mean.w < 0.0f) return o;
@@ -172,25 +182,50 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
     tmax_y = (int)fp.tiles_y - 1;
   }
   uint32_t count = 0;
+  uint32_t bw = (uint32_t)(tmax_x - tmin_x + 1), bh = (uint32_t)(tmax_y - tmin_y + 1);
+  bool small = bw * bh <= 32u;
+  uint32_t mask = 0u, bitpos = 0u;
   for (int ty = tmin_y; ty <= tmax_y; ty++)
-    for (int tx = tmin_x; tx <= tmax_x; tx++)
-      if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) count++;
+    for (int tx = tmin_x; tx <= tmax_x; tx++, bitpos++)
+      if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
+        count++;
+        if (small) mask |= 1u << bitpos;
+      }
   if (count == 0) return o;
+  o.hits = make_uint4((uint32_t)tmin_x | ((uint32_t)tmin_y << 16),
+                      small ? (bw | (bh << 8) | kHitMask) : 0u, mask, count);
   // SH colour (projectionHelpers.ts:118-176), direction = normalized
   // object-space offset from the camera.
   uint32_t degree = fp.sh_degree;
   uint32_t ncoef = (degree + 1) * (degree + 1);
   const uint32_t* shc = sc.sh + (size_t)slot * ncoef;
+  // Degree 3 (16 words, 64-byte aligned rows) is read as four 16-byte
+  // vectors: one load instruction per 16 bytes instead of per word.
+  uint32_t shw[16];
+  if (degree == 3) {
+    const uint4* sh4 = reinterpret_cast<const uint4*>(shc);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      uint4 q4 = sh4[v];
+      shw[4 * v] = q4.x;
+      shw[4 * v + 1] = q4.y;
+      shw[4 * v + 2] = q4.z;
+      shw[4 * v + 3] = q4.w;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 16; v++) shw[v] = (uint32_t)v < ncoef ? shc[v] : 0u;
+  }
   float x = dir[0], y = dir[1], z = dir[2];
   float k[3];
-  decode_sh(shc[0], k);
+  decode_sh(shw[0], k);
   float w0 = 0.28209479177387814f;
   float col[3] = {w0 * k[0], w0 * k[1], w0 * k[2]};
   float sh_energy = 0.0f;
 #define GS_SH_ADD(wexpr, idx)                       \
   {                                                 \
     float w_ = (wexpr);                             \
-    decode_sh(shc[idx], k);                         \
+    decode_sh(shw[idx], k);                         \
     col[0] = col[0] + w_ * k[0];                    \
     col[1] = col[1] + w_ * k[1];                    \
     col[2] = col[2] + w_ * k[2];                    \
@@ -241,6 +276,7 @@ GS_DEV void store_projection(const ProjectedView& out, uint32_t gid, const ProjO
     out.conic[gid] = o.conic;
     out.color[gid] = o.color;
     out.tile_counts[gid] = o.count;
+    out.hits[gid] = o.hits;
   } else {
     out.tile_counts[gid] = 0u;
     out.mean[gid] = o.kind == kProjSubpixel ? o.mean : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -343,8 +379,13 @@ struct LoadVisible {  // scan.ts:78-83: 1 if projectedMean.w > 0
 };
 struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
   const uint2* sorted;
-  const uint32_t* counts;
-  GS_DEV uint32_t operator()(uint32_t r) const { return counts[sorted[r].y]; }
+  const uint4* hits;       // per ordinal; .w is the tile count
+  uint4* ordered_hits;     // the same records in depth order, for emission
+  GS_DEV uint32_t operator()(uint32_t r) const {
+    uint4 h = hits[sorted[r].y];
+    ordered_hits[r] = h;
+    return h.w;
+  }
 };
 struct StoreCompact {  // visibility.ts:37-61 (+ visibleOffsets)
   const float4* mean;
@@ -490,18 +531,21 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 // one kernel per digit pass ("onesweep" with decoupled look-back):
 //  - k_sort_histograms counts the digits of every pass in one read of the
 //    keys (global digit totals, hence each digit's start in the output);
-//  - block b of k_sort_pass takes partition b of kSortPartition keys.
-//    Each thread holds kSortItems keys in registers (key r*kBlock + t of the
-//    partition). Per key, the wave finds its same-digit peers with kBits + 1
-//    ballots (the extra bit marks keys past the end): the popcount of the
-//    lower peers is the key's rank in its wave, and the highest peer records
-//    the group size for its (round, wave) slot. One thread per digit then
-//    turns the slot counts into exclusive offsets in (round, wave) order,
-//    which is input order, publishes the partition's digit count, walks
+//  - block b of k_sort_pass takes partition b of kSortPartition keys; wave
+//    w holds the partition's w-th run of 64 * kSortItems consecutive keys in
+//    registers (key r*64 + lane of the run). Per round, the wave finds each
+//    key's same-digit peers with kBits + 1 ballots (the extra bit marks keys
+//    past the end); the key's rank in the run is the wave's running count of
+//    its digit plus the popcount of its lower peers. One thread per digit
+//    then turns the four waves' counts into offsets in wave order, which is
+//    input order, publishes the partition's digit count, walks
 //    back over the preceding partitions' published counts (kLookbackWindow
 //    partitions per step) until it meets an inclusive prefix, and publishes
 //    its own inclusive prefix. The keys are scattered straight from
 //    registers. Keys keep their input order within a digit.
+// A pass whose digit totals put every key in one digit is a stable identity
+// permutation; its blocks only copy their keys (depth keys of one frame
+// often share the top byte).
 // Passes alternate between two sets of status rows: each block of pass p
 // clears its own row in the other set for pass p + 1 (pass p - 1, its last
 // user, has finished; every pass has the same partitions). A block waits
@@ -510,10 +554,24 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 // exceeds the bound raises DeviceState::sort_error instead of waiting on.
 
 constexpr int kMaxPasses = 8;  // 32-bit keys with 4-bit digits
-constexpr int kLookbackWindow = 16;  // partitions read per look-back step
-constexpr uint32_t kHistBlocks = 128;  // blocks of the digit-total kernel
-constexpr int kSortWaves = kBlock / 64;
-constexpr int kSortSlots = kSortItems * kSortWaves;
+constexpr int kLookbackWindow = 32;  // partitions read per look-back step
+constexpr uint32_t kHistBlocks = 256;  // blocks of the digit-total kernel
+constexpr int kHistItems = 8;          // keys per lane in flight (digit totals)
+constexpr int kSortWaves = kSortThreads / 64;
+
+// Exclusive scan of one value per digit thread (t < 256) across a block of
+// kSortThreads threads; every thread of the block must call it.
+GS_DEV uint32_t digit_exclusive_scan(uint32_t v, uint32_t* lds4) {
+  uint32_t incl = wave_inclusive_scan(v);
+  uint32_t prev = __shfl_up(incl, 1, 64);
+  uint32_t excl = lane_id() == 0 ? 0u : prev;
+  uint32_t wave = threadIdx.x >> 6;
+  if (wave < 4u && lane_id() == 63) lds4[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0u;
+  for (uint32_t w = 0; w < 4u && w < wave; w++) before = sat_add(before, lds4[w]);
+  return sat_add(before, excl);
+}
 constexpr uint32_t kStatusAggregate = 1u << 30;
 constexpr uint32_t kStatusPrefix = 2u << 30;
 constexpr uint32_t kStatusCount = (1u << 30) - 1u;
@@ -567,22 +625,24 @@ __global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restr
   __shared__ uint32_t h[kMaxPasses * kRadix];
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0u;
   __syncthreads();
-  // Wave-aggregated counting: one LDS add per distinct digit per wave (the
-  // high digits of depth keys are nearly constant, and per-key LDS atomics
-  // on one bin serialize).
+  // Each wave takes runs of 64 * kHistItems keys, all loads in flight at once.
   const uint32_t count = *count_ptr, lane = lane_id();
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < count;
-       base += gridDim.x * kBlock) {
-    uint32_t i = base + lane;
-    bool valid = i < count;
-    uint32_t key = valid ? in[i].x : 0u;
+  const uint32_t stride = gridDim.x * kBlock * kHistItems;
+  for (uint32_t base = (blockIdx.x * kBlock + (threadIdx.x & ~63u)) * kHistItems; base < count;
+       base += stride) {
+    uint32_t key[kHistItems];
 #pragma unroll
-    for (int p = 0; p < kMaxPasses; p++) {
-      if ((uint32_t)p >= passes) break;  // uniform
-      uint32_t d = valid ? ((key >> (p * kBits)) & (kRadix - 1)) : kRadix;
-      uint64_t peers = digit_peers<kBits>(d);
-      if (d < kRadix && (peers >> lane) == 1ull)
-        atomicAdd(&h[p * kRadix + d], (uint32_t)__popcll(peers));
+    for (int k = 0; k < kHistItems; k++) {
+      uint32_t i = base + k * 64u + lane;
+      key[k] = i < count ? in[i].x : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kHistItems; k++) {
+      if (base + k * 64u + lane >= count) continue;
+#pragma unroll
+      for (int p = 0; p < kMaxPasses; p++)
+        if ((uint32_t)p < passes)
+          atomicAdd(&h[p * kRadix + ((key[k] >> (p * kBits)) & (kRadix - 1))], 1u);
     }
   }
   __syncthreads();
@@ -591,13 +651,14 @@ __global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restr
 }
 
 template <int kBits>
-__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint2* __restrict__ in,
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restrict__ in,
                                                       uint2* __restrict__ out,
                                                       const uint32_t* __restrict__ count_ptr,
                                                       uint32_t pass, SortWorkspace ws,
                                                       DeviceState* __restrict__ state) {
   constexpr uint32_t kRadix = 1u << kBits;
-  __shared__ uint32_t slot[kSortSlots][kRadix];  // counts, then exclusive offsets
+  constexpr uint32_t kRun = 64u * kSortItems;  // keys of one wave, consecutive
+  __shared__ uint32_t wave_cnt[kSortWaves][kRadix];  // running counts, then offsets
   __shared__ uint32_t digit_base[kRadix];
   __shared__ uint32_t lds4[4];
   const uint32_t t = threadIdx.x;
@@ -605,38 +666,51 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint2* __restrict__ 
   const uint32_t nparts = (count + kSortPartition - 1) / kSortPartition;
   const uint32_t part = blockIdx.x;
   if (part >= nparts) return;  // block-uniform
-  for (uint32_t i = t; i < kSortSlots * kRadix; i += kBlock) (&slot[0][0])[i] = 0u;
-  __syncthreads();
   const uint32_t set = pass & 1u;
   const uint32_t shift = pass * kBits;
-  const uint32_t base = part * (uint32_t)kSortPartition;
   const uint32_t wave = t >> 6, lane = lane_id();
+  const uint32_t run = part * (uint32_t)kSortPartition + wave * kRun;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  // The row this partition uses in the next pass.
-  status_row(ws, set ^ 1u, part)[t] = 0u;
-
+  // Every global load first: the wave's run of keys and this digit's total.
   uint2 rec[kSortItems];
   uint32_t dig[kSortItems], rank[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; r++) {
-    uint32_t i = base + r * kBlock + t;
+    uint32_t i = run + r * 64u + lane;
     rec[r] = i < count ? in[i] : make_uint2(0u, 0u);
-    dig[r] = i < count ? ((rec[r].x >> shift) & (kRadix - 1)) : kRadix;
   }
+  const uint32_t my_total = t < kRadix ? ws.ghist[pass * 256u + t] : 0u;
+  for (uint32_t i = t; i < kSortWaves * kRadix; i += kSortThreads) (&wave_cnt[0][0])[i] = 0u;
+  // The row this partition uses in the next pass.
+  if (t < 256u) status_row(ws, set ^ 1u, part)[t] = 0u;
+  // A pass whose digit is the same for every key keeps the order: copy.
+  if (__syncthreads_or(t < kRadix && my_total == count)) {
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+      uint32_t i = run + r * 64u + lane;
+      if (i < count) out[i] = rec[r];
+    }
+    return;
+  }
+  // Rank within the wave's run, in order: same-digit peers of the round
+  // plus the wave's running count of the digit (read by every lane before
+  // the group's highest lane adds the group size; the wave runs in step).
 #pragma unroll
   for (int r = 0; r < kSortItems; r++) {
+    dig[r] = run + r * 64u + lane < count ? ((rec[r].x >> shift) & (kRadix - 1)) : kRadix;
     uint64_t peers = digit_peers<kBits>(dig[r]);
-    rank[r] = __popcll(peers & lt_mask);
+    uint32_t before = dig[r] < kRadix ? wave_cnt[wave][dig[r]] : 0u;
+    rank[r] = before + (uint32_t)__popcll(peers & lt_mask);
     if (dig[r] < kRadix && (peers >> lane) == 1ull)
-      slot[r * kSortWaves + wave][dig[r]] = (uint32_t)__popcll(peers);
+      wave_cnt[wave][dig[r]] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();
   uint32_t local = 0u;
   if (t < kRadix) {
 #pragma unroll
-    for (int k = 0; k < kSortSlots; k++) {
-      uint32_t c = slot[k][t];
-      slot[k][t] = local;
+    for (int w = 0; w < kSortWaves; w++) {
+      uint32_t c = wave_cnt[w][t];
+      wave_cnt[w][t] = local;
       local += c;
     }
     __hip_atomic_store(&status_row(ws, set, part)[t],
@@ -644,9 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint2* __restrict__ 
                        __HIP_MEMORY_SCOPE_AGENT);
   }
   // Each digit's start in the output: exclusive scan of the digit totals.
-  uint32_t total;
-  uint32_t digit_start =
-      block_exclusive_scan(t < kRadix ? ws.ghist[pass * 256u + t] : 0u, &total, lds4);
+  uint32_t digit_start = digit_exclusive_scan(my_total, lds4);
   if (t < kRadix) {
     uint32_t excl = 0u;
     if (part > 0) {
@@ -692,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint2* __restrict__ 
 #pragma unroll
   for (int r = 0; r < kSortItems; r++) {
     uint32_t d = dig[r];
-    if (d < kRadix) out[digit_base[d] + slot[r * kSortWaves + wave][d] + rank[r]] = rec[r];
+    if (d < kRadix) out[digit_base[d] + wave_cnt[wave][d] + rank[r]] = rec[r];
   }
 }
 
@@ -705,60 +777,69 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
                                                  const uint2* __restrict__ sorted_depth,
                                                  const float4* __restrict__ pmean,
                                                  const float4* __restrict__ pconic,
-                                                 const uint32_t* __restrict__ ordered,
+                                                 const uint4* __restrict__ ordered_hits,
                                                  const uint32_t* __restrict__ offsets,
                                                  uint2* __restrict__ records) {
   uint32_t rank = blockIdx.x * kBlock + threadIdx.x;
   if (state->overflow || rank >= state->visible) return;
   uint32_t capacity = fp.capacity;
   uint32_t gid = sorted_depth[rank].y;
-  float4 mean = pmean[gid];
-  float4 conic4 = pconic[gid];
-  uint32_t packed = gs_f2u(conic4.w);
-  float rx = (float)(packed % 512u), ry = (float)(packed / 512u);
-  float conic[3] = {conic4.x, conic4.y, conic4.z};
-  float thr = gs_logf(mean.w * 255.0f);
-  int max_tx = (int)fp.tiles_x - 1, max_ty = (int)fp.tiles_y - 1;
-  float ts = (float)fp.tile_size;
-  int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) / ts)), 0), max_tx);
-  int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) / ts)), 0), max_ty);
-  int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) / ts)), 0), max_tx);
-  int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) / ts)), 0), max_ty);
-  uint32_t axis = fp.boundary_axis;
-  if (axis == 1) {
-    tmin_x = 0;
-    tmax_x = max_tx;
-  }
-  if (axis == 2) {
-    tmin_y = 0;
-    tmax_y = max_ty;
-  }
-  uint32_t reserved = ordered[rank];
+  uint4 h = ordered_hits[rank];
+  uint32_t reserved = h.w;
   uint32_t base = offsets[rank];
   uint32_t local = 0;
-  float vw = fp.viewport[0], vh = fp.viewport[1];
-  for (int ty = tmin_y; ty <= tmax_y; ty++)
-    for (int tx = tmin_x; tx <= tmax_x; tx++)
-      if (tile_hit(mean.x, mean.y, conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
-        if (local < reserved && base + local < capacity)
-          records[base + local] = make_uint2((uint32_t)ty * fp.tiles_x + (uint32_t)tx, gid);
-        local++;
-      }
+  if (h.y & kHitMask) {
+    // The projection's hits, in the same row-major order.
+    uint32_t tx0 = h.x & 0xffffu, ty0 = h.x >> 16, bw = h.y & 0xffu;
+    for (uint32_t m = h.z; m != 0u; m &= m - 1u, local++) {
+      uint32_t bit = (uint32_t)__builtin_ctz(m);
+      uint32_t tile = (ty0 + bit / bw) * fp.tiles_x + tx0 + bit % bw;
+      if (base + local < capacity) records[base + local] = make_uint2(tile, gid);
+    }
+  } else {
+    float4 mean = pmean[gid];
+    float4 conic4 = pconic[gid];
+    uint32_t packed = gs_f2u(conic4.w);
+    float rx = (float)(packed % 512u), ry = (float)(packed / 512u);
+    float conic[3] = {conic4.x, conic4.y, conic4.z};
+    float thr = gs_logf(mean.w * 255.0f);
+    int max_tx = (int)fp.tiles_x - 1, max_ty = (int)fp.tiles_y - 1;
+    float ts = (float)fp.tile_size;
+    int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) / ts)), 0), max_tx);
+    int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) / ts)), 0), max_ty);
+    int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) / ts)), 0), max_tx);
+    int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) / ts)), 0), max_ty);
+    uint32_t axis = fp.boundary_axis;
+    if (axis == 1) {
+      tmin_x = 0;
+      tmax_x = max_tx;
+    }
+    if (axis == 2) {
+      tmin_y = 0;
+      tmax_y = max_ty;
+    }
+    float vw = fp.viewport[0], vh = fp.viewport[1];
+    for (int ty = tmin_y; ty <= tmax_y; ty++)
+      for (int tx = tmin_x; tx <= tmax_x; tx++)
+        if (tile_hit(mean.x, mean.y, conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
+          if (local < reserved && base + local < capacity)
+            records[base + local] = make_uint2((uint32_t)ty * fp.tiles_x + (uint32_t)tx, gid);
+          local++;
+        }
+  }
   for (uint32_t pad = local; pad < reserved; pad++)
     if (base + pad < capacity) records[base + pad] = make_uint2(fp.tile_count, gid);
 }
 
 // ------------------------------------------------------------ tile ranges ----
 
-// tileOffsets.ts:4-128 in one pass: record i owns the tiles in
-// (tile[i-1], tile[i]], so every tile gets the index of the first record of
-// the first non-empty tile at or after it -- the suffix minimum -- and
-// off[T] ends the last range (K, or the first sentinel). Grid-stride over
-// the device-side count.
-// Record i starts every tile in (tile[i - 1], tile[i]] (tile[-1] = -1,
-// tile[K] = T): off[t] = i there. Runs of empty tiles can be long (a frame
-// with few occupied tiles), so a lane with a run above 8 tiles hands it to
-// its whole wave, which fills it 64 tiles at a time.
+// tileOffsets.ts:4-128 in one pass: record i starts every tile in
+// (tile[i - 1], tile[i]] (tile[-1] = -1, tile[K] = T), so off[t] = i there:
+// every tile gets the index of the first record of the first non-empty tile
+// at or after it (the suffix minimum), and off[T] ends the last range (K, or
+// the first sentinel). Runs of empty tiles can be long (a frame with few
+// occupied tiles), so a lane with a run above 8 tiles hands it to its whole
+// wave, which fills it 64 tiles at a time.
 __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
                                                          const DeviceState* __restrict__ state,
                                                          const uint2* __restrict__ sorted,
@@ -877,10 +958,11 @@ struct PixelSum {
 // threads and stage 1024 records per round (512 for 32-pixel tiles, whose
 // candidate masks would not fit the LDS otherwise).
 constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
-constexpr int kHeavyThreads = 1024;
+constexpr int kHeavyThreads = 1024;  // edge 32; edges 8 and 16 use four lanes per pixel
 constexpr int log2_edge(int e) { return e == 8 ? 3 : (e == 16 ? 4 : 5); }
 constexpr int light_threads(int e, int batch) { return e * e > batch ? e * e : batch; }
-constexpr int heavy_batch(int e) { return e == 32 ? 512 : 1024; }
+constexpr int heavy_batch(int e) { return e == 32 ? 512 : 4 * e * e; }
+constexpr int heavy_threads(int e) { return e == 32 ? kHeavyThreads : 4 * e * e; }
 
 template <int kPixels, int kBatch>
 struct RasterLds {
@@ -1047,6 +1129,173 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   return s;
 }
 
+// DPP broadcast of lane k of each quad of lanes to the whole quad.
+template <int k>
+GS_DEV float quad_bcast(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), k * 85, 0xf, 0xf, false));
+}
+GS_DEV uint32_t quad_or(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 177, 0xf, 0xf, false);  // lane ^ 1
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 78, 0xf, 0xf, false);   // lane ^ 2
+  return v;
+}
+
+// Heavy units with four lanes per pixel (tile edges 8 and 16). Records are
+// staged one per thread (4 * E*E per round) and tested against their support
+// rectangles exactly as in blend_tile_range. The blend of pixel p is shared
+// by lanes 4p..4p+3: per step the quad takes the pixel's next four
+// candidates in record order, lane q evaluates the q-th one (power,
+// exponential and alpha, the expensive part), the quad exchanges the four
+// results (DPP quad broadcasts) and every lane applies the same four blends
+// in the same order, so the four lanes hold the same sum, bit for bit the
+// one-lane result. Next nonzero mask words come from a per-pixel summary
+// word, so no lane scans empty words. The record ordinals of round r + 2
+// are loaded while round r runs, the records of round r + 1 gathered.
+template <int kEdge, bool kRecord>
+GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
+                                 const uint2* __restrict__ sorted,
+                                 const float4* __restrict__ pmean,
+                                 const float4* __restrict__ pconic,
+                                 const float4* __restrict__ pcolor,
+                                 RasterLds<kEdge * kEdge, 4 * kEdge * kEdge>& L, uint32_t& x,
+                                 uint32_t& y, bool& active, uint32_t* __restrict__ record) {
+  constexpr int kPixels = kEdge * kEdge;
+  constexpr int kThreads = 4 * kPixels;
+  constexpr int kBatch = kThreads;
+  constexpr int kMaskWords = kBatch / 32;
+  static_assert(kMaskWords <= 32, "the summary word covers 32 mask words");
+  const uint32_t t = threadIdx.x, q = t & 3u, pix = t >> 2;
+  int tx0 = (int)((tile % fp.tiles_x) * kEdge), ty0 = (int)((tile / fp.tiles_x) * kEdge);
+  x = (uint32_t)tx0 + (pix & (uint32_t)(kEdge - 1));
+  y = (uint32_t)ty0 + (pix >> log2_edge(kEdge));
+  active = x < fp.width && y < fp.height;
+  float pxc = (float)x + 0.5f, pyc = (float)y + 0.5f;
+  float vw = fp.viewport[0], vh = fp.viewport[1];
+  uint32_t axis = fp.boundary_axis;
+  float threshold = fp.transmittance_threshold;
+  PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
+  bool done = !active;
+  uint32_t cg = 0u, cg_next = 0u;
+  float4 m = make_float4(0.f, 0.f, 0.f, 0.f), c = m, k = m;
+  if (cb + t < ce) {
+    cg = sorted[cb + t].y;
+    m = pmean[cg];
+    c = pconic[cg];
+    k = pcolor[cg];
+  }
+  if (cb + kBatch + t < ce) cg_next = sorted[cb + kBatch + t].y;
+  for (uint32_t batch = cb; batch < ce; batch += kBatch) {
+    for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
+      (&L.mask[0][0])[i] = 0u;
+    __syncthreads();
+    bool mine = batch + t < ce;
+    float thr = 0.0f;
+    if (mine) {
+      thr = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
+      L.mean[t] = m;
+      L.conic[t] = make_float4(c.x, c.y, c.z, thr);
+      L.color[t] = k;
+      if (kRecord) L.gid[t] = cg;
+    }
+    float4 cm = m, cc = make_float4(c.x, c.y, c.z, thr);
+    if (batch + kBatch + t < ce) {
+      cg = cg_next;
+      m = pmean[cg];
+      c = pconic[cg];
+      k = pcolor[cg];
+    }
+    if (batch + 2u * kBatch + t < ce) cg_next = sorted[batch + 2u * kBatch + t].y;
+    if (mine) {
+      int x0, x1, y0, y1;
+      if (axis == 0)
+        support_rect(cm.x, cm.y, cc.x, cc.y, cc.z, cc.w, tx0, ty0, kEdge, x0, x1, y0, y1);
+      else {
+        x0 = tx0;
+        x1 = tx0 + kEdge - 1;
+        y0 = ty0;
+        y1 = ty0 + kEdge - 1;
+      }
+      uint32_t bit = 1u << (t & 31u), word = t >> 5;
+      for (int py = y0; py <= y1; py++) {
+        float pyf = (float)py + 0.5f;
+        for (int px = x0; px <= x1; px++) {
+          float power = pair_power((float)px + 0.5f, pyf, cm, cc, vw, vh, axis);
+          if (!(power > 0.0f || power < -cc.w))
+            atomicOr(&L.mask[word][(uint32_t)((py - ty0) * kEdge + (px - tx0))], bit);
+        }
+      }
+    }
+    __syncthreads();
+    if (!done) {  // uniform over the quad
+      uint32_t sum = 0u;
+      for (uint32_t w = q; w < (uint32_t)kMaskWords; w += 4u)
+        if (L.mask[w][pix] != 0u) sum |= 1u << w;
+      sum = quad_or(sum);
+      uint32_t n = min((uint32_t)kBatch, ce - batch);
+      uint32_t stop = n;
+      uint32_t w = 0u, bits = 0u;
+      while (true) {
+        uint32_t j[4];
+        bool v[4];
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+          if (bits == 0u && sum != 0u) {
+            w = (uint32_t)__builtin_ctz(sum);
+            sum &= sum - 1u;
+            bits = L.mask[w][pix];
+          }
+          v[qq] = bits != 0u;
+          j[qq] = w * 32u + (v[qq] ? (uint32_t)__builtin_ctz(bits) : 0u);
+          bits &= bits - 1u;
+        }
+        if (!v[0]) break;
+        uint32_t mj = q == 0u ? j[0] : (q == 1u ? j[1] : (q == 2u ? j[2] : j[3]));
+        bool mv = q == 0u ? v[0] : (q == 1u ? v[1] : (q == 2u ? v[2] : v[3]));
+        float a = 0.0f, cr = 0.0f, cgr = 0.0f, cbl = 0.0f, dep = 0.0f;
+        if (mv) {
+          Candidate cd = eval_candidate(L, mj, pxc, pyc, vw, vh, axis);
+          a = cd.alpha;
+          cr = cd.color.x;
+          cgr = cd.color.y;
+          cbl = cd.color.z;
+          dep = cd.depth;
+        }
+        float qa[4] = {quad_bcast<0>(a), quad_bcast<1>(a), quad_bcast<2>(a), quad_bcast<3>(a)};
+        float qr[4] = {quad_bcast<0>(cr), quad_bcast<1>(cr), quad_bcast<2>(cr), quad_bcast<3>(cr)};
+        float qg[4] = {quad_bcast<0>(cgr), quad_bcast<1>(cgr), quad_bcast<2>(cgr),
+                       quad_bcast<3>(cgr)};
+        float qb[4] = {quad_bcast<0>(cbl), quad_bcast<1>(cbl), quad_bcast<2>(cbl),
+                       quad_bcast<3>(cbl)};
+        float qd[4] = {quad_bcast<0>(dep), quad_bcast<1>(dep), quad_bcast<2>(dep),
+                       quad_bcast<3>(dep)};
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+          if (!v[qq] || done) continue;
+          float alpha = qa[qq];
+          if (alpha < (float)(1.0 / 255.0)) continue;
+          float contribution = s.t * alpha;
+          if (kRecord && q == (uint32_t)qq) atomicMax(&record[L.gid[j[qq]]], gs_f2b(contribution));
+          s.r = s.r + qr[qq] * contribution;
+          s.g = s.g + qg[qq] * contribution;
+          s.b = s.b + qb[qq] * contribution;
+          s.d = s.d + qd[qq] * contribution;
+          s.blended++;
+          s.t = s.t * (1.0f - alpha);
+          if (s.t < threshold) {
+            done = true;
+            stop = j[qq] + 1u;
+          }
+        }
+        if (done) break;
+      }
+      s.checked += stop;  // pairs the reference's loop evaluates
+    }
+    if (__syncthreads_and(done)) break;
+  }
+  return s;
+}
+
 GS_DEV void add_stats(DeviceState* state, uint32_t checked, uint32_t blended, bool heavy) {
   unsigned long long c64 = checked, b64 = blended;
 #pragma unroll
@@ -1093,7 +1342,12 @@ GS_DEV void store_pixel(const FrameParams& fp, uint32_t x, uint32_t y, float r, 
   }
 }
 
-// Light tiles (up to kLightMax records): one block per tile.
+// Light tiles (up to kLightMax records): a grid of at most kLightBlocks
+// blocks walks the tiles in strides (most tiles of a frame can be empty, and
+// a block per tile would spend the raster stage dispatching them). An empty
+// tile stores the background directly.
+constexpr uint32_t kLightBlocks = 2048;
+
 template <int kEdge, int kBatch, bool kRecord>
 __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
@@ -1103,16 +1357,25 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     uint32_t* __restrict__ record) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
   if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
-  uint32_t tile = blockIdx.x;
-  if (tile >= fp.tile_count) return;
-  uint32_t begin = off[tile], end = off[tile + 1];
-  if (end - begin > kLightMax) return;  // heavy unit
-  uint32_t x, y;
-  bool active;
-  PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
-      fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
-  if (fp.raster_stats) add_stats(state, s.checked, s.blended, false);
-  if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+  constexpr uint32_t kPixels = kEdge * kEdge;
+  for (uint32_t tile = blockIdx.x; tile < fp.tile_count; tile += gridDim.x) {
+    uint32_t begin = off[tile], end = off[tile + 1];
+    if (end - begin > kLightMax) continue;  // heavy unit (block-uniform)
+    if (begin == end) {
+      uint32_t t = threadIdx.x;
+      uint32_t x = (tile % fp.tiles_x) * kEdge + (t & (uint32_t)(kEdge - 1));
+      uint32_t y = (tile / fp.tiles_x) * kEdge + ((t >> log2_edge(kEdge)) & (uint32_t)(kEdge - 1));
+      if (t < kPixels && x < fp.width && y < fp.height)
+        store_pixel(fp, x, y, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, color_out, depth_out);
+      continue;
+    }
+    uint32_t x, y;
+    bool active;
+    PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
+        fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
+    if (fp.raster_stats) add_stats(state, s.checked, s.blended, false);
+    if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+  }
 }
 
 // rasterChunks.ts:3-65 in one block, extended into the heavy work list:
@@ -1127,17 +1390,22 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
                                                       uint32_t* __restrict__ chunk_first,
                                                       uint2* __restrict__ units,
                                                       uint32_t unit_capacity) {
-  __shared__ uint32_t wave_sums[16];
-  __shared__ uint32_t carry_s;
+  // Tiles are visited in strides of the block; the waves reserve their chunk
+  // tasks with one LDS add each, so no iteration waits on another (a tile's
+  // chunks are contiguous and start at chunk_first[tile]; the order between
+  // tiles is free). Heavy tiles then take slots after all chunk tasks,
+  // heaviest size class first.
+  __shared__ uint32_t chunk_total;
   __shared__ uint32_t class_count[5], class_next[5];
-  if (threadIdx.x == 0) carry_s = 0;
-  if (threadIdx.x < 5) class_count[threadIdx.x] = 0;
+  if (threadIdx.x == 0) chunk_total = 0u;
+  if (threadIdx.x < 5) class_count[threadIdx.x] = 0u;
   __syncthreads();
-  uint32_t tiles = state->overflow ? 0u : fp.tile_count;
-  uint32_t cs = fp.chunk_size;
-  for (uint32_t base = 0; base < tiles; base += 1024) {
-    uint32_t t = base + threadIdx.x;
-    uint32_t n = 0;
+  const uint32_t tiles = state->overflow ? 0u : fp.tile_count;
+  const uint32_t cs = fp.chunk_size, lane = lane_id();
+#pragma unroll 4
+  for (uint32_t base = threadIdx.x & ~63u; base < tiles; base += 1024u) {
+    uint32_t t = base + lane;
+    uint32_t n = 0u;
     if (t < tiles) {
       uint32_t count = off[t + 1] - off[t];
       if (cs != 0 && count > cs) {
@@ -1148,26 +1416,21 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
       }
     }
     uint32_t incl = wave_inclusive_scan(n);
-    uint32_t wave = threadIdx.x >> 6;
-    if (lane_id() == 63) wave_sums[wave] = incl;
-    __syncthreads();
-    uint32_t first = carry_s;
-    for (uint32_t w = 0; w < wave; w++) first += wave_sums[w];
-    first += incl - n;
+    uint32_t wave_sum = __shfl(incl, 63, 64);
+    uint32_t first = 0u;
+    if (wave_sum != 0u) {  // uniform
+      uint32_t wbase = 0u;
+      if (lane == 0) wbase = atomicAdd(&chunk_total, wave_sum);
+      first = __shfl(wbase, 0, 64) + incl - n;
+    }
     if (t < tiles) chunk_first[t] = first;
     for (uint32_t c = 0; c < n; c++)
       if (first + c < unit_capacity) units[first + c] = make_uint2(t, c);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t c = carry_s;
-      for (uint32_t w = 0; w < 16; w++) c += wave_sums[w];
-      carry_s = c;
-    }
-    __syncthreads();
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t next = carry_s;
-    state->chunk_tasks = min(carry_s, unit_capacity);
+    uint32_t next = chunk_total;
+    state->chunk_tasks = min(chunk_total, unit_capacity);
     for (int c = 4; c >= 0; c--) {
       class_next[c] = next;
       next += class_count[c];
@@ -1175,7 +1438,8 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
     state->heavy_units = min(next, unit_capacity);
   }
   __syncthreads();
-  for (uint32_t t = threadIdx.x; t < tiles; t += 1024) {
+#pragma unroll 4
+  for (uint32_t t = threadIdx.x; t < tiles; t += 1024u) {
     uint32_t count = off[t + 1] - off[t];
     if ((cs == 0 || count <= cs) && count > kLightMax) {
       uint32_t c = min(31u - (uint32_t)__builtin_clz((count - 1u) >> 10), 4u);
@@ -1185,9 +1449,10 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
   }
 }
 
-// Heavy units: one 1024-thread block per chunk task or heavy tile.
+// Heavy units: one block per chunk task or heavy tile (four lanes per pixel
+// for edges 8 and 16, one for edge 32).
 template <int kEdge, bool kRecord>
-__global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
+__global__ __launch_bounds__(heavy_threads(kEdge)) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
     const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
     const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
@@ -1195,6 +1460,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
     float* __restrict__ partial_depth, uint16_t* __restrict__ color_out,
     float* __restrict__ depth_out, uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
+  constexpr bool kQuad = kEdge != 32;
   __shared__ RasterLds<kPixels, heavy_batch(kEdge)> lds;
   uint32_t u = blockIdx.x;
   if (u >= state->heavy_units) return;
@@ -1209,12 +1475,22 @@ __global__ __launch_bounds__(kHeavyThreads) void k_raster_heavy(
   }
   uint32_t x, y;
   bool active;
-  PixelSum s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge), kRecord>(
-      fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
-  if (fp.raster_stats) add_stats(state, s.checked, s.blended, true);
-  if (threadIdx.x >= (uint32_t)kPixels) return;
+  PixelSum s;
+  uint32_t pix = threadIdx.x;
+  bool writer = threadIdx.x < (uint32_t)kPixels;
+  if constexpr (kQuad) {
+    s = blend_heavy_quad<kEdge, kRecord>(fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x,
+                                         y, active, record);
+    pix = threadIdx.x >> 2;
+    writer = (threadIdx.x & 3u) == 0u;
+  } else {
+    s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge), kRecord>(
+        fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
+  }
+  if (fp.raster_stats) add_stats(state, writer ? s.checked : 0u, writer ? s.blended : 0u, true);
+  if (!writer) return;
   if (chunk) {
-    size_t p = (size_t)u * kPixels + threadIdx.x;  // chunk tasks lead the list
+    size_t p = (size_t)u * kPixels + pix;  // chunk tasks lead the list
     partials[p] = make_float4(s.r, s.g, s.b, s.t);
     partial_depth[p] = s.d;
   } else if (active) {
@@ -1278,7 +1554,7 @@ void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max
   for (uint32_t p = 0; p < passes; p++) {
     const uint2* in = (p % 2 == 0) ? a : b;
     uint2* out = (p % 2 == 0) ? b : a;
-    k_sort_pass<kBits><<<blocks, kBlock, 0, s>>>(in, out, count_ptr, p, ws, state);
+    k_sort_pass<kBits><<<blocks, kSortThreads, 0, s>>>(in, out, count_ptr, p, ws, state);
   }
 }
 
@@ -1306,13 +1582,14 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
   if (e != hipSuccess) return e;
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
-  k_raster_heavy<kEdge, kRecord><<<b.unit_capacity, kHeavyThreads, 0, aux>>>(
+  k_raster_heavy<kEdge, kRecord><<<b.unit_capacity, heavy_threads(kEdge), 0, aux>>>(
       fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
       b.chunk_partials, b.partial_depth, b.color_out, b.depth_out, b.record);
   if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], aux);
   e = hipEventRecord(join, aux);
   if (e != hipSuccess) return e;
-  k_raster_light<kEdge, kBatch, kRecord><<<b.max_tiles, light_threads(kEdge, kBatch), 0, s>>>(
+  uint32_t light_blocks = b.max_tiles < kLightBlocks ? b.max_tiles : kLightBlocks;
+  k_raster_light<kEdge, kBatch, kRecord><<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(
       fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.color_out,
       b.depth_out, b.record);
   e = hipStreamWaitEvent(s, join, 0);
@@ -1378,13 +1655,13 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   mark(kStageScan);
   LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
   k_scan<LoadOrdered, StoreOrdered><<<scan_blocks, kBlock, 0, s>>>(
-      LoadOrdered{sorted_depth, b.proj.tile_counts},
+      LoadOrdered{sorted_depth, b.proj.hits, b.ordered_hits},
       StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, sws, kScanIntersections,
       fp.capacity, fp.camera, st);
 
   mark(kStageEmit);
   k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.mean, b.proj.conic,
-                                     b.ordered_counts, b.intersection_offsets, b.records_a);
+                                     b.ordered_hits, b.intersection_offsets, b.records_a);
 
   mark(kStageTileSort);
   radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, ws, st,
