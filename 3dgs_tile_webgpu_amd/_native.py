@@ -52,6 +52,7 @@ class GsConfig(C.Structure):
         ("radix_bits", u32),
         ("raster_batch", u32),
         ("batch_lanes", u32),
+        ("heavy_split_units", u32),
     ]
 
 

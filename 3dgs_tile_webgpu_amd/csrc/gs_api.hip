@@ -46,7 +46,7 @@ struct Lane {
   hipEvent_t fork = nullptr, join = nullptr, done = nullptr;
   // Per-Gaussian frame arrays, sized for slot_capacity Gaussians.
   uint32_t slot_capacity = 0;
-  DeviceBuffer pmean, pconic, pcolor, tile_counts, hits, visible_offsets, depth_a, depth_b,
+  DeviceBuffer pmean, splat, tile_counts, visible_offsets, depth_a, depth_b,
       ordered_counts, ordered_hits, intersection_offsets;
   // Intersection records, sized for `capacity`.
   uint32_t capacity = 0;
@@ -99,6 +99,7 @@ struct CullState {
 
 struct gs_ctx {
   gs_config cfg;
+  uint32_t cu_count = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   hipEvent_t batch_fork = nullptr;
   std::string error;
@@ -190,10 +191,8 @@ int lane_slots(gs_ctx* ctx, Lane& l, uint32_t n) {
   int rc;
   size_t c = cap;
   if ((rc = ensure(ctx, l.pmean, c * 16, "projected-mean")) ||
-      (rc = ensure(ctx, l.pconic, c * 16, "projected-conic")) ||
-      (rc = ensure(ctx, l.pcolor, c * 16, "projected-color")) ||
+      (rc = ensure(ctx, l.splat, c * sizeof(gs::Splat), "projected-splats")) ||
       (rc = ensure(ctx, l.tile_counts, c * 4, "tile-counts")) ||
-      (rc = ensure(ctx, l.hits, c * 16, "tile-hits")) ||
       (rc = ensure(ctx, l.ordered_hits, c * 16, "ordered-tile-hits")) ||
       (rc = ensure(ctx, l.visible_offsets, c * 4, "visible-offsets")) ||
       (rc = ensure(ctx, l.depth_a, c * 8, "depth-records-a")) ||
@@ -255,8 +254,7 @@ int lane_create(gs_ctx* ctx, Lane& l, bool own_stream) {
 }
 
 void lane_destroy(gs_ctx* ctx, Lane& l) {
-  DeviceBuffer* all[] = {&l.pmean,     &l.pconic,         &l.pcolor,       &l.tile_counts,
-                         &l.hits,      &l.ordered_hits,
+  DeviceBuffer* all[] = {&l.pmean,     &l.splat,          &l.tile_counts,  &l.ordered_hits,
                          &l.visible_offsets, &l.depth_a,  &l.depth_b,      &l.ordered_counts,
                          &l.intersection_offsets, &l.records_a, &l.records_b, &l.hist,
                          &l.partials,  &l.tile_offsets,   &l.chunk_first,  &l.tasks,
@@ -325,6 +323,7 @@ extern "C" void gs_config_default(gs_config* c) {
   c->radix_bits = 8;
   c->raster_batch = 256;
   c->batch_lanes = 1;
+  c->heavy_split_units = GS_SPLIT_AUTO;
 }
 
 // GaussianPass.ts:157-260 option validation.
@@ -400,6 +399,11 @@ extern "C" int gs_create(const gs_config* config, gs_ctx** out) {
   if (hipSetDevice(c.device) != hipSuccess) return GS_E_DEVICE;
   gs_ctx* ctx = new gs_ctx();
   ctx->cfg = c;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) ==
+          hipSuccess &&
+      cus > 0)
+    ctx->cu_count = (uint32_t)cus;
   // Default stream priority only.
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->batch_fork, hipEventDisableTiming) != hipSuccess) {
@@ -753,6 +757,12 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   p.output_depth = ctx->cfg.output_depth ? 1u : 0u;
   p.raster_stats = ctx->cfg.raster_stats ? 1u : 0u;
   p.tile_size = ts;
+  // Heavy units run as row slabs above 2.5 units per CU: below that, a
+  // whole-tile block per unit already covers the CUs and slabs would only
+  // repeat the per-record staging (measured crossover on this GPU: 574
+  // units faster whole, 692 faster split).
+  p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 5 / 2
+                                                               : ctx->cfg.heavy_split_units;
   // Training-set frames: chunking off, so every recorded alpha * T uses the
   // true transmittance in front of it.
   if (ctx->cull.collecting) p.chunk_size = 0;
@@ -764,10 +774,8 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.scene.active_slots = (const uint32_t*)ctx->active.ptr;
   b.scene.objects = (const float*)(dyn + gs::kCameraBytes);
   b.proj.mean = (float4*)l.pmean.ptr;
-  b.proj.conic = (float4*)l.pconic.ptr;
-  b.proj.color = (float4*)l.pcolor.ptr;
+  b.proj.splat = (gs::Splat*)l.splat.ptr;
   b.proj.tile_counts = (uint32_t*)l.tile_counts.ptr;
-  b.proj.hits = (uint4*)l.hits.ptr;
   b.ordered_hits = (uint4*)l.ordered_hits.ptr;
   b.visible_offsets = (uint32_t*)l.visible_offsets.ptr;
   b.depth_a = (uint2*)l.depth_a.ptr;
@@ -1211,9 +1219,10 @@ extern "C" int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t by
   const void* src = nullptr;
   size_t len = 0;
   size_t a = b.params.active_count;
+  size_t pitch = 0;  // nonzero: 16-byte fields of the splat records, this far apart
   if (n == "projected_mean") { src = b.proj.mean; len = a * 16; }
-  else if (n == "projected_conic") { src = b.proj.conic; len = a * 16; }
-  else if (n == "projected_color") { src = b.proj.color; len = a * 16; }
+  else if (n == "projected_conic") { src = &b.proj.splat[0].conic; len = a * 16; pitch = sizeof(gs::Splat); }
+  else if (n == "projected_color") { src = &b.proj.splat[0].color; len = a * 16; pitch = sizeof(gs::Splat); }
   else if (n == "tile_counts") { src = b.proj.tile_counts; len = a * 4; }
   else if (n == "visible_offsets") { src = b.visible_offsets; len = a * 4; }
   else if (n == "depth_records") { src = gs::sorted_depth_records(b); len = (size_t)st.visible * 8; }
@@ -1228,7 +1237,9 @@ extern "C" int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t by
   if (!dst) return GS_OK;
   if (bytes != len) return fail(ctx, GS_E_RANGE, "debug buffer size mismatch for " + n);
   if (len == 0) return GS_OK;
-  hipError_t e = hipMemcpyAsync(dst, src, len, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t e = pitch ? hipMemcpy2DAsync(dst, 16, src, pitch, 16, len / 16,
+                                          hipMemcpyDeviceToHost, ctx->stream)
+                       : hipMemcpyAsync(dst, src, len, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "debug readback");
   return GS_OK;

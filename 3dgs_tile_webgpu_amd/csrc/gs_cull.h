@@ -230,15 +230,47 @@ __device__ inline float cull_forward_wave(const CullLdsNet& N, uint16_t* x, uint
   return y[l];
 }
 
+typedef __bf16 gs_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float gs_f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t gs_u32x2 __attribute__((ext_vector_type(2)));
+
+// Two binary32 -> bfloat16 (round to nearest even), `lo` in the low half:
+// one v_cvt_pk_bf16_f32 on gfx950. Equal to gs_f2bf for every non-NaN input.
+__device__ __forceinline__ uint32_t bf16_pack2(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(gs_f32x2{lo, hi}, gs_bf16x2));
+}
+typedef short gs_i16x2 __attribute__((ext_vector_type(2)));
+
+// ReLU on bit patterns: a negative binary32 or bfloat16 has its sign bit
+// set, so as a signed integer it is below zero (one integer max instead of
+// a NaN-canonicalising float max pair); -0 becomes +0 as with x > 0 ? x : 0.
+__device__ __forceinline__ float relu_bits(float v) {
+  int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, b > 0 ? b : 0);
+}
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(gs_i16x2, v),
+                                                                gs_i16x2{0, 0}));
+}
+// bf16(relu(z)) for the four rows of a D fragment (rounding first gives the
+// same bits: rounding preserves the sign).
+__device__ __forceinline__ gs_bf16x4 bf16x4_relu(gs_f32x4 z) {
+  return __builtin_bit_cast(gs_bf16x4, gs_u32x2{relu_bf16x2(bf16_pack2(z[0], z[1])),
+                                                relu_bf16x2(bf16_pack2(z[2], z[3]))});
+}
+
 // Inference forward pass of one wave, computed transposed so that no
 // activation goes through LDS: Z1^T = W1 X^T (M = hidden, N = samples, K =
 // inputs) leaves each lane holding hidden units 16mt + 4g + r of sample
 // 16nt + i, which is exactly the B operand of Z2^T = W2 A1^T for K tile
-// kt = mt. Layer 3 sums over the hidden units in registers and across the
-// four lane groups. Lane l holds the raw features of sample l (zeros for
-// idle lanes) and gets that sample's prediction; xs is the wave's 64 x 16
-// bf16 LDS rows for the standardised inputs. Same bf16 operands and fp32
-// accumulation as cull_forward_wave.
+// kt = mt. The biases enter as the accumulators' initial values (a lane's
+// four rows are four consecutive hidden units), and layer 3 sums over the
+// hidden units in registers and across the four lane groups. Lane l holds
+// the raw features of sample l (zeros for idle lanes) and gets that
+// sample's prediction; xs is the wave's 64 x 16 bf16 LDS rows for the
+// standardised inputs. Same bf16 operands as cull_forward_wave, fp32
+// accumulation (the bias is added inside the accumulation rather than
+// after it).
 __device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
                                         const float f[kCullIn]) {
   const int l = __lane_id();
@@ -246,9 +278,8 @@ __device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
   uint32_t packed[kCullIn / 2];
 #pragma unroll
   for (int k = 0; k < kCullIn / 2; k++)
-    packed[k] = (uint32_t)gs_f2bf((f[2 * k] - N.mu[2 * k]) * N.inv_sigma[2 * k]) |
-                ((uint32_t)gs_f2bf((f[2 * k + 1] - N.mu[2 * k + 1]) * N.inv_sigma[2 * k + 1])
-                 << 16);
+    packed[k] = bf16_pack2((f[2 * k] - N.mu[2 * k]) * N.inv_sigma[2 * k],
+                           (f[2 * k + 1] - N.mu[2 * k + 1]) * N.inv_sigma[2 * k + 1]);
   uint4* row = reinterpret_cast<uint4*>(&xs[l * kCullIn]);
   row[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
   row[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
@@ -257,23 +288,15 @@ __device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) xb[nt] = lds4(&xs[(16 * nt + i) * kCullIn + 4 * g]);
   wave_sync();  // the rows may be rewritten by the next call
-  // Layer 1 (transposed), bias, ReLU, bf16: the B operands of layer 2.
+  // Layer 1 (transposed), ReLU, bf16: the B operands of layer 2.
   gs_bf16x4 a1t[4][4];  // [kt][nt]
 #pragma unroll
   for (int mt = 0; mt < 4; mt++) {
     gs_bf16x4 a = lds4(&N.w1[(16 * mt + i) * kCullIn + 4 * g]);
     float4 b1 = *reinterpret_cast<const float4*>(&N.b1[16 * mt + 4 * g]);
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-      gs_f32x4 z = mfma16(a, xb[nt], gs_f32x4{0.f, 0.f, 0.f, 0.f});
-      float h0 = z[0] + b1.x, h1 = z[1] + b1.y, h2 = z[2] + b1.z, h3 = z[3] + b1.w;
-      gs_bf16x4 v;
-      v[0] = (short)gs_f2bf(h0 > 0.0f ? h0 : 0.0f);
-      v[1] = (short)gs_f2bf(h1 > 0.0f ? h1 : 0.0f);
-      v[2] = (short)gs_f2bf(h2 > 0.0f ? h2 : 0.0f);
-      v[3] = (short)gs_f2bf(h3 > 0.0f ? h3 : 0.0f);
-      a1t[mt][nt] = v;
-    }
+    for (int nt = 0; nt < 4; nt++)
+      a1t[mt][nt] = bf16x4_relu(mfma16(a, xb[nt], gs_f32x4{b1.x, b1.y, b1.z, b1.w}));
   }
   // Layer 2 (transposed) with layer 3 folded in.
   float yp[4] = {0.f, 0.f, 0.f, 0.f};
@@ -286,14 +309,13 @@ __device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
     float4 w3 = *reinterpret_cast<const float4*>(&N.w3[16 * mt + 4 * g]);
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
-      gs_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      gs_f32x4 acc = {b2.x, b2.y, b2.z, b2.w};
 #pragma unroll
       for (int kt = 0; kt < 4; kt++) acc = mfma16(w[kt], a1t[kt][nt], acc);
-      float v0 = acc[0] + b2.x, v1 = acc[1] + b2.y, v2 = acc[2] + b2.z, v3 = acc[3] + b2.w;
-      yp[nt] += (v0 > 0.0f ? v0 : 0.0f) * w3.x;
-      yp[nt] += (v1 > 0.0f ? v1 : 0.0f) * w3.y;
-      yp[nt] += (v2 > 0.0f ? v2 : 0.0f) * w3.z;
-      yp[nt] += (v3 > 0.0f ? v3 : 0.0f) * w3.w;
+      yp[nt] = __builtin_fmaf(relu_bits(acc[0]), w3.x, yp[nt]);
+      yp[nt] = __builtin_fmaf(relu_bits(acc[1]), w3.y, yp[nt]);
+      yp[nt] = __builtin_fmaf(relu_bits(acc[2]), w3.z, yp[nt]);
+      yp[nt] = __builtin_fmaf(relu_bits(acc[3]), w3.w, yp[nt]);
     }
   }
 #pragma unroll

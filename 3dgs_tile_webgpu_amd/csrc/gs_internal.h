@@ -60,6 +60,7 @@ struct FrameParams {
   uint32_t output_depth;
   uint32_t raster_stats;
   uint32_t tile_size;               // tile edge: 8, 16 or 32 (16 matches constants.ts:1)
+  uint32_t split_units;             // heavy units above which 16x16 tiles run as row slabs
 };
 
 // Device-side counters of one frame (dispatch state + outputState).
@@ -75,6 +76,7 @@ struct DeviceState {
   unsigned long long checked_heavy;  // the part of `checked` in heavy units
   uint32_t sort_error;  // a look-back exceeded its wait bound: bit 0 in a
                         // radix pass, bit 1 in a scan
+  uint32_t heavy_split;  // this frame's heavy units run as row slabs
 };
 
 struct SceneView {
@@ -86,12 +88,21 @@ struct SceneView {
   const float* objects;
 };
 
+// A visible Gaussian's projection in one 64-byte line, so that the raster's
+// gather of a key (by active ordinal, in depth order within a tile, hence
+// scattered) reads one line instead of three: mean (cx, cy, depth,
+// opacity), conic (a, b, c, rX + 512 rY), colour (r, g, b, slot bits) and
+// the tile-hit record that emission replays (see kHitMask).
+struct alignas(64) Splat {
+  float4 mean, conic, color;
+  uint4 hits;
+};
+static_assert(sizeof(Splat) == 64, "one cache line per splat");
+
 struct ProjectedView {
-  float4* mean;
-  float4* conic;
-  float4* color;
-  uint32_t* tile_counts;
-  uint4* hits;  // per ordinal: tile-hit record (see kHitMask)
+  float4* mean;           // per ordinal, every active Gaussian (visibility, depth)
+  Splat* splat;           // per ordinal, written for visible Gaussians only
+  uint32_t* tile_counts;  // per ordinal
 };
 
 // Length of a device-sized sequence: (*ptr rounded up to a multiple of

@@ -273,10 +273,12 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
 GS_DEV void store_projection(const ProjectedView& out, uint32_t gid, const ProjOut& o) {
   if (o.kind == kProjVisible) {
     out.mean[gid] = o.mean;
-    out.conic[gid] = o.conic;
-    out.color[gid] = o.color;
+    Splat* s = &out.splat[gid];
+    s->mean = o.mean;
+    s->conic = o.conic;
+    s->color = o.color;
+    s->hits = o.hits;
     out.tile_counts[gid] = o.count;
-    out.hits[gid] = o.hits;
   } else {
     out.tile_counts[gid] = 0u;
     out.mean[gid] = o.kind == kProjSubpixel ? o.mean : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -327,11 +329,13 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
     state->checked_heavy = 0ull;
     state->sort_error = 0u;
   }
+  // The network's loads are issued first, so that their latency overlaps
+  // the projection's.
+  cull_stage_net(net, lnet);
   float feat[kCullIn];
   ProjOut o;
   o.kind = kProjNone;
   if (gid < fp.active_count) o = project_gaussian<kModeCull>(fp, sc, gid, feat);
-  cull_stage_net(net, lnet);
   __syncthreads();
   bool visible = o.kind == kProjVisible;
   if (__ballot(visible) != 0ull) {  // wave-uniform
@@ -379,10 +383,10 @@ struct LoadVisible {  // scan.ts:78-83: 1 if projectedMean.w > 0
 };
 struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
   const uint2* sorted;
-  const uint4* hits;       // per ordinal; .w is the tile count
-  uint4* ordered_hits;     // the same records in depth order, for emission
+  const Splat* splat;      // per ordinal; hits.w is the tile count
+  uint4* ordered_hits;     // the hit records in depth order, for emission
   GS_DEV uint32_t operator()(uint32_t r) const {
-    uint4 h = hits[sorted[r].y];
+    uint4 h = splat[sorted[r].y].hits;
     ordered_hits[r] = h;
     return h.w;
   }
@@ -775,8 +779,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
 __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
                                                  const DeviceState* __restrict__ state,
                                                  const uint2* __restrict__ sorted_depth,
-                                                 const float4* __restrict__ pmean,
-                                                 const float4* __restrict__ pconic,
+                                                 const Splat* __restrict__ splat,
                                                  const uint4* __restrict__ ordered_hits,
                                                  const uint32_t* __restrict__ offsets,
                                                  uint2* __restrict__ records) {
@@ -797,8 +800,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
       if (base + local < capacity) records[base + local] = make_uint2(tile, gid);
     }
   } else {
-    float4 mean = pmean[gid];
-    float4 conic4 = pconic[gid];
+    float4 mean = splat[gid].mean;
+    float4 conic4 = splat[gid].conic;
     uint32_t packed = gs_f2u(conic4.w);
     float rx = (float)(packed % 512u), ry = (float)(packed / 512u);
     float conic[3] = {conic4.x, conic4.y, conic4.z};
@@ -952,17 +955,30 @@ struct PixelSum {
   uint32_t checked, blended;
 };
 
-// Block shapes. A tile of edge E has E*E pixels, one per thread. Light
-// tiles (up to kLightMax records) use max(E*E, batch) threads and stage
-// `batch` records per round (128 or 256, tunable); heavy units use 1024
-// threads and stage 1024 records per round (512 for 32-pixel tiles, whose
-// candidate masks would not fit the LDS otherwise).
+// Block shapes. A tile of edge E has E*E pixels. Light tiles (up to
+// kLightMax records) use one thread per pixel, max(E*E, batch) threads, and
+// stage `batch` records per round (128 or 256, tunable). Heavy units (tiles
+// above kLightMax records and chunk tasks) use four lanes per pixel at
+// edges 8 and 16 and stage one record per thread per round:
+//  - whole tiles: one block per unit (256 threads at edge 8, 1024 at 16);
+//  - row slabs (edge 16, frames with more than FrameParams::split_units
+//    units): four blocks of 256 threads per unit, each covering 4 rows. Such
+//    blocks share a CU four at a time and each stops as soon as its own
+//    pixels are saturated, at the price of staging every record once per
+//    slab, which only pays when the units outnumber the CUs.
+// Edge 32 uses one block of 1024 threads per unit (one lane per pixel, 512
+// records per round).
 constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
-constexpr int kHeavyThreads = 1024;  // edge 32; edges 8 and 16 use four lanes per pixel
+constexpr int kHeavyThreads = 1024;  // edge 32
 constexpr int log2_edge(int e) { return e == 8 ? 3 : (e == 16 ? 4 : 5); }
 constexpr int light_threads(int e, int batch) { return e * e > batch ? e * e : batch; }
-constexpr int heavy_batch(int e) { return e == 32 ? 512 : 4 * e * e; }
-constexpr int heavy_threads(int e) { return e == 32 ? kHeavyThreads : 4 * e * e; }
+constexpr int heavy_slab_rows(int e, bool split) { return e == 16 && split ? 4 : e; }
+constexpr int heavy_slabs(int e, bool split) { return e / heavy_slab_rows(e, split); }
+constexpr int heavy_slab_pixels(int e, bool split) { return e * heavy_slab_rows(e, split); }
+constexpr int heavy_threads(int e, bool split) {
+  return e == 32 ? kHeavyThreads : 4 * heavy_slab_pixels(e, split);
+}
+constexpr int heavy_batch(int e, bool split) { return e == 32 ? 512 : heavy_threads(e, split); }
 
 template <int kPixels, int kBatch>
 struct RasterLds {
@@ -1001,9 +1017,7 @@ GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, 
 template <int kEdge, int kThreads, int kBatch, bool kRecord>
 GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
                                  const uint2* __restrict__ sorted,
-                                 const float4* __restrict__ pmean,
-                                 const float4* __restrict__ pconic,
-                                 const float4* __restrict__ pcolor,
+                                 const Splat* __restrict__ splat,
                                  RasterLds<kEdge * kEdge, kBatch>& L, uint32_t& x, uint32_t& y,
                                  bool& active, uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
@@ -1028,9 +1042,9 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   uint32_t cg = 0u;
   if (stager && idx < ce) {
     cg = sorted[idx].y;
-    m = pmean[cg];
-    c = pconic[cg];
-    k = pcolor[cg];
+    m = splat[cg].mean;
+    c = splat[cg].conic;
+    k = splat[cg].color;
   }
   __syncthreads();
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
@@ -1050,9 +1064,9 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
     uint32_t nidx = batch + kBatch + t;
     if (stager && nidx < ce) {
       cg = sorted[nidx].y;
-      m = pmean[cg];
-      c = pconic[cg];
-      k = pcolor[cg];
+      m = splat[cg].mean;
+      c = splat[cg].conic;
+      k = splat[cg].color;
     }
     // Candidate pixels of record t.
     if (mine) {
@@ -1141,34 +1155,37 @@ GS_DEV uint32_t quad_or(uint32_t v) {
   return v;
 }
 
-// Heavy units with four lanes per pixel (tile edges 8 and 16). Records are
-// staged one per thread (4 * E*E per round) and tested against their support
-// rectangles exactly as in blend_tile_range. The blend of pixel p is shared
-// by lanes 4p..4p+3: per step the quad takes the pixel's next four
-// candidates in record order, lane q evaluates the q-th one (power,
-// exponential and alpha, the expensive part), the quad exchanges the four
-// results (DPP quad broadcasts) and every lane applies the same four blends
-// in the same order, so the four lanes hold the same sum, bit for bit the
-// one-lane result. Next nonzero mask words come from a per-pixel summary
-// word, so no lane scans empty words. The record ordinals of round r + 2
-// are loaded while round r runs, the records of round r + 1 gathered.
-template <int kEdge, bool kRecord>
-GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
-                                 const uint2* __restrict__ sorted,
-                                 const float4* __restrict__ pmean,
-                                 const float4* __restrict__ pconic,
-                                 const float4* __restrict__ pcolor,
-                                 RasterLds<kEdge * kEdge, 4 * kEdge * kEdge>& L, uint32_t& x,
-                                 uint32_t& y, bool& active, uint32_t* __restrict__ record) {
-  constexpr int kPixels = kEdge * kEdge;
+// Heavy-unit slabs with four lanes per pixel (tile edges 8 and 16). The
+// block covers kSlabRows rows of the tile starting at row slab * kSlabRows.
+// Records are staged kBatch per round, kBatch / kThreads per thread, and
+// tested against their support rectangles (clipped to the slab's rows)
+// exactly as in blend_tile_range. The blend of pixel p is shared by lanes
+// 4p..4p+3: per step the quad takes the pixel's next four candidates in
+// record order, lane q evaluates the q-th one (power, exponential and alpha,
+// the expensive part), the quad exchanges the four results (DPP quad
+// broadcasts) and every lane applies the same four blends in the same
+// order, so the four lanes hold the same sum, bit for bit the one-lane
+// result. Next nonzero mask words come from a per-pixel summary word, so no
+// lane scans empty words. The record ordinals of round r + 2 are loaded
+// while round r runs, the records of round r + 1 gathered.
+template <int kEdge, int kSlabRows, int kBatch, bool kRecord>
+GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t slab,
+                                 uint32_t cb, uint32_t ce, const uint2* __restrict__ sorted,
+                                 const Splat* __restrict__ splat,
+                                 RasterLds<kEdge * kSlabRows, kBatch>& L,
+                                 uint32_t& x, uint32_t& y, bool& active,
+                                 uint32_t* __restrict__ record) {
+  constexpr int kPixels = kEdge * kSlabRows;
   constexpr int kThreads = 4 * kPixels;
-  constexpr int kBatch = kThreads;
+  constexpr int kPer = kBatch / kThreads;  // records staged per thread per round
   constexpr int kMaskWords = kBatch / 32;
+  static_assert(kBatch % kThreads == 0, "whole records per thread");
   static_assert(kMaskWords <= 32, "the summary word covers 32 mask words");
   const uint32_t t = threadIdx.x, q = t & 3u, pix = t >> 2;
-  int tx0 = (int)((tile % fp.tiles_x) * kEdge), ty0 = (int)((tile / fp.tiles_x) * kEdge);
+  int tx0 = (int)((tile % fp.tiles_x) * kEdge);
+  int sy0 = (int)((tile / fp.tiles_x) * kEdge + slab * kSlabRows);  // first row of the slab
   x = (uint32_t)tx0 + (pix & (uint32_t)(kEdge - 1));
-  y = (uint32_t)ty0 + (pix >> log2_edge(kEdge));
+  y = (uint32_t)sy0 + (pix >> log2_edge(kEdge));
   active = x < fp.width && y < fp.height;
   float pxc = (float)x + 0.5f, pyc = (float)y + 0.5f;
   float vw = fp.viewport[0], vh = fp.viewport[1];
@@ -1176,53 +1193,73 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
   float threshold = fp.transmittance_threshold;
   PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
   bool done = !active;
-  uint32_t cg = 0u, cg_next = 0u;
-  float4 m = make_float4(0.f, 0.f, 0.f, 0.f), c = m, k = m;
-  if (cb + t < ce) {
-    cg = sorted[cb + t].y;
-    m = pmean[cg];
-    c = pconic[cg];
-    k = pcolor[cg];
+  uint32_t cg[kPer], cg_next[kPer];
+  float4 m[kPer], c[kPer], k[kPer];
+#pragma unroll
+  for (int e = 0; e < kPer; e++) {
+    uint32_t r = cb + e * kThreads + t;
+    cg[e] = cg_next[e] = 0u;
+    m[e] = c[e] = k[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r < ce) {
+      cg[e] = sorted[r].y;
+      m[e] = splat[cg[e]].mean;
+      c[e] = splat[cg[e]].conic;
+      k[e] = splat[cg[e]].color;
+    }
+    if (r + kBatch < ce) cg_next[e] = sorted[r + kBatch].y;
   }
-  if (cb + kBatch + t < ce) cg_next = sorted[cb + kBatch + t].y;
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
     for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
       (&L.mask[0][0])[i] = 0u;
     __syncthreads();
-    bool mine = batch + t < ce;
-    float thr = 0.0f;
-    if (mine) {
-      thr = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
-      L.mean[t] = m;
-      L.conic[t] = make_float4(c.x, c.y, c.z, thr);
-      L.color[t] = k;
-      if (kRecord) L.gid[t] = cg;
+    float4 cm[kPer], cc[kPer];
+    bool mine[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+      uint32_t slot = e * kThreads + t;
+      mine[e] = batch + slot < ce;
+      float thr = 0.0f;
+      if (mine[e]) {
+        thr = gs_logf(m[e].w * 255.0f);  // TileRasterizer.ts:579-581
+        L.mean[slot] = m[e];
+        L.conic[slot] = make_float4(c[e].x, c[e].y, c[e].z, thr);
+        L.color[slot] = k[e];
+        if (kRecord) L.gid[slot] = cg[e];
+      }
+      cm[e] = m[e];
+      cc[e] = make_float4(c[e].x, c[e].y, c[e].z, thr);
+      uint32_t r = batch + kBatch + slot;
+      if (r < ce) {
+        cg[e] = cg_next[e];
+        m[e] = splat[cg[e]].mean;
+        c[e] = splat[cg[e]].conic;
+        k[e] = splat[cg[e]].color;
+      }
+      if (r + kBatch < ce) cg_next[e] = sorted[r + kBatch].y;
     }
-    float4 cm = m, cc = make_float4(c.x, c.y, c.z, thr);
-    if (batch + kBatch + t < ce) {
-      cg = cg_next;
-      m = pmean[cg];
-      c = pconic[cg];
-      k = pcolor[cg];
-    }
-    if (batch + 2u * kBatch + t < ce) cg_next = sorted[batch + 2u * kBatch + t].y;
-    if (mine) {
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+      if (!mine[e]) continue;
       int x0, x1, y0, y1;
       if (axis == 0)
-        support_rect(cm.x, cm.y, cc.x, cc.y, cc.z, cc.w, tx0, ty0, kEdge, x0, x1, y0, y1);
+        support_rect(cm[e].x, cm[e].y, cc[e].x, cc[e].y, cc[e].z, cc[e].w, tx0,
+                     (int)((tile / fp.tiles_x) * kEdge), kEdge, x0, x1, y0, y1);
       else {
         x0 = tx0;
         x1 = tx0 + kEdge - 1;
-        y0 = ty0;
-        y1 = ty0 + kEdge - 1;
+        y0 = sy0;
+        y1 = sy0 + kSlabRows - 1;
       }
-      uint32_t bit = 1u << (t & 31u), word = t >> 5;
+      if (y0 < sy0) y0 = sy0;
+      if (y1 > sy0 + kSlabRows - 1) y1 = sy0 + kSlabRows - 1;
+      uint32_t slot = e * kThreads + t;
+      uint32_t bit = 1u << (slot & 31u), word = slot >> 5;
       for (int py = y0; py <= y1; py++) {
         float pyf = (float)py + 0.5f;
         for (int px = x0; px <= x1; px++) {
-          float power = pair_power((float)px + 0.5f, pyf, cm, cc, vw, vh, axis);
-          if (!(power > 0.0f || power < -cc.w))
-            atomicOr(&L.mask[word][(uint32_t)((py - ty0) * kEdge + (px - tx0))], bit);
+          float power = pair_power((float)px + 0.5f, pyf, cm[e], cc[e], vw, vh, axis);
+          if (!(power > 0.0f || power < -cc[e].w))
+            atomicOr(&L.mask[word][(uint32_t)((py - sy0) * kEdge + (px - tx0))], bit);
         }
       }
     }
@@ -1351,8 +1388,7 @@ constexpr uint32_t kLightBlocks = 2048;
 template <int kEdge, int kBatch, bool kRecord>
 __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
-    const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
-    const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
+    const uint32_t* __restrict__ off, const Splat* __restrict__ splat,
     uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
     uint32_t* __restrict__ record) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
@@ -1372,7 +1408,7 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     uint32_t x, y;
     bool active;
     PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
-        fp, tile, begin, end, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
+        fp, tile, begin, end, sorted, splat, lds, x, y, active, record);
     if (fp.raster_stats) add_stats(state, s.checked, s.blended, false);
     if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
   }
@@ -1436,6 +1472,7 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
       next += class_count[c];
     }
     state->heavy_units = min(next, unit_capacity);
+    state->heavy_split = next > fp.split_units ? 1u : 0u;
   }
   __syncthreads();
 #pragma unroll 4
@@ -1449,20 +1486,29 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
   }
 }
 
-// Heavy units: one block per chunk task or heavy tile (four lanes per pixel
-// for edges 8 and 16, one for edge 32).
-template <int kEdge, bool kRecord>
-__global__ __launch_bounds__(heavy_threads(kEdge)) void k_raster_heavy(
+// Heavy units: per chunk task or heavy tile, S = heavy_slabs(E, kSplit)
+// blocks, one per row slab; the kSplit and whole-tile launches both run and
+// the one the frame did not choose (DeviceState::heavy_split, set by
+// k_heavy_units) exits at once. Block b takes unit u = (j / S) * 8 + b % 8
+// and slab j % S, with j = b / 8: workgroups go to the eight XCDs in turn,
+// so a unit's slabs run on one XCD and share its L2, and units start in list
+// order (heaviest first). The grid is the unit capacity rounded up to 8,
+// times S.
+template <int kEdge, bool kSplit, bool kRecord>
+__global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
-    const uint32_t* __restrict__ off, const float4* __restrict__ pmean,
-    const float4* __restrict__ pconic, const float4* __restrict__ pcolor,
+    const uint32_t* __restrict__ off, const Splat* __restrict__ splat,
     const uint2* __restrict__ units, float4* __restrict__ partials,
     float* __restrict__ partial_depth, uint16_t* __restrict__ color_out,
     float* __restrict__ depth_out, uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr bool kQuad = kEdge != 32;
-  __shared__ RasterLds<kPixels, heavy_batch(kEdge)> lds;
-  uint32_t u = blockIdx.x;
+  constexpr uint32_t kSlabs = heavy_slabs(kEdge, kSplit);
+  constexpr int kBatch = heavy_batch(kEdge, kSplit);
+  __shared__ RasterLds<heavy_slab_pixels(kEdge, kSplit), kBatch> lds;
+  if (kEdge == 16 && (state->heavy_split != 0u) != kSplit) return;
+  const uint32_t j = blockIdx.x >> 3;
+  const uint32_t u = (j / kSlabs) * 8u + (blockIdx.x & 7u), slab = j % kSlabs;
   if (u >= state->heavy_units) return;
   uint2 unit = units[u];
   uint32_t tile = unit.x;
@@ -1476,16 +1522,16 @@ __global__ __launch_bounds__(heavy_threads(kEdge)) void k_raster_heavy(
   uint32_t x, y;
   bool active;
   PixelSum s;
-  uint32_t pix = threadIdx.x;
+  uint32_t pix = threadIdx.x;  // pixel within the tile
   bool writer = threadIdx.x < (uint32_t)kPixels;
   if constexpr (kQuad) {
-    s = blend_heavy_quad<kEdge, kRecord>(fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x,
-                                         y, active, record);
-    pix = threadIdx.x >> 2;
+    s = blend_heavy_quad<kEdge, heavy_slab_rows(kEdge, kSplit), kBatch, kRecord>(
+        fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
+    pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> 2);
     writer = (threadIdx.x & 3u) == 0u;
   } else {
-    s = blend_tile_range<kEdge, kHeavyThreads, heavy_batch(kEdge), kRecord>(
-        fp, tile, cb, ce, sorted, pmean, pconic, pcolor, lds, x, y, active, record);
+    s = blend_tile_range<kEdge, kHeavyThreads, kBatch, kRecord>(
+        fp, tile, cb, ce, sorted, splat, lds, x, y, active, record);
   }
   if (fp.raster_stats) add_stats(state, writer ? s.checked : 0u, writer ? s.blended : 0u, true);
   if (!writer) return;
@@ -1496,6 +1542,12 @@ __global__ __launch_bounds__(heavy_threads(kEdge)) void k_raster_heavy(
   } else if (active) {
     store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
   }
+}
+
+// Blocks of k_raster_heavy for `units` heavy units.
+template <int kEdge, bool kSplit>
+inline uint32_t heavy_grid(uint32_t units) {
+  return ((units + 7u) & ~7u) * (uint32_t)heavy_slabs(kEdge, kSplit);
 }
 
 // TileRasterizer.ts:807-902: ordered over-composite of a tile's chunks.
@@ -1582,15 +1634,21 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
   if (e != hipSuccess) return e;
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
-  k_raster_heavy<kEdge, kRecord><<<b.unit_capacity, heavy_threads(kEdge), 0, aux>>>(
-      fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.tasks,
-      b.chunk_partials, b.partial_depth, b.color_out, b.depth_out, b.record);
+  k_raster_heavy<kEdge, false, kRecord><<<heavy_grid<kEdge, false>(b.unit_capacity),
+                                           heavy_threads(kEdge, false), 0, aux>>>(
+      fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials, b.partial_depth,
+      b.color_out, b.depth_out, b.record);
+  if constexpr (kEdge == 16)
+    k_raster_heavy<kEdge, true, kRecord><<<heavy_grid<kEdge, true>(b.unit_capacity),
+                                           heavy_threads(kEdge, true), 0, aux>>>(
+        fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+        b.partial_depth, b.color_out, b.depth_out, b.record);
   if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], aux);
   e = hipEventRecord(join, aux);
   if (e != hipSuccess) return e;
   uint32_t light_blocks = b.max_tiles < kLightBlocks ? b.max_tiles : kLightBlocks;
   k_raster_light<kEdge, kBatch, kRecord><<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(
-      fp, st, sorted, b.tile_offsets, b.proj.mean, b.proj.conic, b.proj.color, b.color_out,
+      fp, st, sorted, b.tile_offsets, b.proj.splat, b.color_out,
       b.depth_out, b.record);
   e = hipStreamWaitEvent(s, join, 0);
   if (e != hipSuccess) return e;
@@ -1655,13 +1713,12 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   mark(kStageScan);
   LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
   k_scan<LoadOrdered, StoreOrdered><<<scan_blocks, kBlock, 0, s>>>(
-      LoadOrdered{sorted_depth, b.proj.hits, b.ordered_hits},
+      LoadOrdered{sorted_depth, b.proj.splat, b.ordered_hits},
       StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, sws, kScanIntersections,
       fp.capacity, fp.camera, st);
 
   mark(kStageEmit);
-  k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.mean, b.proj.conic,
-                                     b.ordered_hits, b.intersection_offsets, b.records_a);
+  k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.splat, b.ordered_hits, b.intersection_offsets, b.records_a);
 
   mark(kStageTileSort);
   radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, ws, st,

@@ -47,7 +47,7 @@ DEFAULT_OPTIONS = {
 }
 KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | {
     "fixedSize", "depthRange", "intersectionCapacity", "initialIntersectionCapacity",
-    "maxIntersectionCapacity", "device", "dataMode", "batchLanes",
+    "maxIntersectionCapacity", "device", "dataMode", "batchLanes", "heavySplitUnits",
 }
 MAX_CAPACITY = 1 << 28
 
@@ -129,6 +129,10 @@ def validate_options(options: dict) -> dict:
     lanes = o.get("batchLanes", 1)
     if lanes != 1:
         raise N.GsRangeError(N.GS_E_RANGE, "batch lanes above 1 are disabled")
+    split = o.get("heavySplitUnits")
+    if split is not None and (not isinstance(split, int) or isinstance(split, bool)
+                              or not 0 <= split < 0xFFFFFFFF):
+        raise N.GsRangeError(N.GS_E_RANGE, "heavySplitUnits must be a non-negative integer")
     return o
 
 
@@ -153,6 +157,8 @@ def config_for(o: dict, policy: IntersectionCapacityPolicy) -> N.GsConfig:
     cfg.radix_bits = int(o["radixBits"])
     cfg.raster_batch = int(o["rasterBatch"])
     cfg.batch_lanes = int(o.get("batchLanes", 1))
+    split = o.get("heavySplitUnits")
+    cfg.heavy_split_units = 0xFFFFFFFF if split is None else int(split)
     return cfg
 
 

@@ -30,6 +30,9 @@ export interface GaussianPassOptions {
   tileSize?: 8 | 16 | 32;
   radixBits?: 4 | 6 | 8;
   rasterBatch?: 128 | 256;
+  /** Heavy raster units above which 16-pixel tiles run as 4-row slabs
+   * (default: 2.5 per compute unit); never changes any output. */
+  heavySplitUnits?: number;
   device?: number;
 }
 

@@ -83,7 +83,14 @@ typedef struct gs_config {
   uint32_t raster_batch;             /* 128 or 256; default 256 */
   uint32_t batch_lanes;              /* concurrent frames of a batch; only 1
                                         is accepted at present (default 1) */
+  uint32_t heavy_split_units;        /* heavy raster units (tiles above 1024
+                                        records, chunk tasks) above which 16-pixel
+                                        tiles are rasterized as 4-row slabs;
+                                        GS_SPLIT_AUTO (default) = 2.5 per compute
+                                        unit; never changes any output */
 } gs_config;
+
+#define GS_SPLIT_AUTO 0xffffffffu
 
 /* Packed Gaussian buffers (GaussianData.ts:10-33 / GaussianBuffers), copied
  * during the call (host memory) or device-to-device (device memory). No

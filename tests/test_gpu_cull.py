@@ -29,10 +29,14 @@ def small(gs):
     return gs.workload_scene("small", count=60_000)
 
 
-def test_collected_features_and_record_match_oracle(gs, oracle, gpu_available, small):
+@pytest.mark.parametrize("split", [None, 0], ids=["auto", "slabs"])
+def test_collected_features_and_record_match_oracle(gs, oracle, gpu_available, small, split):
     w = gs.WORKLOADS["small"]
     cam = w.camera()
-    p = gs.gaussian_pass(cam, small, {"intersectionCapacity": 1 << 21})
+    opts = {"intersectionCapacity": 1 << 21}
+    if split is not None:
+        opts["heavySplitUnits"] = split  # the instrumented raster as 4-row slabs
+    p = gs.gaussian_pass(cam, small, opts)
     cull = gs.LearnedCull(p, seed=1)
     cull.reserve(1 << 17)
     st = cull.collect(cam, 640, 360, rate=1.0, seed=7)

@@ -89,6 +89,9 @@ def test_small_workload_bit_exact(gs, oracle, gpu_available, small):
     {"background": (0.2, 0.4, 0.6, 0.5), "rasterTransmittanceThreshold": 0.01},
     {"boundaryPolicy": "repeat-x"},
     {"boundaryPolicy": "repeat-y"},
+    {"heavySplitUnits": 0},
+    {"heavySplitUnits": 0, "rasterChunkSize": 256},
+    {"heavySplitUnits": 0, "boundaryPolicy": "repeat-x"},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_option_variants_bit_exact(gs, oracle, gpu_available, small, options):
     w = gs.WORKLOADS["small"]
@@ -96,6 +99,8 @@ def test_option_variants_bit_exact(gs, oracle, gpu_available, small, options):
     p, ref = render_both(gs, oracle, small, cam, 640, 360, options)
     if options.get("rasterChunkSize") == 256:
         assert ref.counts.chunk_tasks > 0
+    if "heavySplitUnits" in options:  # the slab path has heavy tiles to split
+        assert (np.diff(ref.get("tile_offsets").astype(np.int64)) > 1024).any()
     assert_frame_matches(p, ref)
     p.dispose()
 
@@ -171,10 +176,14 @@ def test_graph_replay_and_recapture(gs, oracle, gpu_available, small):
     p.dispose()
 
 
-def test_1m_1080p_bit_exact(gs, oracle, gpu_available):
+@pytest.mark.parametrize("split", [None, 0], ids=["auto", "slabs"])
+def test_1m_1080p_bit_exact(gs, oracle, gpu_available, split):
+    """Heavy units run as whole tiles here by default (275 units); with
+    heavySplitUnits=0 as 4-row slabs. Both must give the oracle's bits."""
     w = gs.WORKLOADS["1m_1080p"]
     scene = gs.workload_scene("1m_1080p")
-    p, ref = render_both(gs, oracle, scene, w.camera(), w.width, w.height)
+    opts = {} if split is None else {"heavySplitUnits": split}
+    p, ref = render_both(gs, oracle, scene, w.camera(), w.width, w.height, opts)
     assert ref.counts.chunk_tasks > 0  # dense cluster tiles take the chunk path
     assert_frame_matches(p, ref)
     p.dispose()
