@@ -46,7 +46,7 @@ struct Lane {
   hipEvent_t fork = nullptr, join = nullptr, done = nullptr;
   // Per-Gaussian frame arrays, sized for slot_capacity Gaussians.
   uint32_t slot_capacity = 0;
-  DeviceBuffer pmean, splat, tile_counts, visible_offsets, depth_a, depth_b,
+  DeviceBuffer pmean, splat, tile_counts, hits, visible_offsets, depth_a, depth_b,
       ordered_counts, ordered_hits, intersection_offsets;
   // Intersection records, sized for `capacity`.
   uint32_t capacity = 0;
@@ -193,6 +193,7 @@ int lane_slots(gs_ctx* ctx, Lane& l, uint32_t n) {
   if ((rc = ensure(ctx, l.pmean, c * 16, "projected-mean")) ||
       (rc = ensure(ctx, l.splat, c * sizeof(gs::Splat), "projected-splats")) ||
       (rc = ensure(ctx, l.tile_counts, c * 4, "tile-counts")) ||
+      (rc = ensure(ctx, l.hits, c * 16, "tile-hits")) ||
       (rc = ensure(ctx, l.ordered_hits, c * 16, "ordered-tile-hits")) ||
       (rc = ensure(ctx, l.visible_offsets, c * 4, "visible-offsets")) ||
       (rc = ensure(ctx, l.depth_a, c * 8, "depth-records-a")) ||
@@ -254,7 +255,8 @@ int lane_create(gs_ctx* ctx, Lane& l, bool own_stream) {
 }
 
 void lane_destroy(gs_ctx* ctx, Lane& l) {
-  DeviceBuffer* all[] = {&l.pmean,     &l.splat,          &l.tile_counts,  &l.ordered_hits,
+  DeviceBuffer* all[] = {&l.pmean,     &l.splat,          &l.tile_counts,  &l.hits,
+                         &l.ordered_hits,
                          &l.visible_offsets, &l.depth_a,  &l.depth_b,      &l.ordered_counts,
                          &l.intersection_offsets, &l.records_a, &l.records_b, &l.hist,
                          &l.partials,  &l.tile_offsets,   &l.chunk_first,  &l.tasks,
@@ -687,9 +689,10 @@ uint32_t task_capacity_for(const gs_config& c, uint32_t capacity) {
   return t < 1 ? 1 : t;
 }
 
-// Heavy raster units: the chunk tasks plus tiles above 1024 records.
-uint32_t unit_capacity_for(uint32_t task_capacity, uint32_t capacity) {
-  return task_capacity + div_up(capacity, 1024) + 1;
+// Raster units: the chunk tasks, tiles above 1024 records and (row-slab
+// frames) the light tiles.
+uint32_t unit_capacity_for(uint32_t task_capacity, uint32_t capacity, uint32_t tiles) {
+  return task_capacity + div_up(capacity, 1024) + 1 + tiles;
 }
 
 int lane_frame_buffers(gs_ctx* ctx, Lane& l, uint32_t capacity, uint32_t tiles) {
@@ -705,7 +708,7 @@ int lane_frame_buffers(gs_ctx* ctx, Lane& l, uint32_t capacity, uint32_t tiles) 
     l.tile_capacity = cap;
   }
   uint32_t tasks = task_capacity_for(ctx->cfg, l.capacity);
-  uint32_t units = unit_capacity_for(tasks, l.capacity);
+  uint32_t units = unit_capacity_for(tasks, l.capacity, l.tile_capacity);
   size_t px = (size_t)ctx->cfg.tile_size * ctx->cfg.tile_size;
   if ((rc = ensure(ctx, l.tasks, (size_t)units * 8, "raster-heavy-units")) ||
       (rc = ensure(ctx, l.partials_px, (size_t)tasks * px * 16, "raster-chunk-partials")) ||
@@ -776,6 +779,7 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.proj.mean = (float4*)l.pmean.ptr;
   b.proj.splat = (gs::Splat*)l.splat.ptr;
   b.proj.tile_counts = (uint32_t*)l.tile_counts.ptr;
+  b.proj.hits = (uint4*)l.hits.ptr;
   b.ordered_hits = (uint4*)l.ordered_hits.ptr;
   b.visible_offsets = (uint32_t*)l.visible_offsets.ptr;
   b.depth_a = (uint2*)l.depth_a.ptr;
@@ -790,7 +794,7 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.chunk_partials = (float4*)l.partials_px.ptr;
   b.partial_depth = (float*)l.partial_depth.ptr;
   b.task_capacity = task_capacity_for(ctx->cfg, capacity);
-  b.unit_capacity = unit_capacity_for(b.task_capacity, capacity);
+  b.unit_capacity = unit_capacity_for(b.task_capacity, capacity, l.tile_capacity);
   b.hist = (uint32_t*)l.hist.ptr;
   b.sort_parts = l.sort_parts;
   b.partials = (uint32_t*)l.partials.ptr;
@@ -802,6 +806,7 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.max_tiles = tiles;
   b.radix_bits = ctx->cfg.radix_bits;
   b.raster_batch = ctx->cfg.raster_batch;
+  b.cu_count = ctx->cu_count;
   uint32_t key_bits = ctx->cfg.depth_sort_mode == GS_DEPTH_FLOAT32 ? 32u : 16u;
   b.depth_passes = div_up(key_bits, b.radix_bits);
   b.tile_passes = div_up(bits_for(tiles), b.radix_bits);
@@ -876,7 +881,8 @@ hipError_t enqueue_set(gs_ctx* ctx, ViewSet& set, hipEvent_t* marks) {
   }
   for (uint32_t v = 0; v < set.n; v++) {
     Lane& l = ctx->lanes[set.lane_of[v]];
-    e = gs::launch_frame(set.frames[v], l.stream, l.aux, l.fork, l.join, marks);
+    e = gs::launch_frame(set.frames[v], l.stream,
+                         gs::RasterStreams{l.aux, l.fork, l.join}, marks);
     if (e != hipSuccess) return e;
   }
   for (uint32_t l = 1; l < used; l++) {

@@ -77,6 +77,8 @@ struct DeviceState {
   uint32_t sort_error;  // a look-back exceeded its wait bound: bit 0 in a
                         // radix pass, bit 1 in a scan
   uint32_t heavy_split;  // this frame's heavy units run as row slabs
+  uint32_t light_units;  // row-slab frames: light tiles listed after the heavy units
+  uint32_t heavy_next[8];  // per XCD: heavy work items taken beyond the first round
 };
 
 struct SceneView {
@@ -92,10 +94,10 @@ struct SceneView {
 // gather of a key (by active ordinal, in depth order within a tile, hence
 // scattered) reads one line instead of three: mean (cx, cy, depth,
 // opacity), conic (a, b, c, rX + 512 rY), colour (r, g, b, slot bits) and
-// the tile-hit record that emission replays (see kHitMask).
+// aux.x = ln(255 opacity), the raster's per-Gaussian cutoff
+// (TileRasterizer.ts:579-581), computed once by the projection.
 struct alignas(64) Splat {
-  float4 mean, conic, color;
-  uint4 hits;
+  float4 mean, conic, color, aux;
 };
 static_assert(sizeof(Splat) == 64, "one cache line per splat");
 
@@ -103,6 +105,7 @@ struct ProjectedView {
   float4* mean;           // per ordinal, every active Gaussian (visibility, depth)
   Splat* splat;           // per ordinal, written for visible Gaussians only
   uint32_t* tile_counts;  // per ordinal
+  uint4* hits;            // per ordinal: tile-hit record (see kHitMask)
 };
 
 // Length of a device-sized sequence: (*ptr rounded up to a multiple of
@@ -140,11 +143,12 @@ struct FrameBuffers {
   uint32_t* partials;   // scan workspace: one 64-bit status word per scan
                         // partition (zero between scans)
   uint32_t* chunk_first;  // per tile: first chunk task (tile capacity)
-  uint2* tasks;           // raster heavy units: (tile, chunk or ~0u)
+  uint2* tasks;           // raster units: (tile, chunk or ~0u); heavy units,
+                          // then (row-slab frames) the non-empty light tiles
   float4* chunk_partials; // per task and pixel: (C.rgb, T)
   float* partial_depth;   // per task and pixel: weighted depth
   uint32_t task_capacity;  // chunk tasks (partials slots)
-  uint32_t unit_capacity;  // heavy units (chunk tasks + heavy tiles)
+  uint32_t unit_capacity;  // raster units (chunk tasks + heavy tiles + tiles)
   uint16_t* color_out;  // W*H*4 binary16
   float* depth_out;     // W*H f32 or null
   uint32_t max_n;       // slot capacity of the per-Gaussian arrays
@@ -154,6 +158,7 @@ struct FrameBuffers {
   uint32_t tile_passes;
   uint32_t radix_bits;   // digit width: 4, 6 or 8
   uint32_t raster_batch; // records staged per light-tile round: 128 or 256
+  uint32_t cu_count;     // compute units of the device (persistent grids)
   // Learned cull: the network run inside projection (or null), and for
   // training-set collection the per-Gaussian features written by projection
   // and the per-Gaussian max alpha * T recorded by the raster (or null).
@@ -164,7 +169,7 @@ struct FrameBuffers {
 
 // Frame stages, in launch order; launch_frame can record an event before
 // each stage and one after the last (kStageCount + 1 events), then two on
-// the raster's second stream around the heavy-unit kernel (kMarks in all).
+// the raster's second stream around the heavy-unit kernels (kMarks in all).
 enum Stage : int {
   kStageProject = 0,
   kStageVisible,
@@ -180,12 +185,18 @@ constexpr int kMarkHeavyBegin = kStageCount + 1;
 constexpr int kMarkHeavyEnd = kStageCount + 2;
 constexpr int kMarks = kStageCount + 3;
 
+// The raster's second branch: the heavy-unit kernels run on aux, forked
+// from the frame's stream at `fork` and joined back into it at `join`,
+// concurrently with the light-tile kernel on the frame's stream.
+struct RasterStreams {
+  hipStream_t aux;
+  hipEvent_t fork, join;
+};
+
 // Enqueue one frame on `stream` (all kernels; sizes read on the device).
-// `marks` is null or kMarks events (stage boundaries, heavy-unit kernel).
-// The raster's light and heavy kernels run concurrently: `aux` is forked
-// from and joined back into `stream` through the two events.
-hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, hipStream_t aux,
-                        hipEvent_t fork, hipEvent_t join, hipEvent_t* marks);
+// `marks` is null or kMarks events (stage boundaries, heavy-unit kernels).
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, const RasterStreams& rs,
+                        hipEvent_t* marks);
 // Location of the sorted depth / tile records after launch_frame.
 const uint2* sorted_depth_records(const FrameBuffers& b);
 const uint2* sorted_tile_records(const FrameBuffers& b);

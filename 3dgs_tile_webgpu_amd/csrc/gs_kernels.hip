@@ -78,6 +78,7 @@ enum ProjKind : int { kProjNone = 0, kProjSubpixel = 1, kProjVisible = 2 };
 struct ProjOut {
   int kind;
   float4 mean, conic, color;
+  float thr;  // ln(255 opacity)
   uint32_t count;
   uint4 hits;  // HitRecord
 };
@@ -263,6 +264,7 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   o.mean = make_float4(p.cx, p.cy, p.depth, opacity);
   o.conic = make_float4(p.conic[0], p.conic[1], p.conic[2], rx + ry * 512.0f);
   o.color = make_float4(r, g, b, gs_b2f(slot ^ 0x3f800000u));
+  o.thr = thr;
   o.count = count;
   if (kMode != kModePlain)
     cull_features(feat, rx, ry, p.conic, opacity, p.depth, dir, local, sh_energy, p.cx, p.cy,
@@ -277,8 +279,9 @@ GS_DEV void store_projection(const ProjectedView& out, uint32_t gid, const ProjO
     s->mean = o.mean;
     s->conic = o.conic;
     s->color = o.color;
-    s->hits = o.hits;
+    s->aux = make_float4(o.thr, 0.0f, 0.0f, 0.0f);
     out.tile_counts[gid] = o.count;
+    out.hits[gid] = o.hits;
   } else {
     out.tile_counts[gid] = 0u;
     out.mean[gid] = o.kind == kProjSubpixel ? o.mean : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -383,10 +386,10 @@ struct LoadVisible {  // scan.ts:78-83: 1 if projectedMean.w > 0
 };
 struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
   const uint2* sorted;
-  const Splat* splat;      // per ordinal; hits.w is the tile count
-  uint4* ordered_hits;     // the hit records in depth order, for emission
+  const uint4* hits;       // per ordinal; .w is the tile count
+  uint4* ordered_hits;     // the same records in depth order, for emission
   GS_DEV uint32_t operator()(uint32_t r) const {
-    uint4 h = splat[sorted[r].y].hits;
+    uint4 h = hits[sorted[r].y];
     ordered_hits[r] = h;
     return h.w;
   }
@@ -891,49 +894,50 @@ __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
 // into consecutive chunks blended from T = 1 by independent tasks and
 // composited in order (TileRasterizer.ts:331-412, 807-902).
 
-// Tile-local rectangle [x0, x1] x [y0, y1] (global pixel indices) outside of
-// which the binary32 quadratic form q = c0 dx^2 + 2 c1 dx dy + c2 dy^2
-// provably exceeds 2 thr, i.e. the reference's test `power < -thr` holds.
-// With S = |c0| dx^2 + 2|c1 dx dy| + |c2| dy^2, the evaluated q obeys
-// q_hat >= q - g S (g = 3e-7 > 4 unit roundoffs), q >= (det / c2) dx^2 and
-// S <= (c0 + |c1|) dx^2 + (c2 + |c1|) dy^2; with |dy| <= Y over the tile,
-// q_hat > 2 thr whenever dx^2 (det / c2 - g (c0 + |c1|)) > 2 thr + g (c2 +
-// |c1|) Y^2. The same bound with x and y exchanged limits rows. Evaluated in
-// binary64 from the binary32 operands; pixels left out lie at least one
-// pixel beyond the bound.
-GS_DEV void support_rect(float cx, float cy, float c0, float c1, float c2, float thr, int tx0,
-                         int ty0, int edge, int& x0, int& x1, int& y0, int& y1) {
-  x0 = tx0;
-  x1 = tx0 + edge - 1;
-  y0 = ty0;
-  y1 = ty0 + edge - 1;
-  double C0 = c0, C1 = c1, C2 = c2, T = thr;
-  double det = C0 * C2 - C1 * C1;
-  if (!(det > 0.0 && C0 > 0.0 && C2 > 0.0 && T >= 0.0 && T < 1e30)) return;
+// Support rectangle of a record: the pixels outside [x0, x1] x [y0, y1]
+// (global pixel indices) are those where the binary32 quadratic form
+// q = c0 dx^2 + 2 c1 dx dy + c2 dy^2 provably exceeds 2 thr, i.e. the
+// reference's test `power < -thr` holds. With S = |c0| dx^2 + 2|c1 dx dy| +
+// |c2| dy^2, the evaluated q obeys q_hat >= q - g S (g = 3e-7 > 4 unit
+// roundoffs), q >= (det / c2) dx^2 and S <= (c0 + |c1|) dx^2 + (c2 + |c1|)
+// dy^2; so over any set of rows with |dy| <= Y, q_hat > 2 thr whenever
+// dx^2 (det / c2 - g (c0 + |c1|)) > 2 thr + g (c2 + |c1|) Y^2. The same
+// bound with x and y exchanged limits rows. Evaluated in binary64 from the
+// binary32 operands; pixels left out lie at least one pixel beyond the
+// bound. Rows are bounded first (over the block's columns), then columns
+// over the rows that remain, so a block covering part of a tile can drop a
+// record before looking at its columns.
+struct SupportForm {
+  bool bounded;  // positive definite with a finite cutoff
+  double c0, c1a, c2, det, two_t;
+};
+GS_DEV SupportForm support_form(float c0, float c1, float c2, float thr) {
+  SupportForm f;
+  f.c0 = c0;
+  f.c1a = fabs((double)c1);
+  f.c2 = c2;
+  f.det = f.c0 * f.c2 - (double)c1 * (double)c1;
+  double T = thr;
+  f.bounded = f.det > 0.0 && f.c0 > 0.0 && f.c2 > 0.0 && T >= 0.0 && T < 1e30;
+  f.two_t = 2.0 * T * (1.0 + 1e-6);
+  return f;
+}
+// Narrows [p0, p1] along one axis (coefficient cm, centre pc) given that the
+// other axis (coefficient co, centre oc) only takes pixels o0..o1. With
+// hx^2 = (two_t + g (co + |c1|) O^2) co / (det - g (cm + |c1|) co): the
+// quotient and root are taken in binary32 (a few roundings, < 1e-6
+// relative) and widened by a 1e-5 relative margin.
+GS_DEV void support_axis(const SupportForm& f, double cm, double co, float pc, float oc, int o0,
+                         int o1, int& p0, int& p1) {
   const double g = 3.0e-7;
-  double a1 = C0 + fabs(C1), a2 = C2 + fabs(C1);
-  double last = (double)edge - 0.5;
-  double X = fmax(fabs(tx0 + 0.5 - (double)cx), fabs(tx0 + last - (double)cx)) + 1.0;
-  double Y = fmax(fabs(ty0 + 0.5 - (double)cy), fabs(ty0 + last - (double)cy)) + 1.0;
-  double two_t = 2.0 * T * (1.0 + 1e-6);
-  // kx = det / c2 - g a1 > 0  <=>  det - g a1 c2 > 0 (c2 > 0); then
-  // hx^2 = (two_t + g a2 Y^2) c2 / (det - g a1 c2). The quotient and root
-  // are taken in binary32 (a few roundings, < 1e-6 relative) and widened by
-  // a 1e-5 relative margin.
-  double nx = det - g * a1 * C2, ny = det - g * a2 * C0;
-  if (nx > 0.0) {
-    float h2 = (float)((two_t + g * a2 * Y * Y) * C2) / (float)nx;
-    double hx = (double)gs_sqrtf(h2) * (1.0 + 1e-5) + 1e-6;
-    double lo = floor((double)cx - hx - 0.5), hi = ceil((double)cx + hx - 0.5);
-    if (lo > (double)x0) x0 = lo > (double)x1 ? x1 + 1 : (int)lo;
-    if (hi < (double)x1) x1 = hi < (double)x0 ? x0 - 1 : (int)hi;
-  }
-  if (ny > 0.0) {
-    float h2 = (float)((two_t + g * a1 * X * X) * C0) / (float)ny;
-    double hy = (double)gs_sqrtf(h2) * (1.0 + 1e-5) + 1e-6;
-    double lo = floor((double)cy - hy - 0.5), hi = ceil((double)cy + hy - 0.5);
-    if (lo > (double)y0) y0 = lo > (double)y1 ? y1 + 1 : (int)lo;
-    if (hi < (double)y1) y1 = hi < (double)y0 ? y0 - 1 : (int)hi;
+  double O = fmax(fabs(o0 + 0.5 - (double)oc), fabs(o1 + 0.5 - (double)oc)) + 1.0;
+  double n = f.det - g * (cm + f.c1a) * co;
+  if (n > 0.0) {
+    float h2 = (float)((f.two_t + g * (co + f.c1a) * O * O) * co) / (float)n;
+    double h = (double)gs_sqrtf(h2) * (1.0 + 1e-5) + 1e-6;
+    double lo = floor((double)pc - h - 0.5), hi = ceil((double)pc + h - 0.5);
+    if (lo > (double)p0) p0 = lo > (double)p1 ? p1 + 1 : (int)lo;
+    if (hi < (double)p1) p1 = hi < (double)p0 ? p0 - 1 : (int)hi;
   }
 }
 
@@ -1009,11 +1013,61 @@ GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, 
   return c;
 }
 
+// A record's candidate pixels in a block: [x0, x1] x [y0, y1], empty when
+// x0 > x1 or y0 > y1.
+struct Rect {
+  int x0, x1, y0, y1;
+};
+// The support rectangle (above) of the record with mean m and conic c
+// (c.w = ln(255 opacity)) within columns tx0..tx0+E-1 and rows ry0..ry1;
+// under the periodic boundary policies every pixel there is a candidate.
+template <int kEdge>
+GS_DEV Rect record_rect(float4 m, float4 c, int tx0, int ry0, int ry1, uint32_t axis) {
+  Rect r = {tx0, tx0 + kEdge - 1, ry0, ry1};
+  if (axis != 0) return r;
+  SupportForm f = support_form(c.x, c.y, c.z, c.w);
+  if (!f.bounded) return r;
+  support_axis(f, f.c2, f.c0, m.y, m.x, r.x0, r.x1, r.y0, r.y1);  // rows
+  if (r.y0 > r.y1) return r;
+  support_axis(f, f.c0, f.c2, m.x, m.y, r.y0, r.y1, r.x0, r.x1);  // columns
+  return r;
+}
+
+// Splat fields a raster round stages: mean, (conic.xyz, ln(255 opacity)),
+// colour.
+GS_DEV void gather_splat(const Splat* __restrict__ splat, uint32_t id, float4& m, float4& c,
+                         float4& k) {
+  const Splat& g = splat[id];
+  m = g.mean;
+  float4 cn = g.conic;
+  c = make_float4(cn.x, cn.y, cn.z, g.aux.x);
+  k = g.color;
+}
+
+// Set the candidate bits of the record in round slot `slot` for the pixels
+// of rc (absolute pixel bounds, inside the block's rows) that pass the
+// reference's test (TileRasterizer.ts:624-656); pixel p of the block is
+// (px - tx0) + (py - ry0) * E.
+template <int kEdge, class Lds>
+GS_DEV void mark_candidates(Lds& L, uint32_t slot, const Rect& rc, int tx0, int ry0, float4 m,
+                            float4 c, float vw, float vh, uint32_t axis) {
+  uint32_t bit = 1u << (slot & 31u), word = slot >> 5;
+  for (int py = rc.y0; py <= rc.y1; py++) {
+    float pyf = (float)py + 0.5f;
+    for (int px = rc.x0; px <= rc.x1; px++) {
+      float power = pair_power((float)px + 0.5f, pyf, m, c, vw, vh, axis);
+      if (!(power > 0.0f || power < -c.w))
+        atomicOr(&L.mask[word][(uint32_t)((py - ry0) * kEdge + (px - tx0))], bit);
+    }
+  }
+}
+
 // Blend records [cb, ce) of `tile` into one pixel per thread (threads
 // 0..E*E-1), from T = 1. All kThreads threads of the block must call it;
-// threads below kBatch stage one record per round. The instrumented
-// variant (kRecord) also raises record[id] to every alpha * T that
-// Gaussian id contributes (binary32 bits; non-negative, so unsigned order).
+// threads below kBatch stage one record per round (a record whose support
+// rectangle is empty sets no bits and is not staged). The instrumented variant
+// (kRecord) also raises record[id] to every alpha * T that Gaussian id
+// contributes (binary32 bits; non-negative, so unsigned order).
 template <int kEdge, int kThreads, int kBatch, bool kRecord>
 GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t cb, uint32_t ce,
                                  const uint2* __restrict__ sorted,
@@ -1042,53 +1096,32 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   uint32_t cg = 0u;
   if (stager && idx < ce) {
     cg = sorted[idx].y;
-    m = splat[cg].mean;
-    c = splat[cg].conic;
-    k = splat[cg].color;
+    gather_splat(splat, cg, m, c, k);
   }
   __syncthreads();
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
     for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
       (&L.mask[0][0])[i] = 0u;
     __syncthreads();
-    bool mine = stager && batch + t < ce;
-    float thr = 0.0f;
-    if (mine) {
-      thr = gs_logf(m.w * 255.0f);  // TileRasterizer.ts:579-581
+    bool hit = stager && batch + t < ce;
+    Rect crc = {0, -1, 0, -1};
+    if (hit) {
+      crc = record_rect<kEdge>(m, c, tx0, ty0, ty0 + kEdge - 1, axis);
+      hit = crc.x0 <= crc.x1 && crc.y0 <= crc.y1;
+    }
+    if (hit) {
       L.mean[t] = m;
-      L.conic[t] = make_float4(c.x, c.y, c.z, thr);
+      L.conic[t] = c;
       L.color[t] = k;
       if (kRecord) L.gid[t] = cg;
     }
-    float4 cm = m, cc = make_float4(c.x, c.y, c.z, thr);
+    float4 cm = m, cc = c;
     uint32_t nidx = batch + kBatch + t;
     if (stager && nidx < ce) {
       cg = sorted[nidx].y;
-      m = splat[cg].mean;
-      c = splat[cg].conic;
-      k = splat[cg].color;
+      gather_splat(splat, cg, m, c, k);
     }
-    // Candidate pixels of record t.
-    if (mine) {
-      int x0, x1, y0, y1;
-      if (axis == 0)
-        support_rect(cm.x, cm.y, cc.x, cc.y, cc.z, cc.w, tx0, ty0, kEdge, x0, x1, y0, y1);
-      else {
-        x0 = tx0;
-        x1 = tx0 + kEdge - 1;
-        y0 = ty0;
-        y1 = ty0 + kEdge - 1;
-      }
-      uint32_t bit = 1u << (t & 31u), word = t >> 5;
-      for (int py = y0; py <= y1; py++) {
-        float pyf = (float)py + 0.5f;
-        for (int px = x0; px <= x1; px++) {
-          float power = pair_power((float)px + 0.5f, pyf, cm, cc, vw, vh, axis);
-          if (!(power > 0.0f || power < -cc.w))
-            atomicOr(&L.mask[word][(uint32_t)((py - ty0) * kEdge + (px - tx0))], bit);
-        }
-      }
-    }
+    if (hit) mark_candidates<kEdge>(L, t, crc, tx0, ty0, cm, cc, vw, vh, axis);
     __syncthreads();
     // Ordered blend over this pixel's candidates: each lane walks its own
     // candidate bits across the round's mask words, four at a time (their
@@ -1193,6 +1226,8 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
   float threshold = fp.transmittance_threshold;
   PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
   bool done = !active;
+  // Per staged record: its ordinal two rounds ahead, its splat one round
+  // ahead.
   uint32_t cg[kPer], cg_next[kPer];
   float4 m[kPer], c[kPer], k[kPer];
 #pragma unroll
@@ -1202,9 +1237,7 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
     m[e] = c[e] = k[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (r < ce) {
       cg[e] = sorted[r].y;
-      m[e] = splat[cg[e]].mean;
-      c[e] = splat[cg[e]].conic;
-      k[e] = splat[cg[e]].color;
+      gather_splat(splat, cg[e], m[e], c[e], k[e]);
     }
     if (r + kBatch < ce) cg_next[e] = sorted[r + kBatch].y;
   }
@@ -1213,56 +1246,36 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
       (&L.mask[0][0])[i] = 0u;
     __syncthreads();
     float4 cm[kPer], cc[kPer];
-    bool mine[kPer];
+    Rect crc[kPer];
+    bool hit[kPer];
 #pragma unroll
     for (int e = 0; e < kPer; e++) {
       uint32_t slot = e * kThreads + t;
-      mine[e] = batch + slot < ce;
-      float thr = 0.0f;
-      if (mine[e]) {
-        thr = gs_logf(m[e].w * 255.0f);  // TileRasterizer.ts:579-581
+      hit[e] = batch + slot < ce;
+      crc[e] = {0, -1, 0, -1};
+      if (hit[e]) {
+        crc[e] = record_rect<kEdge>(m[e], c[e], tx0, sy0, sy0 + kSlabRows - 1, axis);
+        hit[e] = crc[e].x0 <= crc[e].x1 && crc[e].y0 <= crc[e].y1;
+      }
+      if (hit[e]) {
         L.mean[slot] = m[e];
-        L.conic[slot] = make_float4(c[e].x, c[e].y, c[e].z, thr);
+        L.conic[slot] = c[e];
         L.color[slot] = k[e];
         if (kRecord) L.gid[slot] = cg[e];
       }
       cm[e] = m[e];
-      cc[e] = make_float4(c[e].x, c[e].y, c[e].z, thr);
+      cc[e] = c[e];
       uint32_t r = batch + kBatch + slot;
       if (r < ce) {
         cg[e] = cg_next[e];
-        m[e] = splat[cg[e]].mean;
-        c[e] = splat[cg[e]].conic;
-        k[e] = splat[cg[e]].color;
+        gather_splat(splat, cg[e], m[e], c[e], k[e]);
       }
       if (r + kBatch < ce) cg_next[e] = sorted[r + kBatch].y;
     }
 #pragma unroll
-    for (int e = 0; e < kPer; e++) {
-      if (!mine[e]) continue;
-      int x0, x1, y0, y1;
-      if (axis == 0)
-        support_rect(cm[e].x, cm[e].y, cc[e].x, cc[e].y, cc[e].z, cc[e].w, tx0,
-                     (int)((tile / fp.tiles_x) * kEdge), kEdge, x0, x1, y0, y1);
-      else {
-        x0 = tx0;
-        x1 = tx0 + kEdge - 1;
-        y0 = sy0;
-        y1 = sy0 + kSlabRows - 1;
-      }
-      if (y0 < sy0) y0 = sy0;
-      if (y1 > sy0 + kSlabRows - 1) y1 = sy0 + kSlabRows - 1;
-      uint32_t slot = e * kThreads + t;
-      uint32_t bit = 1u << (slot & 31u), word = slot >> 5;
-      for (int py = y0; py <= y1; py++) {
-        float pyf = (float)py + 0.5f;
-        for (int px = x0; px <= x1; px++) {
-          float power = pair_power((float)px + 0.5f, pyf, cm[e], cc[e], vw, vh, axis);
-          if (!(power > 0.0f || power < -cc[e].w))
-            atomicOr(&L.mask[word][(uint32_t)((py - sy0) * kEdge + (px - tx0))], bit);
-        }
-      }
-    }
+    for (int e = 0; e < kPer; e++)
+      if (hit[e])
+        mark_candidates<kEdge>(L, e * kThreads + t, crc[e], tx0, sy0, cm[e], cc[e], vw, vh, axis);
     __syncthreads();
     if (!done) {  // uniform over the quad
       uint32_t sum = 0u;
@@ -1385,6 +1398,36 @@ GS_DEV void store_pixel(const FrameParams& fp, uint32_t x, uint32_t y, float r, 
 // tile stores the background directly.
 constexpr uint32_t kLightBlocks = 2048;
 
+// One light tile (at most kLightMax records) with one thread per pixel, or
+// the background for an empty tile.
+template <int kEdge, int kBatch, bool kRecord>
+GS_DEV void light_tile(const FrameParams& fp, DeviceState* __restrict__ state, uint32_t tile,
+                       uint32_t begin, uint32_t end, const uint2* __restrict__ sorted,
+                       const Splat* __restrict__ splat, RasterLds<kEdge * kEdge, kBatch>& L,
+                       uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
+                       uint32_t* __restrict__ record) {
+  constexpr uint32_t kPixels = kEdge * kEdge;
+  if (begin == end) {
+    uint32_t t = threadIdx.x;
+    uint32_t x = (tile % fp.tiles_x) * kEdge + (t & (uint32_t)(kEdge - 1));
+    uint32_t y = (tile / fp.tiles_x) * kEdge + ((t >> log2_edge(kEdge)) & (uint32_t)(kEdge - 1));
+    if (t < kPixels && x < fp.width && y < fp.height)
+      store_pixel(fp, x, y, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, color_out, depth_out);
+    return;
+  }
+  uint32_t x, y;
+  bool active;
+  PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
+      fp, tile, begin, end, sorted, splat, L, x, y, active, record);
+  if (fp.raster_stats) add_stats(state, s.checked, s.blended, false);
+  if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+}
+
+// Light tiles, beside the whole-tile heavy kernel: the grid walks the tiles
+// in strides. In frames whose heavy units run as row slabs (edge 16) the
+// slab kernel takes the non-empty light tiles too, and this kernel only
+// stores the background of the empty ones: once the slab kernel's
+// persistent grid has filled the GPU, no other block would find room.
 template <int kEdge, int kBatch, bool kRecord>
 __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
@@ -1393,24 +1436,13 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     uint32_t* __restrict__ record) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
   if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
-  constexpr uint32_t kPixels = kEdge * kEdge;
+  const bool empty_only = kEdge == 16 && state->heavy_split != 0u;
   for (uint32_t tile = blockIdx.x; tile < fp.tile_count; tile += gridDim.x) {
     uint32_t begin = off[tile], end = off[tile + 1];
     if (end - begin > kLightMax) continue;  // heavy unit (block-uniform)
-    if (begin == end) {
-      uint32_t t = threadIdx.x;
-      uint32_t x = (tile % fp.tiles_x) * kEdge + (t & (uint32_t)(kEdge - 1));
-      uint32_t y = (tile / fp.tiles_x) * kEdge + ((t >> log2_edge(kEdge)) & (uint32_t)(kEdge - 1));
-      if (t < kPixels && x < fp.width && y < fp.height)
-        store_pixel(fp, x, y, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, color_out, depth_out);
-      continue;
-    }
-    uint32_t x, y;
-    bool active;
-    PixelSum s = blend_tile_range<kEdge, light_threads(kEdge, kBatch), kBatch, kRecord>(
-        fp, tile, begin, end, sorted, splat, lds, x, y, active, record);
-    if (fp.raster_stats) add_stats(state, s.checked, s.blended, false);
-    if (active) store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+    if (empty_only && end != begin) continue;
+    light_tile<kEdge, kBatch, kRecord>(fp, state, tile, begin, end, sorted, splat, lds,
+                                       color_out, depth_out, record);
   }
 }
 
@@ -1431,9 +1463,9 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
   // chunks are contiguous and start at chunk_first[tile]; the order between
   // tiles is free). Heavy tiles then take slots after all chunk tasks,
   // heaviest size class first.
-  __shared__ uint32_t chunk_total;
+  __shared__ uint32_t chunk_total, light_total;
   __shared__ uint32_t class_count[5], class_next[5];
-  if (threadIdx.x == 0) chunk_total = 0u;
+  if (threadIdx.x == 0) chunk_total = light_total = 0u;
   if (threadIdx.x < 5) class_count[threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t tiles = state->overflow ? 0u : fp.tile_count;
@@ -1473,81 +1505,127 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
     }
     state->heavy_units = min(next, unit_capacity);
     state->heavy_split = next > fp.split_units ? 1u : 0u;
+    for (int x = 0; x < 8; x++) state->heavy_next[x] = 0u;
   }
   __syncthreads();
+  // Row-slab frames (tile edge 16) also list the non-empty light tiles that
+  // are not chunked, after the heavy units, for the slab kernel.
+  const bool list_light = fp.tile_size == 16 && state->heavy_split != 0u;
+  const uint32_t heavy_total = state->heavy_units;
 #pragma unroll 4
   for (uint32_t t = threadIdx.x; t < tiles; t += 1024u) {
     uint32_t count = off[t + 1] - off[t];
-    if ((cs == 0 || count <= cs) && count > kLightMax) {
+    bool chunked = cs != 0 && count > cs;
+    if (!chunked && count > kLightMax) {
       uint32_t c = min(31u - (uint32_t)__builtin_clz((count - 1u) >> 10), 4u);
       uint32_t slot = atomicAdd(&class_next[c], 1u);
       if (slot < unit_capacity) units[slot] = make_uint2(t, ~0u);
+    } else if (list_light && !chunked && count != 0u) {
+      uint32_t slot = heavy_total + atomicAdd(&light_total, 1u);
+      if (slot < unit_capacity) units[slot] = make_uint2(t, ~0u);
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = list_light ? light_total : 0u;
+    state->light_units = min(n, unit_capacity - heavy_total);
   }
 }
 
 // Heavy units: per chunk task or heavy tile, S = heavy_slabs(E, kSplit)
-// blocks, one per row slab; the kSplit and whole-tile launches both run and
-// the one the frame did not choose (DeviceState::heavy_split, set by
-// k_heavy_units) exits at once. Block b takes unit u = (j / S) * 8 + b % 8
-// and slab j % S, with j = b / 8: workgroups go to the eight XCDs in turn,
-// so a unit's slabs run on one XCD and share its L2, and units start in list
-// order (heaviest first). The grid is the unit capacity rounded up to 8,
-// times S.
+// work items, one per row slab; the kSplit and whole-tile launches both
+// run and the one the frame did not choose (DeviceState::heavy_split, set
+// by k_heavy_units) exits at once. The grid is persistent (as many blocks
+// as fit on the GPU at once, a multiple of 8). Workgroups go to the eight
+// XCDs in turn, so block b works for XCD x = b % 8: its items are
+// k = 0, 1, 2, ... of that XCD's sequence, where item k is unit
+// (k / S) * 8 + x, slab k % S. A unit's slabs therefore run on one XCD and
+// share its L2, and units start in list order (heaviest first). The row-slab
+// kernel then takes the light tiles listed after the heavy units, as slab
+// items of the same kind. A block starts with item b / 8 and then takes the
+// XCD's next unclaimed item (one atomic per item), so blocks that finish
+// early keep working.
 template <int kEdge, bool kSplit, bool kRecord>
 __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
     const uint32_t* __restrict__ off, const Splat* __restrict__ splat,
-    const uint2* __restrict__ units, float4* __restrict__ partials,
-    float* __restrict__ partial_depth, uint16_t* __restrict__ color_out,
-    float* __restrict__ depth_out, uint32_t* __restrict__ record) {
+    const uint2* __restrict__ units,
+    float4* __restrict__ partials, float* __restrict__ partial_depth,
+    uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
+    uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr bool kQuad = kEdge != 32;
   constexpr uint32_t kSlabs = heavy_slabs(kEdge, kSplit);
   constexpr int kBatch = heavy_batch(kEdge, kSplit);
   __shared__ RasterLds<heavy_slab_pixels(kEdge, kSplit), kBatch> lds;
+  __shared__ uint32_t next_item;
   if (kEdge == 16 && (state->heavy_split != 0u) != kSplit) return;
-  const uint32_t j = blockIdx.x >> 3;
-  const uint32_t u = (j / kSlabs) * 8u + (blockIdx.x & 7u), slab = j % kSlabs;
-  if (u >= state->heavy_units) return;
-  uint2 unit = units[u];
-  uint32_t tile = unit.x;
-  uint32_t begin = off[tile], end = off[tile + 1];
-  uint32_t cb = begin, ce = end;
-  bool chunk = unit.y != ~0u;
-  if (chunk) {
-    cb = begin + unit.y * fp.chunk_size;
-    ce = min(cb + fp.chunk_size, end);
-  }
-  uint32_t x, y;
-  bool active;
-  PixelSum s;
-  uint32_t pix = threadIdx.x;  // pixel within the tile
-  bool writer = threadIdx.x < (uint32_t)kPixels;
-  if constexpr (kQuad) {
-    s = blend_heavy_quad<kEdge, heavy_slab_rows(kEdge, kSplit), kBatch, kRecord>(
-        fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
-    pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> 2);
-    writer = (threadIdx.x & 3u) == 0u;
-  } else {
-    s = blend_tile_range<kEdge, kHeavyThreads, kBatch, kRecord>(
-        fp, tile, cb, ce, sorted, splat, lds, x, y, active, record);
-  }
-  if (fp.raster_stats) add_stats(state, writer ? s.checked : 0u, writer ? s.blended : 0u, true);
-  if (!writer) return;
-  if (chunk) {
-    size_t p = (size_t)u * kPixels + pix;  // chunk tasks lead the list
-    partials[p] = make_float4(s.r, s.g, s.b, s.t);
-    partial_depth[p] = s.d;
-  } else if (active) {
-    store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+  const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
+  const uint32_t n_heavy = state->heavy_units;
+  const uint32_t n_light = kSplit ? state->light_units : 0u;
+  const uint32_t heavy_items = ((n_heavy + 7u) >> 3) * kSlabs;  // per XCD
+  const uint32_t items = heavy_items + ((n_light + 7u) >> 3) * kSlabs;
+  for (uint32_t item = blockIdx.x >> 3; item < items;) {
+    {
+      const bool light = item >= heavy_items;
+      const uint32_t k = light ? item - heavy_items : item;
+      const uint32_t slab = k % kSlabs;
+      const uint32_t u = (light ? n_heavy : 0u) + (k / kSlabs) * 8u + xcd;
+      if (u < n_heavy + (light ? n_light : 0u)) {
+        uint2 unit = units[u];
+        uint32_t tile = unit.x;
+        uint32_t begin = off[tile], end = off[tile + 1];
+        uint32_t cb = begin, ce = end;
+        bool chunk = unit.y != ~0u;
+        if (chunk) {
+          cb = begin + unit.y * fp.chunk_size;
+          ce = min(cb + fp.chunk_size, end);
+        }
+        uint32_t x, y;
+        bool active;
+        PixelSum s;
+        uint32_t pix = threadIdx.x;  // pixel within the tile
+        bool writer = threadIdx.x < (uint32_t)kPixels;
+        if constexpr (kQuad) {
+          s = blend_heavy_quad<kEdge, heavy_slab_rows(kEdge, kSplit), kBatch, kRecord>(
+              fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
+          pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> 2);
+          writer = (threadIdx.x & 3u) == 0u;
+        } else {
+          s = blend_tile_range<kEdge, kHeavyThreads, kBatch, kRecord>(
+              fp, tile, cb, ce, sorted, splat, lds, x, y, active, record);
+        }
+        if (fp.raster_stats)
+          add_stats(state, writer ? s.checked : 0u, writer ? s.blended : 0u, !light);
+        if (writer) {
+          if (chunk) {
+            size_t p = (size_t)u * kPixels + pix;  // chunk tasks lead the list
+            partials[p] = make_float4(s.r, s.g, s.b, s.t);
+            partial_depth[p] = s.d;
+          } else if (active) {
+            store_pixel(fp, x, y, s.r, s.g, s.b, s.d, s.t, color_out, depth_out);
+          }
+        }
+      }
+    }
+    __syncthreads();  // the LDS and next_item are free again
+    if (threadIdx.x == 0) next_item = per_xcd + atomicAdd(&state->heavy_next[xcd], 1u);
+    __syncthreads();
+    item = next_item;
   }
 }
 
-// Blocks of k_raster_heavy for `units` heavy units.
+// Persistent grid of k_raster_heavy, a multiple of 8 blocks (at least 8):
+// whole-tile blocks (16 waves) one per CU, leaving room for the light-tile
+// kernel beside them; row-slab blocks (4 waves) five per CU, the waves per
+// CU their registers allow (measured faster than leaving the light kernel
+// room: the slab units outnumber the CUs and gain most from occupancy).
 template <int kEdge, bool kSplit>
-inline uint32_t heavy_grid(uint32_t units) {
-  return ((units + 7u) & ~7u) * (uint32_t)heavy_slabs(kEdge, kSplit);
+inline uint32_t heavy_grid(uint32_t cu_count) {
+  uint32_t waves = (uint32_t)heavy_threads(kEdge, kSplit) / 64u;
+  uint32_t per_cu = waves >= 16u ? 1u : 20u / waves;
+  uint32_t g = (cu_count * per_cu) & ~7u;
+  return g < 8u ? 8u : g;
 }
 
 // TileRasterizer.ts:807-902: ordered over-composite of a tile's chunks.
@@ -1625,32 +1703,33 @@ void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_coun
 // concurrently with light tiles on s, then the chunk composite.
 template <int kEdge, int kBatch, bool kRecord>
 hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
-                         hipStream_t aux, hipEvent_t fork, hipEvent_t join, hipEvent_t* marks) {
+                         const RasterStreams& rs, hipEvent_t* marks) {
   const FrameParams& fp = b.params;
   DeviceState* st = b.state;
   k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
                                    b.unit_capacity);
-  hipError_t e = hipEventRecord(fork, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+  hipError_t e = hipEventRecord(rs.fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
   if (e != hipSuccess) return e;
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
-  k_raster_heavy<kEdge, false, kRecord><<<heavy_grid<kEdge, false>(b.unit_capacity),
-                                           heavy_threads(kEdge, false), 0, aux>>>(
+  // The whole-tile kernel, then (edge 16) the row-slab kernel: one of them
+  // exits at once, and does so before the other can fill the GPU.
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], rs.aux);
+  k_raster_heavy<kEdge, false, kRecord><<<heavy_grid<kEdge, false>(b.cu_count),
+                                           heavy_threads(kEdge, false), 0, rs.aux>>>(
       fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials, b.partial_depth,
       b.color_out, b.depth_out, b.record);
   if constexpr (kEdge == 16)
-    k_raster_heavy<kEdge, true, kRecord><<<heavy_grid<kEdge, true>(b.unit_capacity),
-                                           heavy_threads(kEdge, true), 0, aux>>>(
+    k_raster_heavy<kEdge, true, kRecord><<<heavy_grid<kEdge, true>(b.cu_count),
+                                           heavy_threads(kEdge, true), 0, rs.aux>>>(
         fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
         b.partial_depth, b.color_out, b.depth_out, b.record);
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], aux);
-  e = hipEventRecord(join, aux);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], rs.aux);
+  e = hipEventRecord(rs.join, rs.aux);
   if (e != hipSuccess) return e;
   uint32_t light_blocks = b.max_tiles < kLightBlocks ? b.max_tiles : kLightBlocks;
   k_raster_light<kEdge, kBatch, kRecord><<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(
-      fp, st, sorted, b.tile_offsets, b.proj.splat, b.color_out,
-      b.depth_out, b.record);
-  e = hipStreamWaitEvent(s, join, 0);
+      fp, st, sorted, b.tile_offsets, b.proj.splat, b.color_out, b.depth_out, b.record);
+  e = hipStreamWaitEvent(s, rs.join, 0);
   if (e != hipSuccess) return e;
   if (fp.chunk_size != 0)
     k_composite<kEdge><<<b.max_tiles, kEdge * kEdge, 0, s>>>(
@@ -1662,19 +1741,17 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
 // The instrumented raster (contribution record) uses the 256-record batch.
 template <int kEdge>
 hipError_t launch_raster_edge(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
-                              hipStream_t aux, hipEvent_t fork, hipEvent_t join,
-                              hipEvent_t* marks) {
-  if (b.record) return launch_raster<kEdge, 256, true>(b, sorted, s, aux, fork, join, marks);
-  if (b.raster_batch == 128)
-    return launch_raster<kEdge, 128, false>(b, sorted, s, aux, fork, join, marks);
-  return launch_raster<kEdge, 256, false>(b, sorted, s, aux, fork, join, marks);
+                              const RasterStreams& rs, hipEvent_t* marks) {
+  if (b.record) return launch_raster<kEdge, 256, true>(b, sorted, s, rs, marks);
+  if (b.raster_batch == 128) return launch_raster<kEdge, 128, false>(b, sorted, s, rs, marks);
+  return launch_raster<kEdge, 256, false>(b, sorted, s, rs, marks);
 }
 
 }  // namespace
 
 
-hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, hipEvent_t fork,
-                        hipEvent_t join, hipEvent_t* marks) {
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStreams& rs,
+                        hipEvent_t* marks) {
   auto mark = [&](int stage) {
     if (marks) (void)hipEventRecord(marks[stage], s);
   };
@@ -1713,7 +1790,7 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   mark(kStageScan);
   LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
   k_scan<LoadOrdered, StoreOrdered><<<scan_blocks, kBlock, 0, s>>>(
-      LoadOrdered{sorted_depth, b.proj.splat, b.ordered_hits},
+      LoadOrdered{sorted_depth, b.proj.hits, b.ordered_hits},
       StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, sws, kScanIntersections,
       fp.capacity, fp.camera, st);
 
@@ -1732,11 +1809,11 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, hipStream_t aux, h
   mark(kStageRaster);
   hipError_t e;
   if (fp.tile_size == 8)
-    e = launch_raster_edge<8>(b, sorted, s, aux, fork, join, marks);
+    e = launch_raster_edge<8>(b, sorted, s, rs, marks);
   else if (fp.tile_size == 32)
-    e = launch_raster_edge<32>(b, sorted, s, aux, fork, join, marks);
+    e = launch_raster_edge<32>(b, sorted, s, rs, marks);
   else
-    e = launch_raster_edge<16>(b, sorted, s, aux, fork, join, marks);
+    e = launch_raster_edge<16>(b, sorted, s, rs, marks);
   if (e != hipSuccess) return e;
   mark(kStageCount);
   return hipGetLastError();
