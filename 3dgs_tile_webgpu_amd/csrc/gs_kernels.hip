@@ -170,10 +170,13 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   }
   float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
   float ts = (float)fp.tile_size;
-  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx / ts), 0.0f, max_tx));
-  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy / ts), 0.0f, max_ty));
-  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx / ts), 0.0f, max_tx));
-  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy / ts), 0.0f, max_ty));
+  // x / ts as x * (1 / ts): the tile edge is a power of two, so both are the
+  // exact quotient rounded once, the same binary32 value.
+  float inv_ts = 1.0f / ts;
+  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx * inv_ts), 0.0f, max_tx));
+  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy * inv_ts), 0.0f, max_ty));
+  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx * inv_ts), 0.0f, max_tx));
+  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy * inv_ts), 0.0f, max_ty));
   if (axis == 1) {
     tmin_x = 0;
     tmax_x = (int)fp.tiles_x - 1;
@@ -231,7 +234,9 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
     col[1] = col[1] + w_ * k[1];                    \
     col[2] = col[2] + w_ * k[2];                    \
     if (kMode != kModePlain)                        \
-      sh_energy += k[0] * k[0] + k[1] * k[1] + k[2] * k[2]; \
+      sh_energy = __builtin_fmaf(                   \
+          k[2], k[2],                               \
+          __builtin_fmaf(k[1], k[1], __builtin_fmaf(k[0], k[0], sh_energy))); \
   }
   if (degree >= 1) {
     GS_SH_ADD(-0.4886025119029199f * y, 1);
@@ -565,6 +570,11 @@ constexpr int kLookbackWindow = 32;  // partitions read per look-back step
 constexpr uint32_t kHistBlocks = 256;  // blocks of the digit-total kernel
 constexpr int kHistItems = 8;          // keys per lane in flight (digit totals)
 constexpr int kSortWaves = kSortThreads / 64;
+// k_sort_pass gives each of the up to kMaxRadix digits its own thread (digit
+// totals, offsets, look-back, status rows) and its block scan takes four
+// waves; fewer threads would leave digits without an output offset.
+static_assert(kSortThreads >= kMaxRadix && kSortThreads % 64 == 0,
+              "a radix-pass block needs a thread per digit");
 
 // Exclusive scan of one value per digit thread (t < 256) across a block of
 // kSortThreads threads; every thread of the block must call it.
@@ -810,11 +820,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
     float conic[3] = {conic4.x, conic4.y, conic4.z};
     float thr = gs_logf(mean.w * 255.0f);
     int max_tx = (int)fp.tiles_x - 1, max_ty = (int)fp.tiles_y - 1;
-    float ts = (float)fp.tile_size;
-    int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) / ts)), 0), max_tx);
-    int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) / ts)), 0), max_ty);
-    int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) / ts)), 0), max_tx);
-    int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) / ts)), 0), max_ty);
+    float ts = (float)fp.tile_size, inv_ts = 1.0f / ts;  // a power of two: exact
+    int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) * inv_ts)), 0), max_tx);
+    int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) * inv_ts)), 0), max_ty);
+    int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) * inv_ts)), 0), max_tx);
+    int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) * inv_ts)), 0), max_ty);
     uint32_t axis = fp.boundary_axis;
     if (axis == 1) {
       tmin_x = 0;

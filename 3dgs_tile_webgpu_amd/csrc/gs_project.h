@@ -52,15 +52,15 @@ GS_DEV float dot3(const float a[3], const float b[3]) {
 
 // kernels/compactGaussians.ts:10-19
 GS_DEV void decode_rotation(uint32_t packed, float q[4]) {
-  float px = (float)(packed & 1023u) / 1023.0f * 2.0f - 1.0f;
-  float py = (float)((packed >> 10) & 1023u) / 1023.0f * 2.0f - 1.0f;
+  float px = gs_div_small((float)(packed & 1023u), 1023.0f, 1.0f / 1023.0f) * 2.0f - 1.0f;
+  float py = gs_div_small((float)((packed >> 10) & 1023u), 1023.0f, 1.0f / 1023.0f) * 2.0f - 1.0f;
   float z = 1.0f - gs_fabsf(px) - gs_fabsf(py);
   float t = gs_maxf(-z, 0.0f);
   float x = px >= 0.0f ? px + -t : px + t;
   float y = py >= 0.0f ? py + -t : py + t;
   float len = gs_sqrtf((x * x + y * y) + z * z);
   float ax = x / len, ay = y / len, az = z / len;
-  float half_angle = (float)(packed >> 20) / 4095.0f * 1.5707963267948966f;
+  float half_angle = gs_div_small((float)(packed >> 20), 4095.0f, 1.0f / 4095.0f) * 1.5707963267948966f;
   float s = gs_sinf(half_angle);
   q[0] = ax * s;
   q[1] = ay * s;
@@ -74,7 +74,7 @@ GS_DEV void decode_sh(uint32_t packed, float out[3]) {
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     int8_t b = (int8_t)((packed >> (8 * i)) & 0xffu);
-    out[i] = gs_maxf((float)b / 127.0f, -1.0f) * scale;
+    out[i] = gs_maxf(gs_div_small((float)b, 127.0f, 1.0f / 127.0f), -1.0f) * scale;
   }
 }
 

@@ -118,7 +118,8 @@ def batched(gs, reps=10):
 
 def learned_cull(gs, train_views=512, eval_views=64, reps=3):
     """Train the cull on orbit views of the 3M-Gaussian scene, calibrate its
-    output bias on 16 training views (mean PSNR >= 47.5 dB against plain),
+    output bias on 16 training views (mean PSNR >= 46.5 dB against plain,
+    1.5 dB above the 45 dB bar for the held-out views),
     then time held-out views batched with and without it."""
     w = gs.WORKLOADS["3m_cull"]
     scene = gs.workload_scene("3m_cull")
@@ -143,7 +144,7 @@ def learned_cull(gs, train_views=512, eval_views=64, reps=3):
     train_s = time.perf_counter() - t0
     progress("learned cull: calibrating and evaluating")
     shift, rows = cull.calibrate([c for c, _ in train[:16]], [s for _, s in train[:16]],
-                                 target_psnr=47.5)
+                                 target_psnr=46.5)
     cams, sizes = [c for c, _ in held], [s for _, s in held]
 
     def timed():

@@ -262,7 +262,7 @@ V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir,
     color.x = color.x + w * k.x;
     color.y = color.y + w * k.y;
     color.z = color.z + w * k.z;
-    e = e + (k.x * k.x + k.y * k.y + k.z * k.z);
+    e = __builtin_fmaf(k.z, k.z, __builtin_fmaf(k.y, k.y, __builtin_fmaf(k.x, k.x, e)));
   };
   if (degree >= 1) {
     add(-0.4886025119029199f * y, 1);
@@ -935,6 +935,23 @@ extern "C" size_t gs_oracle_get(const gs_oracle_result* r, const char* name, voi
 }
 
 extern "C" void gs_oracle_free(gs_oracle_result* r) { delete r; }
+
+// Numerators b (of the packed formats' ranges) for which the kernels'
+// gs_div_small(b, d, 1/d) differs from the IEEE quotient b / d, summed over
+// d = 127 (b in [-128, 127]), 1023 ([0, 1023]) and 4095 ([0, 4095]).
+extern "C" uint32_t gs_oracle_div_small_mismatches() {
+  struct Range {
+    float d;
+    int lo, hi;
+  } ranges[] = {{127.0f, -128, 127}, {1023.0f, 0, 1023}, {4095.0f, 0, 4095}};
+  uint32_t bad = 0;
+  for (const Range& r : ranges) {
+    volatile float d = r.d;  // the quotient below is evaluated at run time
+    for (int b = r.lo; b <= r.hi; b++)
+      if (gs_f2b(gs_div_small((float)b, r.d, 1.0f / r.d)) != gs_f2b((float)b / d)) bad++;
+  }
+  return bad;
+}
 
 // FP64 blend: the same per-tile lists and projected f32 values, blended in
 // double precision with one uninterrupted front-to-back stream per pixel.

@@ -93,3 +93,37 @@ def test_half_conversions_exhaustive(mathfn):
     with np.errstate(over="ignore"):
         expect = x.astype(np.float16).view(np.uint16)
     np.testing.assert_array_equal(got.astype(np.uint16), expect)
+
+
+def test_small_integer_division_matches_ieee(oracle):
+    """gs_div_small (the kernels' b / d for the packed formats' integer
+    numerators: rgb8e8 bytes / 127, oct101012 fields / 1023 and / 4095) is
+    the IEEE quotient for every numerator in range. The oracle divides
+    directly, so this is what keeps the two bit-identical."""
+    h = oracle.lib()
+    h.gs_oracle_div_small_mismatches.restype = C.c_uint32
+    assert h.gs_oracle_div_small_mismatches() == 0
+
+
+def test_small_integer_division_rational_proof():
+    """The same identity from exact rationals, independent of any float
+    library: q1 = RN(b/d'), r = RN(b - q1 d), q = RN(r/d' + q1) with
+    d' = RN(1/d) equals RN(b/d)."""
+    from fractions import Fraction as F
+
+    def rn32(x):
+        c = np.float32(float(x))
+        best = None
+        for v in (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf))):
+            d = abs(F(float(v)) - x)
+            even = (int(np.float32(v).view(np.uint32)) & 1) == 0
+            if best is None or d < best[0] or (d == best[0] and even):
+                best = (d, v)
+        return F(float(best[1]))
+
+    for d, lo, hi in ((127, -128, 127), (1023, 0, 1023), (4095, 0, 4095)):
+        inv = rn32(F(1, d))
+        for b in range(lo, hi + 1, 7 if d == 4095 else 1):
+            q1 = rn32(F(b) * inv)
+            r = rn32(F(b) - q1 * d)
+            assert rn32(r * inv + q1) == rn32(F(b, d)), (d, b)
