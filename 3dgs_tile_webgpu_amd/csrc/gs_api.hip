@@ -737,6 +737,8 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   p.camera = (const FrameCamera*)dyn;
   p.viewport[0] = (float)f->width;
   p.viewport[1] = (float)f->height;
+  p.inv_viewport[0] = 1.0f / p.viewport[0];
+  p.inv_viewport[1] = 1.0f / p.viewport[1];
   p.filter_variance = f->filter_variance;
   p.transmittance_threshold = ctx->cfg.transmittance_threshold;
   memcpy(p.background, ctx->cfg.background, sizeof(p.background));
@@ -760,11 +762,13 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   p.output_depth = ctx->cfg.output_depth ? 1u : 0u;
   p.raster_stats = ctx->cfg.raster_stats ? 1u : 0u;
   p.tile_size = ts;
-  // Heavy units run as row slabs above 2.5 units per CU: below that, a
-  // whole-tile block per unit already covers the CUs and slabs would only
-  // repeat the per-record staging (measured crossover on this GPU: 574
-  // units faster whole, 692 faster split).
-  p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 5 / 2
+  // Heavy units run as row slabs above 1.75 units per CU: below that, a
+  // whole-tile block per unit already covers the CUs and slabs mostly
+  // repeat the per-record staging. Measured on this GPU: the 1M/1080p frame
+  // (275 units) is 30% faster whole, mixed-resolution views of 567-574
+  // units 2-5% faster whole, dense culled 1080p views of 510-850 units
+  // 9-20% faster as slabs.
+  p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 7 / 4
                                                                : ctx->cfg.heavy_split_units;
   // Training-set frames: chunking off, so every recorded alpha * T uses the
   // true transmittance in front of it.

@@ -61,27 +61,28 @@ GS_HD inline float gs_bf2f(uint16_t h) { return gs_b2f((uint32_t)h << 16); }
 // viewing ray (object-space position . view direction, a proxy for how
 // deep in the scene the Gaussian sits), log view depth, the object-space
 // view direction, the object-space position, log SH energy (sum of squared
-// higher-band coefficients) and the normalised screen position.
+// higher-band coefficients) and the normalised screen position (inv_vw,
+// inv_vh: 1 / viewport size). Every log argument is positive and normal.
 GS_HD inline void cull_features(float f[kCullIn], float rx, float ry, const float conic[3],
                                 float opacity, float depth, const float dir[3],
                                 const float pos[3], float sh_energy, float cx, float cy,
-                                float vw, float vh) {
-  f[0] = gs_logf(rx > ry ? rx : ry);
-  f[1] = gs_logf(conic[0]);
+                                float inv_vw, float inv_vh) {
+  f[0] = gs_logf_pos(rx > ry ? rx : ry);
+  f[1] = gs_logf_pos(conic[0]);
   f[2] = conic[1] / gs_sqrtf(conic[0] * conic[2]);
-  f[3] = gs_logf(conic[2]);
+  f[3] = gs_logf_pos(conic[2]);
   f[4] = opacity;
   f[5] = pos[0] * dir[0] + pos[1] * dir[1] + pos[2] * dir[2];
-  f[6] = gs_logf(depth);
+  f[6] = gs_logf_pos(depth);
   f[7] = dir[0];
   f[8] = dir[1];
   f[9] = dir[2];
   f[10] = pos[0];
   f[11] = pos[1];
   f[12] = pos[2];
-  f[13] = gs_logf(sh_energy + 1e-6f);
-  f[14] = cx / vw;
-  f[15] = cy / vh;
+  f[13] = gs_logf_pos(sh_energy + 1e-6f);
+  f[14] = cx * inv_vw;
+  f[15] = cy * inv_vh;
 }
 
 #ifdef __HIPCC__

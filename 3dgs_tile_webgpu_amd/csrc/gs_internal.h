@@ -39,6 +39,7 @@ constexpr size_t kCameraBytes = 256;  // FrameCamera slot, padded
 struct FrameParams {
   const FrameCamera* camera;
   float viewport[2];  // width, height
+  float inv_viewport[2];  // 1 / width, 1 / height (IEEE quotients, host-computed)
   float filter_variance;
   float transmittance_threshold;
   float background[4];

@@ -273,7 +273,7 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   o.count = count;
   if (kMode != kModePlain)
     cull_features(feat, rx, ry, p.conic, opacity, p.depth, dir, local, sh_energy, p.cx, p.cy,
-                  vw, vh);
+                  fp.inv_viewport[0], fp.inv_viewport[1]);
   return o;
 }
 
@@ -1435,9 +1435,9 @@ GS_DEV void light_tile(const FrameParams& fp, DeviceState* __restrict__ state, u
 
 // Light tiles, beside the whole-tile heavy kernel: the grid walks the tiles
 // in strides. In frames whose heavy units run as row slabs (edge 16) the
-// slab kernel takes the non-empty light tiles too, and this kernel only
-// stores the background of the empty ones: once the slab kernel's
-// persistent grid has filled the GPU, no other block would find room.
+// slab kernel takes the light tiles and the empty ones too, and this kernel
+// exits at once (while the whole-tile kernel it runs beside does the same),
+// before the slab kernel's persistent grid takes every CU.
 template <int kEdge, int kBatch, bool kRecord>
 __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
@@ -1446,11 +1446,10 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     uint32_t* __restrict__ record) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
   if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
-  const bool empty_only = kEdge == 16 && state->heavy_split != 0u;
+  if (kEdge == 16 && state->heavy_split != 0u) return;
   for (uint32_t tile = blockIdx.x; tile < fp.tile_count; tile += gridDim.x) {
     uint32_t begin = off[tile], end = off[tile + 1];
     if (end - begin > kLightMax) continue;  // heavy unit (block-uniform)
-    if (empty_only && end != begin) continue;
     light_tile<kEdge, kBatch, kRecord>(fp, state, tile, begin, end, sorted, splat, lds,
                                        color_out, depth_out, record);
   }
@@ -1552,9 +1551,10 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
 // (k / S) * 8 + x, slab k % S. A unit's slabs therefore run on one XCD and
 // share its L2, and units start in list order (heaviest first). The row-slab
 // kernel then takes the light tiles listed after the heavy units, as slab
-// items of the same kind. A block starts with item b / 8 and then takes the
-// XCD's next unclaimed item (one atomic per item), so blocks that finish
-// early keep working.
+// items of the same kind, and last the background of the empty tiles, 64
+// tiles per item. A block starts with item b / 8 and then takes the XCD's
+// next unclaimed item (one atomic per item), so blocks that finish early
+// keep working.
 template <int kEdge, bool kSplit, bool kRecord>
 __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
@@ -1574,9 +1574,27 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
   const uint32_t n_heavy = state->heavy_units;
   const uint32_t n_light = kSplit ? state->light_units : 0u;
   const uint32_t heavy_items = ((n_heavy + 7u) >> 3) * kSlabs;  // per XCD
-  const uint32_t items = heavy_items + ((n_light + 7u) >> 3) * kSlabs;
+  const uint32_t unit_items = heavy_items + ((n_light + 7u) >> 3) * kSlabs;
+  constexpr uint32_t kBgTiles = 64;  // empty tiles per background item
+  const uint32_t tiles = (kSplit && !state->overflow) ? fp.tile_count : 0u;
+  const uint32_t bg_groups = (tiles + kBgTiles - 1u) / kBgTiles;
+  const uint32_t items = unit_items + ((bg_groups + 7u) >> 3);
   for (uint32_t item = blockIdx.x >> 3; item < items;) {
-    {
+    if (item >= unit_items) {
+      // Background of the empty tiles of group (item - unit_items) * 8 + x.
+      const uint32_t g = (item - unit_items) * 8u + xcd;
+      for (uint32_t i = 0; i < kBgTiles && g < bg_groups; i++) {
+        uint32_t tile = g * kBgTiles + i;
+        if (tile >= tiles) break;
+        if (off[tile + 1] != off[tile]) continue;  // block-uniform
+        for (uint32_t px = threadIdx.x; px < (uint32_t)kPixels; px += blockDim.x) {
+          uint32_t x = (tile % fp.tiles_x) * kEdge + (px & (uint32_t)(kEdge - 1));
+          uint32_t y = (tile / fp.tiles_x) * kEdge + (px >> log2_edge(kEdge));
+          if (x < fp.width && y < fp.height)
+            store_pixel(fp, x, y, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, color_out, depth_out);
+        }
+      }
+    } else {
       const bool light = item >= heavy_items;
       const uint32_t k = light ? item - heavy_items : item;
       const uint32_t slab = k % kSlabs;

@@ -153,6 +153,36 @@ GS_INLINE float gs_logf(float x) {
   return z;
 }
 
+// Natural log of a positive normal binary32, without the special values:
+// the Cephes logf reduction and polynomial above, evaluated with fused
+// multiply-adds (correctly rounded on both sides, so host and device agree
+// bit for bit) and no branches. Used by the learned-cull features only,
+// whose arguments are positive and normal by construction; the reference's
+// own arithmetic keeps gs_logf.
+GS_INLINE float gs_logf_pos(float x) {
+  uint32_t ix = gs_f2b(x);
+  int e = (int)((ix >> 23) & 0xffu) - 126;
+  float m = gs_b2f((ix & 0x007fffffu) | 0x3f000000u);  // in [0.5, 1)
+  bool low = m < 0.70710678118654752440f;
+  e -= low ? 1 : 0;
+  m = low ? (m + m) - 1.0f : m - 1.0f;
+  float z = m * m;
+  float y = 7.0376836292E-2f;
+  y = __builtin_fmaf(y, m, -1.1514610310E-1f);
+  y = __builtin_fmaf(y, m, 1.1676998740E-1f);
+  y = __builtin_fmaf(y, m, -1.2420140846E-1f);
+  y = __builtin_fmaf(y, m, 1.4249322787E-1f);
+  y = __builtin_fmaf(y, m, -1.6668057665E-1f);
+  y = __builtin_fmaf(y, m, 2.0000714765E-1f);
+  y = __builtin_fmaf(y, m, -2.4999993993E-1f);
+  y = __builtin_fmaf(y, m, 3.3333331174E-1f);
+  y = y * m * z;
+  float fe = (float)e;
+  y = __builtin_fmaf(-2.12194440e-4f, fe, y);
+  y = __builtin_fmaf(-0.5f, z, y);
+  return __builtin_fmaf(0.693359375f, fe, m + y);
+}
+
 // Cephes sinf/cosf share one reduction to [-pi/4, pi/4]; valid for
 // |x| < 8192 (beyond that the reduction loses all bits; not used here).
 GS_INLINE float gs_sincos_poly_sin(float x, float z) {

@@ -31,7 +31,7 @@ export interface GaussianPassOptions {
   radixBits?: 4 | 6 | 8;
   rasterBatch?: 128 | 256;
   /** Heavy raster units above which 16-pixel tiles run as 4-row slabs
-   * (default: 2.5 per compute unit); never changes any output. */
+   * (default: 1.75 per compute unit); never changes any output. */
   heavySplitUnits?: number;
   device?: number;
 }

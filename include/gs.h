@@ -86,7 +86,7 @@ typedef struct gs_config {
   uint32_t heavy_split_units;        /* heavy raster units (tiles above 1024
                                         records, chunk tasks) above which 16-pixel
                                         tiles are rasterized as 4-row slabs;
-                                        GS_SPLIT_AUTO (default) = 2.5 per compute
+                                        GS_SPLIT_AUTO (default) = 1.75 per compute
                                         unit; never changes any output */
 } gs_config;
 

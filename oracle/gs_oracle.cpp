@@ -564,22 +564,22 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
     // direction, object-space position, log SH energy, normalised screen
     // position.
     float* f = &r.cull_features[(size_t)gid * 16];
-    f[0] = gs_logf(radius_x > radius_y ? radius_x : radius_y);
-    f[1] = gs_logf(conic.x);
+    f[0] = gs_logf_pos(radius_x > radius_y ? radius_x : radius_y);
+    f[1] = gs_logf_pos(conic.x);
     f[2] = conic.y / gs_sqrtf(conic.x * conic.z);
-    f[3] = gs_logf(conic.z);
+    f[3] = gs_logf_pos(conic.z);
     f[4] = opacity;
     f[5] = local.x * view_dir.x + local.y * view_dir.y + local.z * view_dir.z;
-    f[6] = gs_logf(depth);
+    f[6] = gs_logf_pos(depth);
     f[7] = view_dir.x;
     f[8] = view_dir.y;
     f[9] = view_dir.z;
     f[10] = local.x;
     f[11] = local.y;
     f[12] = local.z;
-    f[13] = gs_logf(sh_energy + 1e-6f);
-    f[14] = center.x / s.width;
-    f[15] = center.y / s.height;
+    f[13] = gs_logf_pos(sh_energy + 1e-6f);
+    f[14] = center.x * (1.0f / (float)s.width);
+    f[15] = center.y * (1.0f / (float)s.height);
   }
   pm[0] = center.x;
   pm[1] = center.y;
@@ -1119,6 +1119,7 @@ extern "C" void gs_oracle_math(int fn, const float* a, const float* b, float* ou
       case 5: out[i] = gs_atan2f(x, b[i]); break;
       case 6: out[i] = gs_half_to_float(gs_f2b(x) & 0xffffu); break;
       case 7: out[i] = gs_b2f(gs_float_to_half(x)); break;
+      case 8: out[i] = gs_logf_pos(x); break;
       default: out[i] = 0.0f;
     }
   }

@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-EXP, LOG, SIN, COS, ATAN, ATAN2, H2F, F2H = range(8)
+EXP, LOG, SIN, COS, ATAN, ATAN2, H2F, F2H, LOGPOS = range(9)
 
 
 @pytest.fixture(scope="module")
@@ -46,6 +46,19 @@ def test_log(mathfn):
     assert ulps(mathfn(LOG, x), np.log(x.astype(np.float64))).max() <= 1.0
     x = RNG.uniform(1.0, 255.0, 100_000).astype(np.float32)  # ln(255 * opacity)
     assert ulps(mathfn(LOG, x), np.log(x.astype(np.float64))).max() <= 1.0
+
+
+def test_log_positive_normal(mathfn):
+    """The learned-cull features' log (fused-multiply-add Cephes, no special
+    values) over positive normal arguments: within 1 ulp of libm, and within
+    1 ulp of gs_logf."""
+    x = np.exp(RNG.uniform(-85, 85, 100_000)).astype(np.float32)
+    x = x[x >= np.finfo(np.float32).tiny]
+    got = mathfn(LOGPOS, x)
+    assert ulps(got, np.log(x.astype(np.float64))).max() <= 1.0
+    assert ulps(got, mathfn(LOG, x).astype(np.float64)).max() <= 1.0
+    x = np.arange(1, 512, dtype=np.float32)  # pixel radii
+    assert ulps(mathfn(LOGPOS, x), np.log(x.astype(np.float64))).max() <= 1.0
 
 
 @pytest.mark.parametrize("fn,ref", [(SIN, np.sin), (COS, np.cos)])
