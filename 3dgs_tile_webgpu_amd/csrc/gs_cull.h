@@ -60,12 +60,13 @@ GS_HD inline float gs_bf2f(uint16_t h) { return gs_b2f((uint32_t)h << 16); }
 // log conic diagonal, conic correlation), opacity, the position along the
 // viewing ray (object-space position . view direction, a proxy for how
 // deep in the scene the Gaussian sits), log view depth, the object-space
-// view direction, the object-space position, log SH energy (sum of squared
-// higher-band coefficients) and the normalised screen position (inv_vw,
-// inv_vh: 1 / viewport size). Every log argument is positive and normal.
+// view direction, the object-space position, the log2 scale of the
+// strongest higher-band SH coefficient (its rgb8e8 exponent minus the bias)
+// and the normalised screen position (inv_vw, inv_vh: 1 / viewport size).
+// Every log argument is positive and normal.
 GS_HD inline void cull_features(float f[kCullIn], float rx, float ry, const float conic[3],
                                 float opacity, float depth, const float dir[3],
-                                const float pos[3], float sh_energy, float cx, float cy,
+                                const float pos[3], float sh_top, float cx, float cy,
                                 float inv_vw, float inv_vh) {
   f[0] = gs_logf_pos(rx > ry ? rx : ry);
   f[1] = gs_logf_pos(conic[0]);
@@ -80,7 +81,7 @@ GS_HD inline void cull_features(float f[kCullIn], float rx, float ry, const floa
   f[10] = pos[0];
   f[11] = pos[1];
   f[12] = pos[2];
-  f[13] = gs_logf_pos(sh_energy + 1e-6f);
+  f[13] = sh_top;
   f[14] = cx * inv_vw;
   f[15] = cy * inv_vh;
 }

@@ -95,16 +95,24 @@ constexpr uint32_t kHitMask = 1u << 16;
 // collection for its training set.
 enum ProjMode : int { kModePlain = 0, kModeCull = 1, kModeCollect = 2 };
 
-// ProjectionStage.ts:385-649 for active ordinal gid. With kMode != plain it
-// also fills the 16 cull features of a visible Gaussian.
-template <int kMode>
-GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint32_t gid,
-                                float feat[kCullIn]) {
-  ProjOut o;
+// A Gaussian projected up to its tile footprint (ProjectionStage.ts:385-
+// 600): kind is kProjNone, kProjSubpixel (mean published with w = -1) or
+// kProjVisible for a candidate whose exact tile count is still to be taken.
+struct ProjGeom {
+  int kind;
+  uint32_t slot;
+  float cx, cy, depth, opacity;
+  float conic[3];
+  float thr, rx, ry;
+  float dir[3];    // normalised object-space offset from the camera (SH)
+  float local[3];  // object-space mean (learned-cull features)
+};
+
+GS_DEV ProjGeom project_geometry(const FrameParams& fp, const SceneView& sc, uint32_t gid) {
+  ProjGeom o;
   o.kind = kProjNone;
-  o.count = 0u;
-  o.hits = make_uint4(0u, 0u, 0u, 0u);
   uint32_t slot = fp.has_active_slots ? sc.active_slots[gid] : gid;
+  o.slot = slot;
   float4 mean = sc.means[slot];
   if (mean.w < 0.0f) return o;
   uint32_t object = gs_f2u(mean.w);
@@ -161,13 +169,50 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   if ((axis != 1 && (bmaxx < 0.0f || bminx >= vw)) ||
       (axis != 2 && (bmaxy < 0.0f || bminy >= vh)))
     return o;
+  o.cx = p.cx;
+  o.cy = p.cy;
+  o.depth = p.depth;
   if (fp.subpixel_culling &&
       !subpixel_has_sample(p.cx, p.cy, p.conic, thr, ext_x, ext_y, gs_f2u(vw), gs_f2u(vh),
                            axis)) {
     o.kind = kProjSubpixel;
-    o.mean = make_float4(p.cx, p.cy, p.depth, -1.0f);
     return o;
   }
+  o.kind = kProjVisible;
+  o.opacity = opacity;
+  o.conic[0] = p.conic[0];
+  o.conic[1] = p.conic[1];
+  o.conic[2] = p.conic[2];
+  o.thr = thr;
+  o.rx = rx;
+  o.ry = ry;
+  o.dir[0] = dir[0];
+  o.dir[1] = dir[1];
+  o.dir[2] = dir[2];
+  o.local[0] = local[0];
+  o.local[1] = local[1];
+  o.local[2] = local[2];
+  return o;
+}
+
+// The exact tile count, hit record and SH colour of a candidate
+// (ProjectionStage.ts:600-649): visible if it hits a tile.
+GS_DEV ProjOut project_finish(const FrameParams& fp, const SceneView& sc, const ProjGeom& g) {
+  ProjOut o;
+  o.kind = kProjNone;
+  o.count = 0u;
+  o.hits = make_uint4(0u, 0u, 0u, 0u);
+  if (g.kind != kProjVisible) {
+    o.kind = g.kind;
+    if (g.kind == kProjSubpixel) o.mean = make_float4(g.cx, g.cy, g.depth, -1.0f);
+    return o;
+  }
+  const uint32_t slot = g.slot;
+  const float thr = g.thr, rx = g.rx, ry = g.ry;
+  const float conic[3] = {g.conic[0], g.conic[1], g.conic[2]};
+  float bminx = g.cx - rx, bminy = g.cy - ry, bmaxx = g.cx + rx, bmaxy = g.cy + ry;
+  float vw = fp.viewport[0], vh = fp.viewport[1];
+  uint32_t axis = fp.boundary_axis;
   float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
   float ts = (float)fp.tile_size;
   // x / ts as x * (1 / ts): the tile edge is a power of two, so both are the
@@ -191,7 +236,7 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   uint32_t mask = 0u, bitpos = 0u;
   for (int ty = tmin_y; ty <= tmax_y; ty++)
     for (int tx = tmin_x; tx <= tmax_x; tx++, bitpos++)
-      if (tile_hit(p.cx, p.cy, p.conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
+      if (tile_hit(g.cx, g.cy, conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
         count++;
         if (small) mask |= 1u << bitpos;
       }
@@ -220,12 +265,11 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
 #pragma unroll
     for (int v = 0; v < 16; v++) shw[v] = (uint32_t)v < ncoef ? shc[v] : 0u;
   }
-  float x = dir[0], y = dir[1], z = dir[2];
+  float x = g.dir[0], y = g.dir[1], z = g.dir[2];
   float k[3];
   decode_sh(shw[0], k);
   float w0 = 0.28209479177387814f;
   float col[3] = {w0 * k[0], w0 * k[1], w0 * k[2]};
-  float sh_energy = 0.0f;
 #define GS_SH_ADD(wexpr, idx)                       \
   {                                                 \
     float w_ = (wexpr);                             \
@@ -233,10 +277,6 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
     col[0] = col[0] + w_ * k[0];                    \
     col[1] = col[1] + w_ * k[1];                    \
     col[2] = col[2] + w_ * k[2];                    \
-    if (kMode != kModePlain)                        \
-      sh_energy = __builtin_fmaf(                   \
-          k[2], k[2],                               \
-          __builtin_fmaf(k[1], k[1], __builtin_fmaf(k[0], k[0], sh_energy))); \
   }
   if (degree >= 1) {
     GS_SH_ADD(-0.4886025119029199f * y, 1);
@@ -263,17 +303,50 @@ GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint
   }
 #undef GS_SH_ADD
   float r = gs_clampf(col[0] + 0.5f, 0.0f, 1.0f);
-  float g = gs_clampf(col[1] + 0.5f, 0.0f, 1.0f);
+  float g_ = gs_clampf(col[1] + 0.5f, 0.0f, 1.0f);
   float b = gs_clampf(col[2] + 0.5f, 0.0f, 1.0f);
   o.kind = kProjVisible;
-  o.mean = make_float4(p.cx, p.cy, p.depth, opacity);
-  o.conic = make_float4(p.conic[0], p.conic[1], p.conic[2], rx + ry * 512.0f);
-  o.color = make_float4(r, g, b, gs_b2f(slot ^ 0x3f800000u));
+  o.mean = make_float4(g.cx, g.cy, g.depth, g.opacity);
+  o.conic = make_float4(conic[0], conic[1], conic[2], rx + ry * 512.0f);
+  o.color = make_float4(r, g_, b, gs_b2f(slot ^ 0x3f800000u));
   o.thr = thr;
   o.count = count;
-  if (kMode != kModePlain)
-    cull_features(feat, rx, ry, p.conic, opacity, p.depth, dir, local, sh_energy, p.cx, p.cy,
-                  fp.inv_viewport[0], fp.inv_viewport[1]);
+  return o;
+}
+
+// The largest rgb8e8 exponent of the higher SH bands, minus the bias: the
+// log2 scale of the strongest view-dependent colour term (a learned-cull
+// feature; -127 without higher bands).
+GS_DEV float sh_top_exponent(const FrameParams& fp, const SceneView& sc, uint32_t slot) {
+  uint32_t degree = fp.sh_degree;
+  uint32_t ncoef = (degree + 1) * (degree + 1);
+  const uint32_t* shc = sc.sh + (size_t)slot * ncoef;
+  uint32_t top = 0u;
+  if (degree == 3) {
+    const uint4* sh4 = reinterpret_cast<const uint4*>(shc);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      uint4 q4 = sh4[v];
+      if (v > 0) top = max(top, q4.x >> 24);
+      top = max(max(top, q4.y >> 24), max(q4.z >> 24, q4.w >> 24));
+    }
+  } else {
+    for (uint32_t v = 1; v < ncoef; v++) top = max(top, shc[v] >> 24);
+  }
+  return (float)(int)top - 127.0f;
+}
+
+// ProjectionStage.ts:385-649 for active ordinal gid. With kMode != plain it
+// also fills the 16 cull features of a visible Gaussian.
+template <int kMode>
+GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint32_t gid,
+                                float feat[kCullIn]) {
+  ProjGeom g = project_geometry(fp, sc, gid);
+  ProjOut o = project_finish(fp, sc, g);
+  if (kMode != kModePlain && o.kind == kProjVisible)
+    cull_features(feat, g.rx, g.ry, g.conic, g.opacity, g.depth, g.dir, g.local,
+                  sh_top_exponent(fp, sc, g.slot), g.cx, g.cy, fp.inv_viewport[0],
+                  fp.inv_viewport[1]);
   return o;
 }
 
@@ -320,16 +393,32 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
   store_projection(out, gid, o);
 }
 
-// The learned-cull projection: after projecting, every wave with a visible
-// Gaussian runs the MLP over its 64 lanes (matrix cores, weights staged in
-// LDS per block) and drops those predicted below the threshold as if they
-// touched no tile.
+// The learned-cull projection, in two phases. Every Gaussian is projected
+// up to its tile footprint; every wave with a candidate runs the MLP over
+// its 64 lanes (matrix cores, weights staged in LDS per block) and drops
+// those predicted below the threshold as if they touched no tile. The
+// survivors (a minority once the cull is trained) are then gathered in
+// LDS, packed at the front of the block, and only they take the exact tile
+// count and SH colour -- in dense waves, so the culled lanes cost nothing
+// there.
+struct CullSurvivor {
+  uint32_t gid, slot;
+  float cx, cy, depth, opacity, c0, c1, c2, thr, rx, ry, d0, d1, d2;
+};
 __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
                                                          DeviceState* __restrict__ state,
                                                          SceneView sc, ProjectedView out,
                                                          const CullNet* __restrict__ net) {
-  __shared__ CullLdsNet lnet;
-  __shared__ __align__(16) uint16_t lx[kBlock / 64][64 * kCullIn];
+  struct Mlp {
+    CullLdsNet net;
+    __align__(16) uint16_t x[kBlock / 64][64 * kCullIn];
+  };
+  // The survivors overwrite the network once every wave has run it.
+  __shared__ union {
+    Mlp mlp;
+    CullSurvivor surv[kBlock];
+  } lds;
+  __shared__ uint32_t wave_kept[kBlock / 64];
   uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
   if (gid == 0) {
     state->checked = 0ull;
@@ -339,20 +428,70 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
   }
   // The network's loads are issued first, so that their latency overlaps
   // the projection's.
-  cull_stage_net(net, lnet);
+  cull_stage_net(net, lds.mlp.net);
+  ProjGeom g;
+  g.kind = kProjNone;
+  if (gid < fp.active_count) g = project_geometry(fp, sc, gid);
+  bool cand = g.kind == kProjVisible;
   float feat[kCullIn];
-  ProjOut o;
-  o.kind = kProjNone;
-  if (gid < fp.active_count) o = project_gaussian<kModeCull>(fp, sc, gid, feat);
+#pragma unroll
+  for (int k = 0; k < kCullIn; k++) feat[k] = 0.0f;
+  if (cand)
+    cull_features(feat, g.rx, g.ry, g.conic, g.opacity, g.depth, g.dir, g.local,
+                  sh_top_exponent(fp, sc, g.slot), g.cx, g.cy, fp.inv_viewport[0],
+                  fp.inv_viewport[1]);
   __syncthreads();
-  bool visible = o.kind == kProjVisible;
-  if (__ballot(visible) != 0ull) {  // wave-uniform
-    if (!visible)
-      for (int k = 0; k < kCullIn; k++) feat[k] = 0.0f;
-    float y = cull_infer_wave(lnet, lx[threadIdx.x >> 6], feat);
-    if (visible && y < lnet.log_threshold) o.kind = kProjNone;
+  bool keep = false;
+  if (__ballot(cand) != 0ull) {  // wave-uniform
+    float y = cull_infer_wave(lds.mlp.net, lds.mlp.x[threadIdx.x >> 6], feat);
+    keep = cand && !(y < lds.mlp.net.log_threshold);
   }
-  if (gid < fp.active_count) store_projection(out, gid, o);
+  // Culled, subpixel and invisible Gaussians are published now.
+  if (gid < fp.active_count && !keep) {
+    ProjOut o;
+    o.kind = g.kind == kProjSubpixel ? kProjSubpixel : kProjNone;
+    o.mean = make_float4(g.cx, g.cy, g.depth, -1.0f);
+    store_projection(out, gid, o);
+  }
+  // Pack the survivors at the front of the block, in thread order.
+  const uint64_t kept = __ballot(keep);
+  const uint32_t wave = threadIdx.x >> 6;
+  if (lane_id() == 0) wave_kept[wave] = (uint32_t)__popcll(kept);
+  __syncthreads();  // every wave is past the network: its LDS is free
+  uint32_t before = 0u, total = 0u;
+#pragma unroll
+  for (uint32_t w = 0; w < kBlock / 64; w++) {
+    uint32_t c = wave_kept[w];
+    if (w < wave) before += c;
+    total += c;
+  }
+  if (keep) {
+    uint64_t lower = lane_id() == 0 ? 0ull : (kept & (~0ull >> (64 - lane_id())));
+    CullSurvivor& v = lds.surv[before + (uint32_t)__popcll(lower)];
+    v = {gid, g.slot, g.cx, g.cy, g.depth, g.opacity, g.conic[0], g.conic[1], g.conic[2],
+         g.thr, g.rx, g.ry, g.dir[0], g.dir[1], g.dir[2]};
+  }
+  __syncthreads();
+  if (threadIdx.x < total) {
+    const CullSurvivor v = lds.surv[threadIdx.x];
+    ProjGeom s;
+    s.kind = kProjVisible;
+    s.slot = v.slot;
+    s.cx = v.cx;
+    s.cy = v.cy;
+    s.depth = v.depth;
+    s.opacity = v.opacity;
+    s.conic[0] = v.c0;
+    s.conic[1] = v.c1;
+    s.conic[2] = v.c2;
+    s.thr = v.thr;
+    s.rx = v.rx;
+    s.ry = v.ry;
+    s.dir[0] = v.d0;
+    s.dir[1] = v.d1;
+    s.dir[2] = v.d2;
+    store_projection(out, v.gid, project_finish(fp, sc, s));
+  }
 }
 
 // ----------------------------------------------------------------- scans ----

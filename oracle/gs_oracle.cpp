@@ -242,9 +242,18 @@ bool projection_is_valid(const M4& p) {
          gs_fabsf(conic.z - e.z) <= gs_maxf(1e-6f, gs_fabsf(e.z) * 1e-3f);
 }
 
+// The largest rgb8e8 exponent of the higher SH bands minus the bias (a
+// learned-cull feature; no reference counterpart): -127 without them.
+float sh_top_exponent(const uint32_t* sh, uint32_t slot, uint32_t degree) {
+  uint32_t n = (degree + 1) * (degree + 1);
+  const uint32_t* c = sh + (size_t)slot * n;
+  uint32_t top = 0u;
+  for (uint32_t i = 1; i < n; i++) top = std::max(top, c[i] >> 24);
+  return (float)(int)top - 127.0f;
+}
+
 // kernels/projectionHelpers.ts:118-176 (evaluate_gaussian_sh_rgb8e8)
-V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir,
-               float* energy = nullptr) {
+V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir) {
   float x = dir.x, y = dir.y, z = dir.z;
   uint32_t n = (degree + 1) * (degree + 1);
   const uint32_t* c = sh + (size_t)slot * n;
@@ -254,15 +263,11 @@ V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir,
     float w = 0.28209479177387814f;
     color = {w * k.x, w * k.y, w * k.z};
   }
-  // energy: sum over the higher-band coefficients of |k|^2, in evaluation
-  // order (a learned-cull feature; no reference counterpart).
-  float e = 0.0f;
   auto add = [&](float w, uint32_t i) {
     V3 k = decode_sh(c[i]);
     color.x = color.x + w * k.x;
     color.y = color.y + w * k.y;
     color.z = color.z + w * k.z;
-    e = __builtin_fmaf(k.z, k.z, __builtin_fmaf(k.y, k.y, __builtin_fmaf(k.x, k.x, e)));
   };
   if (degree >= 1) {
     add(-0.4886025119029199f * y, 1);
@@ -287,7 +292,6 @@ V3 evaluate_sh(const uint32_t* sh, uint32_t slot, uint32_t degree, V3 dir,
     add(1.445305721320277f * z * (xx - yy), 14);
     add(-0.5900435899266435f * x * (xx - 3.0f * yy), 15);
   }
-  if (energy) *energy = e;
   return {gs_clampf(color.x + 0.5f, 0.0f, 1.0f), gs_clampf(color.y + 0.5f, 0.0f, 1.0f),
           gs_clampf(color.z + 0.5f, 0.0f, 1.0f)};
 }
@@ -552,8 +556,7 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
   int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmax.y / ts), 0.0f, max_ty));
   if (s.boundary_axis == 1) { tmin_x = 0; tmax_x = (int)tiles_x - 1; }
   if (s.boundary_axis == 2) { tmin_y = 0; tmax_y = (int)tiles_y - 1; }
-  float sh_energy = 0.0f;
-  V3 color = evaluate_sh(s.sh, slot, s.sh_degree, view_dir, &sh_energy);
+  V3 color = evaluate_sh(s.sh, slot, s.sh_degree, view_dir);
   uint32_t count = count_tiles(center, conic, thr, V2{radius_x, radius_y}, tmin_x,
                                tmin_y, tmax_x, tmax_y, V2{s.width, s.height},
                                s.boundary_axis, ts);
@@ -561,7 +564,7 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
   if (s.cull_outputs) {
     // Learned-cull features: log pixel radius, log conic diagonal, conic
     // correlation, opacity, position . view direction, log depth, view
-    // direction, object-space position, log SH energy, normalised screen
+    // direction, object-space position, top higher-band SH exponent, normalised screen
     // position.
     float* f = &r.cull_features[(size_t)gid * 16];
     f[0] = gs_logf_pos(radius_x > radius_y ? radius_x : radius_y);
@@ -577,7 +580,7 @@ void project_one(const gs_oracle_scene& s, uint32_t tiles_x, uint32_t tiles_y,
     f[10] = local.x;
     f[11] = local.y;
     f[12] = local.z;
-    f[13] = gs_logf_pos(sh_energy + 1e-6f);
+    f[13] = sh_top_exponent(s.sh, slot, s.sh_degree);
     f[14] = center.x * (1.0f / (float)s.width);
     f[15] = center.y * (1.0f / (float)s.height);
   }
