@@ -75,6 +75,23 @@ def cpu_baseline(gs, scene, camera, workload, threads):
     }
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1m_1080p_hbm_traffic.md"
+
+
+def profiled_traffic(kernel_prefix):
+    """HBM bytes per dispatch of a kernel (read + written, counter-scaled) from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 1M/1080p
+    frame, or None when the summary has no such row."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 6 and cells[0].strip("`").startswith(kernel_prefix):
+                return int(round((float(cells[4]) + float(cells[5])) * 1e6))
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -291,7 +308,11 @@ def main():
         achieved = flops / (heavy_kernel_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                    "traffic": None, "kernel": "k_raster_heavy",
+                    "traffic": (profiled_traffic("gs::k_raster_heavy<16, false, false>")
+                                if args.workload == "1m_1080p" else None),
+                    "traffic_source": "profiles/frame_1m_1080p_hbm_traffic.md (rocprofv3 "
+                                      "FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                    "kernel": "k_raster_heavy",
                     "kernel_ms": round(heavy_kernel_ms, 4), "algorithmic_flops": int(flops),
                     "checked_pairs_kernel": int(checked_heavy), "checked_pairs_frame": int(checked),
                     "raster_stage_ms": round(stage_ms["raster"], 4),
