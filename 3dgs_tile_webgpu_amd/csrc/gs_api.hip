@@ -1235,7 +1235,15 @@ extern "C" int gs_read_debug(gs_ctx* ctx, const char* name, void* dst, size_t by
   else if (n == "projected_color") { src = &b.proj.splat[0].color; len = a * 16; pitch = sizeof(gs::Splat); }
   else if (n == "tile_counts") { src = b.proj.tile_counts; len = a * 4; }
   else if (n == "visible_offsets") { src = b.visible_offsets; len = a * 4; }
-  else if (n == "depth_records") { src = gs::sorted_depth_records(b); len = (size_t)st.visible * 8; }
+  else if (n == "depth_records") {
+    // Which buffer holds them depends on whether the last depth pass ran.
+    uint32_t in_place = 0u;
+    hipError_t e = hipMemcpy(&in_place, &b.state->depth_in_place, sizeof(in_place),
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "state readback");
+    src = gs::sorted_depth_records(b, in_place);
+    len = (size_t)st.visible * 8;
+  }
   else if (n == "ordered_counts") { src = b.ordered_counts; len = (size_t)st.visible * 4; }
   else if (n == "intersection_offsets") { src = b.intersection_offsets; len = (size_t)st.visible * 4; }
   else if (n == "sorted_records") { src = gs::sorted_tile_records(b); len = (size_t)st.intersections * 8; }

@@ -80,6 +80,8 @@ struct DeviceState {
   uint32_t heavy_split;  // this frame's heavy units run as row slabs
   uint32_t light_units;  // row-slab frames: light tiles listed after the heavy units
   uint32_t heavy_next[8];  // per XCD: heavy work items taken beyond the first round
+  uint32_t depth_in_place;  // the depth sort's last digit was one value for every key: its
+                            // pass was skipped and the result is that pass's input buffer
 };
 
 struct SceneView {
@@ -198,8 +200,9 @@ struct RasterStreams {
 // `marks` is null or kMarks events (stage boundaries, heavy-unit kernels).
 hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, const RasterStreams& rs,
                         hipEvent_t* marks);
-// Location of the sorted depth / tile records after launch_frame.
-const uint2* sorted_depth_records(const FrameBuffers& b);
+// Location of the sorted depth / tile records after launch_frame
+// (depth_in_place: DeviceState::depth_in_place of the frame).
+const uint2* sorted_depth_records(const FrameBuffers& b, uint32_t depth_in_place);
 const uint2* sorted_tile_records(const FrameBuffers& b);
 
 // Learned-cull training and evaluation launches (gs_cull.hip).

@@ -529,10 +529,13 @@ struct LoadVisible {  // scan.ts:78-83: 1 if projectedMean.w > 0
   GS_DEV uint32_t operator()(uint32_t i) const { return mean[i].w > 0.0f ? 1u : 0u; }
 };
 struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
-  const uint2* sorted;
+  const uint2* sorted_a;   // the depth sort's last output ...
+  const uint2* sorted_b;   // ... and its last input (when that pass was skipped)
+  const uint32_t* in_place;
   const uint4* hits;       // per ordinal; .w is the tile count
   uint4* ordered_hits;     // the same records in depth order, for emission
   GS_DEV uint32_t operator()(uint32_t r) const {
+    const uint2* sorted = *in_place ? sorted_b : sorted_a;
     uint4 h = hits[sorted[r].y];
     ordered_hits[r] = h;
     return h.w;
@@ -696,7 +699,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 //    registers. Keys keep their input order within a digit.
 // A pass whose digit totals put every key in one digit is a stable identity
 // permutation; its blocks only copy their keys (depth keys of one frame
-// often share the top byte).
+// often share the top byte). When the caller passes `in_place` for the last
+// pass, such a pass does not even copy: it records 1 there (0 otherwise) and
+// the consumers read the pass's input buffer.
 // Passes alternate between two sets of status rows: each block of pass p
 // clears its own row in the other set for pass p + 1 (pass p - 1, its last
 // user, has finished; every pass has the same partitions). A block waits
@@ -811,7 +816,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
                                                       uint2* __restrict__ out,
                                                       const uint32_t* __restrict__ count_ptr,
                                                       uint32_t pass, SortWorkspace ws,
-                                                      DeviceState* __restrict__ state) {
+                                                      DeviceState* __restrict__ state,
+                                                      uint32_t* __restrict__ in_place) {
   constexpr uint32_t kRadix = 1u << kBits;
   constexpr uint32_t kRun = 64u * kSortItems;  // keys of one wave, consecutive
   __shared__ uint32_t wave_cnt[kSortWaves][kRadix];  // running counts, then offsets
@@ -821,6 +827,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   const uint32_t count = *count_ptr;
   const uint32_t nparts = (count + kSortPartition - 1) / kSortPartition;
   const uint32_t part = blockIdx.x;
+  if (in_place) {
+    // The last pass: a digit shared by every key leaves the order as it is.
+    const uint32_t total = t < kRadix ? ws.ghist[pass * 256u + t] : 0u;
+    bool uniform = __syncthreads_or(t < kRadix && total == count);
+    if (part == 0 && t == 0) *in_place = uniform ? 1u : 0u;
+    if (uniform) return;
+  }
   if (part >= nparts) return;  // block-uniform
   const uint32_t set = pass & 1u;
   const uint32_t shift = pass * kBits;
@@ -930,7 +943,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
 // exact tile test as the projection; sentinel (T, id) pads reserved slots.
 __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
                                                  const DeviceState* __restrict__ state,
-                                                 const uint2* __restrict__ sorted_depth,
+                                                 const uint2* __restrict__ sorted_a,
+                                                 const uint2* __restrict__ sorted_b,
                                                  const Splat* __restrict__ splat,
                                                  const uint4* __restrict__ ordered_hits,
                                                  const uint32_t* __restrict__ offsets,
@@ -938,6 +952,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
   uint32_t rank = blockIdx.x * kBlock + threadIdx.x;
   if (state->overflow || rank >= state->visible) return;
   uint32_t capacity = fp.capacity;
+  // The depth sort's last output, or its last input if that pass was skipped.
+  const uint2* sorted_depth = state->depth_in_place ? sorted_b : sorted_a;
   uint32_t gid = sorted_depth[rank].y;
   uint4 h = ordered_hits[rank];
   uint32_t reserved = h.w;
@@ -1840,7 +1856,7 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 template <int kBits>
 void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
                      uint32_t passes, const SortWorkspace& ws, DeviceState* state,
-                     hipStream_t s) {
+                     uint32_t* in_place, hipStream_t s) {
   uint32_t blocks = div_up(max_count, kSortPartition);
   if (blocks == 0) blocks = 1;
   uint32_t clear_blocks = div_up((uint64_t)blocks * 256u + kMaxPasses * 256u, kBlock);
@@ -1851,20 +1867,25 @@ void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max
   for (uint32_t p = 0; p < passes; p++) {
     const uint2* in = (p % 2 == 0) ? a : b;
     uint2* out = (p % 2 == 0) ? b : a;
-    k_sort_pass<kBits><<<blocks, kSortThreads, 0, s>>>(in, out, count_ptr, p, ws, state);
+    k_sort_pass<kBits><<<blocks, kSortThreads, 0, s>>>(in, out, count_ptr, p, ws, state,
+                                                       p + 1 == passes ? in_place : nullptr);
   }
 }
 
+// Sort pairs in a <-> b (see radix_sort_bits). With in_place, a last pass
+// whose digit is one value for every key is skipped and *in_place records
+// it; the sorted pairs are then in the last pass's input buffer.
 void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
                 uint32_t passes, uint32_t bits, const SortWorkspace& ws, DeviceState* state,
-                hipStream_t s) {
+                uint32_t* in_place, hipStream_t s) {
   if (bits == 4)
-    radix_sort_bits<4>(a, b, count_ptr, max_count, passes, ws, state, s);
+    radix_sort_bits<4>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
   else if (bits == 6)
-    radix_sort_bits<6>(a, b, count_ptr, max_count, passes, ws, state, s);
+    radix_sort_bits<6>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
   else
-    radix_sort_bits<8>(a, b, count_ptr, max_count, passes, ws, state, s);
+    radix_sort_bits<8>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
 }
+
 
 // The raster stage: the heavy work list on s, then heavy units on aux
 // concurrently with light tiles on s, then the chunk composite.
@@ -1950,23 +1971,24 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStream
   mark(kStageDepthSort);
   SortWorkspace ws = {b.hist, b.hist + kMaxPasses * 256, b.sort_parts, sws.status, scan_blocks};
   radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.radix_bits, ws, st,
-             s);
-  const uint2* sorted_depth = sorted_depth_records(b);
+             &st->depth_in_place, s);
+  const uint2* depth_out = sorted_depth_records(b, 0u), *depth_in = sorted_depth_records(b, 1u);
 
   // Depth-ordered tile counts -> write offsets; K, overflow, output state.
   mark(kStageScan);
   LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
   k_scan<LoadOrdered, StoreOrdered><<<scan_blocks, kBlock, 0, s>>>(
-      LoadOrdered{sorted_depth, b.proj.hits, b.ordered_hits},
+      LoadOrdered{depth_out, depth_in, &st->depth_in_place, b.proj.hits, b.ordered_hits},
       StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, sws, kScanIntersections,
       fp.capacity, fp.camera, st);
 
   mark(kStageEmit);
-  k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, sorted_depth, b.proj.splat, b.ordered_hits, b.intersection_offsets, b.records_a);
+  k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, depth_out, depth_in, b.proj.splat, b.ordered_hits,
+                                     b.intersection_offsets, b.records_a);
 
   mark(kStageTileSort);
   radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, ws, st,
-             s);
+             nullptr, s);
   const uint2* sorted = sorted_tile_records(b);
 
   mark(kStageTileOffsets);
@@ -1986,8 +2008,11 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStream
   return hipGetLastError();
 }
 
-const uint2* sorted_depth_records(const FrameBuffers& b) {
-  return (b.depth_passes % 2 == 0) ? b.depth_a : b.depth_b;
+const uint2* sorted_depth_records(const FrameBuffers& b, uint32_t depth_in_place) {
+  // Pass p reads a for even p; the result is the last pass's output, or its
+  // input when that pass was skipped.
+  uint32_t filled = depth_in_place ? b.depth_passes - 1u : b.depth_passes;
+  return (filled % 2 == 0) ? b.depth_a : b.depth_b;
 }
 const uint2* sorted_tile_records(const FrameBuffers& b) {
   return (b.tile_passes % 2 == 0) ? b.records_a : b.records_b;
