@@ -1227,7 +1227,7 @@ GS_DEV void mark_candidates(Lds& L, uint32_t slot, const Rect& rc, int tx0, int 
   }
 }
 
-// Blend records [cb, ce) of `tile` into one pixel per thread (threads
+// Blend records [cb, ce) (cb < ce) of `tile` into one pixel per thread (threads
 // 0..E*E-1), from T = 1. All kThreads threads of the block must call it;
 // threads below kBatch stage one record per round (a record whose support
 // rectangle is empty sets no bits and is not staged). The instrumented variant
@@ -1255,14 +1255,15 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   float threshold = fp.transmittance_threshold;
   PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
   bool done = !active;
-  // Prefetch the first round.
-  uint32_t idx = cb + t;
-  float4 m = make_float4(0.f, 0.f, 0.f, 0.f), c = m, k = m;
-  uint32_t cg = 0u;
-  if (stager && idx < ce) {
-    cg = sorted[idx].y;
-    gather_splat(splat, cg, m, c, k);
-  }
+  // Each thread holds the splat of its slot one round ahead and the ordinal
+  // two rounds ahead. Slots past the range load its last record (never
+  // staged): unconditional loads land in the loop-carried registers, where
+  // a conditional one makes the compiler wait for the load at the join.
+  const uint32_t last = ce - 1u;
+  float4 m, c, k;
+  uint32_t cg = sorted[min(cb + t, last)].y;
+  uint32_t cg_next = sorted[min(cb + kBatch + t, last)].y;
+  gather_splat(splat, cg, m, c, k);
   __syncthreads();
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
     for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
@@ -1281,11 +1282,9 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
       if (kRecord) L.gid[t] = cg;
     }
     float4 cm = m, cc = c;
-    uint32_t nidx = batch + kBatch + t;
-    if (stager && nidx < ce) {
-      cg = sorted[nidx].y;
-      gather_splat(splat, cg, m, c, k);
-    }
+    cg = cg_next;
+    gather_splat(splat, cg, m, c, k);
+    cg_next = sorted[min(batch + 2u * kBatch + t, last)].y;
     if (hit) mark_candidates<kEdge>(L, t, crc, tx0, ty0, cm, cc, vw, vh, axis);
     __syncthreads();
     // Ordered blend over this pixel's candidates: each lane walks its own
@@ -1365,7 +1364,7 @@ GS_DEV uint32_t quad_or(uint32_t v) {
 // order, so the four lanes hold the same sum, bit for bit the one-lane
 // result. Next nonzero mask words come from a per-pixel summary word, so no
 // lane scans empty words. The record ordinals of round r + 2 are loaded
-// while round r runs, the records of round r + 1 gathered.
+// while round r runs, the records of round r + 1 gathered. Needs cb < ce.
 template <int kEdge, int kSlabRows, int kBatch, bool kRecord>
 GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t slab,
                                  uint32_t cb, uint32_t ce, const uint2* __restrict__ sorted,
@@ -1392,19 +1391,19 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
   PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
   bool done = !active;
   // Per staged record: its ordinal two rounds ahead, its splat one round
-  // ahead.
+  // ahead. Slots past the unit load the unit's last record (and are never
+  // staged): unconditional loads land in the loop-carried registers, where
+  // a conditional one makes the compiler wait for the gather as soon as it
+  // is issued (to move its result at the join).
+  const uint32_t last = ce - 1u;
   uint32_t cg[kPer], cg_next[kPer];
   float4 m[kPer], c[kPer], k[kPer];
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     uint32_t r = cb + e * kThreads + t;
-    cg[e] = cg_next[e] = 0u;
-    m[e] = c[e] = k[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (r < ce) {
-      cg[e] = sorted[r].y;
-      gather_splat(splat, cg[e], m[e], c[e], k[e]);
-    }
-    if (r + kBatch < ce) cg_next[e] = sorted[r + kBatch].y;
+    cg[e] = sorted[min(r, last)].y;
+    gather_splat(splat, cg[e], m[e], c[e], k[e]);
+    cg_next[e] = sorted[min(r + kBatch, last)].y;
   }
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
     for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
@@ -1430,12 +1429,9 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
       }
       cm[e] = m[e];
       cc[e] = c[e];
-      uint32_t r = batch + kBatch + slot;
-      if (r < ce) {
-        cg[e] = cg_next[e];
-        gather_splat(splat, cg[e], m[e], c[e], k[e]);
-      }
-      if (r + kBatch < ce) cg_next[e] = sorted[r + kBatch].y;
+      cg[e] = cg_next[e];
+      gather_splat(splat, cg[e], m[e], c[e], k[e]);
+      cg_next[e] = sorted[min(batch + 2u * kBatch + slot, last)].y;
     }
 #pragma unroll
     for (int e = 0; e < kPer; e++)
