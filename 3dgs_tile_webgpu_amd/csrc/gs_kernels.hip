@@ -1127,14 +1127,17 @@ struct PixelSum {
 // Block shapes. A tile of edge E has E*E pixels. Light tiles (up to
 // kLightMax records) use one thread per pixel, max(E*E, batch) threads, and
 // stage `batch` records per round (128 or 256, tunable). Heavy units (tiles
-// above kLightMax records and chunk tasks) use four lanes per pixel at
-// edges 8 and 16 and stage one record per thread per round:
-//  - whole tiles: one block per unit (256 threads at edge 8, 1024 at 16);
+// above kLightMax records and chunk tasks) use a group of lanes per pixel
+// at edges 8 and 16 (heavy_lanes) and stage one record per thread per
+// round:
+//  - whole tiles: one block per unit, four lanes per pixel (256 threads at
+//    edge 8, 1024 at 16);
 //  - row slabs (edge 16, frames with more than FrameParams::split_units
-//    units): four blocks of 256 threads per unit, each covering 4 rows. Such
-//    blocks share a CU four at a time and each stops as soon as its own
-//    pixels are saturated, at the price of staging every record once per
-//    slab, which only pays when the units outnumber the CUs.
+//    units): four blocks of 128 threads per unit, each covering 4 rows with
+//    two lanes per pixel. Such blocks share a CU ten at a time and each
+//    stops as soon as its own pixels are saturated, at the price of staging
+//    every record once per slab, which only pays when the units outnumber
+//    the CUs.
 // Edge 32 uses one block of 1024 threads per unit (one lane per pixel, 512
 // records per round).
 constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
@@ -1144,8 +1147,14 @@ constexpr int light_threads(int e, int batch) { return e * e > batch ? e * e : b
 constexpr int heavy_slab_rows(int e, bool split) { return e == 16 && split ? 4 : e; }
 constexpr int heavy_slabs(int e, bool split) { return e / heavy_slab_rows(e, split); }
 constexpr int heavy_slab_pixels(int e, bool split) { return e * heavy_slab_rows(e, split); }
+// Lanes per pixel at edges 8 and 16: four for whole tiles, whose frames
+// have about as many units as CUs and wait on the longest unit (more lanes,
+// shorter units); two for row slabs, whose frames have many more units
+// than CUs and are bound by total VALU work (the lanes of a group repeat
+// every blend, so fewer lanes do less of it).
+constexpr int heavy_lanes(int e, bool split) { return e == 16 && split ? 2 : 4; }
 constexpr int heavy_threads(int e, bool split) {
-  return e == 32 ? kHeavyThreads : 4 * heavy_slab_pixels(e, split);
+  return e == 32 ? kHeavyThreads : heavy_lanes(e, split) * heavy_slab_pixels(e, split);
 }
 constexpr int heavy_batch(int e, bool split) { return e == 32 ? 512 : heavy_threads(e, split); }
 
@@ -1340,45 +1349,59 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   return s;
 }
 
-// DPP broadcast of lane k of each quad of lanes to the whole quad.
-template <int k>
-GS_DEV float quad_bcast(float v) {
+// DPP broadcast of lane k of each group of kLanes lanes (2 or 4) to the
+// group: quad_perm [k, k, k, k], or [k, k, k + 2, k + 2] for pairs.
+template <int kLanes, int k>
+GS_DEV float group_bcast(float v) {
+  constexpr int kSel = kLanes == 4 ? k * 85 : (k == 0 ? 0xa0 : 0xf5);
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), k * 85, 0xf, 0xf, false));
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kSel, 0xf, 0xf, false));
 }
-GS_DEV uint32_t quad_or(uint32_t v) {
+template <int kLanes>
+GS_DEV void group_bcast_all(float v, float (&o)[kLanes]) {
+  o[0] = group_bcast<kLanes, 0>(v);
+  o[1] = group_bcast<kLanes, 1>(v);
+  if constexpr (kLanes == 4) {
+    o[2] = group_bcast<4, 2>(v);
+    o[3] = group_bcast<4, 3>(v);
+  }
+}
+template <int kLanes>
+GS_DEV uint32_t group_or(uint32_t v) {
   v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 177, 0xf, 0xf, false);  // lane ^ 1
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 78, 0xf, 0xf, false);   // lane ^ 2
+  if constexpr (kLanes == 4)
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 78, 0xf, 0xf, false);  // lane ^ 2
   return v;
 }
 
-// Heavy-unit slabs with four lanes per pixel (tile edges 8 and 16). The
-// block covers kSlabRows rows of the tile starting at row slab * kSlabRows.
-// Records are staged kBatch per round, kBatch / kThreads per thread, and
-// tested against their support rectangles (clipped to the slab's rows)
-// exactly as in blend_tile_range. The blend of pixel p is shared by lanes
-// 4p..4p+3: per step the quad takes the pixel's next four candidates in
-// record order, lane q evaluates the q-th one (power, exponential and alpha,
-// the expensive part), the quad exchanges the four results (DPP quad
-// broadcasts) and every lane applies the same four blends in the same
-// order, so the four lanes hold the same sum, bit for bit the one-lane
-// result. Next nonzero mask words come from a per-pixel summary word, so no
+// Heavy-unit slabs with kLanes (2 or 4) lanes per pixel (tile edges 8 and
+// 16). The block covers kSlabRows rows of the tile starting at row
+// slab * kSlabRows. Records are staged kBatch per round, kBatch / kThreads
+// per thread, and tested against their support rectangles (clipped to the
+// slab's rows) exactly as in blend_tile_range. The blend of pixel p is
+// shared by lanes kLanes p .. kLanes p + kLanes - 1: per step the group
+// takes the pixel's next kLanes candidates in record order, lane q
+// evaluates the q-th one (power, exponential and alpha, the expensive part),
+// the group exchanges the results (DPP broadcasts) and every lane applies
+// the same blends in the same order, so the lanes hold the same sum, bit
+// for bit the one-lane result. Next nonzero mask words come from a per-pixel summary word, so no
 // lane scans empty words. The record ordinals of round r + 2 are loaded
 // while round r runs, the records of round r + 1 gathered. Needs cb < ce.
-template <int kEdge, int kSlabRows, int kBatch, bool kRecord>
-GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t slab,
-                                 uint32_t cb, uint32_t ce, const uint2* __restrict__ sorted,
-                                 const Splat* __restrict__ splat,
-                                 RasterLds<kEdge * kSlabRows, kBatch>& L,
-                                 uint32_t& x, uint32_t& y, bool& active,
-                                 uint32_t* __restrict__ record) {
+template <int kEdge, int kSlabRows, int kLanes, int kBatch, bool kRecord>
+GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t slab,
+                                  uint32_t cb, uint32_t ce, const uint2* __restrict__ sorted,
+                                  const Splat* __restrict__ splat,
+                                  RasterLds<kEdge * kSlabRows, kBatch>& L,
+                                  uint32_t& x, uint32_t& y, bool& active,
+                                  uint32_t* __restrict__ record) {
+  static_assert(kLanes == 2 || kLanes == 4, "pairs or quads of lanes per pixel");
   constexpr int kPixels = kEdge * kSlabRows;
-  constexpr int kThreads = 4 * kPixels;
+  constexpr int kThreads = kLanes * kPixels;
   constexpr int kPer = kBatch / kThreads;  // records staged per thread per round
   constexpr int kMaskWords = kBatch / 32;
   static_assert(kBatch % kThreads == 0, "whole records per thread");
   static_assert(kMaskWords <= 32, "the summary word covers 32 mask words");
-  const uint32_t t = threadIdx.x, q = t & 3u, pix = t >> 2;
+  const uint32_t t = threadIdx.x, q = t & (uint32_t)(kLanes - 1), pix = t >> (kLanes / 2);
   int tx0 = (int)((tile % fp.tiles_x) * kEdge);
   int sy0 = (int)((tile / fp.tiles_x) * kEdge + slab * kSlabRows);  // first row of the slab
   x = (uint32_t)tx0 + (pix & (uint32_t)(kEdge - 1));
@@ -1438,19 +1461,19 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
       if (hit[e])
         mark_candidates<kEdge>(L, e * kThreads + t, crc[e], tx0, sy0, cm[e], cc[e], vw, vh, axis);
     __syncthreads();
-    if (!done) {  // uniform over the quad
+    if (!done) {  // uniform over the group
       uint32_t sum = 0u;
-      for (uint32_t w = q; w < (uint32_t)kMaskWords; w += 4u)
+      for (uint32_t w = q; w < (uint32_t)kMaskWords; w += (uint32_t)kLanes)
         if (L.mask[w][pix] != 0u) sum |= 1u << w;
-      sum = quad_or(sum);
+      sum = group_or<kLanes>(sum);
       uint32_t n = min((uint32_t)kBatch, ce - batch);
       uint32_t stop = n;
       uint32_t w = 0u, bits = 0u;
       while (true) {
-        uint32_t j[4];
-        bool v[4];
+        uint32_t j[kLanes];
+        bool v[kLanes];
 #pragma unroll
-        for (int qq = 0; qq < 4; qq++) {
+        for (int qq = 0; qq < kLanes; qq++) {
           if (bits == 0u && sum != 0u) {
             w = (uint32_t)__builtin_ctz(sum);
             sum &= sum - 1u;
@@ -1461,8 +1484,13 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
           bits &= bits - 1u;
         }
         if (!v[0]) break;
-        uint32_t mj = q == 0u ? j[0] : (q == 1u ? j[1] : (q == 2u ? j[2] : j[3]));
-        bool mv = q == 0u ? v[0] : (q == 1u ? v[1] : (q == 2u ? v[2] : v[3]));
+        uint32_t mj = j[0];
+        bool mv = v[0];
+#pragma unroll
+        for (int qq = 1; qq < kLanes; qq++) {
+          mj = q == (uint32_t)qq ? j[qq] : mj;
+          mv = q == (uint32_t)qq ? v[qq] : mv;
+        }
         float a = 0.0f, cr = 0.0f, cgr = 0.0f, cbl = 0.0f, dep = 0.0f;
         if (mv) {
           Candidate cd = eval_candidate(L, mj, pxc, pyc, vw, vh, axis);
@@ -1472,16 +1500,14 @@ GS_DEV PixelSum blend_heavy_quad(const FrameParams& fp, uint32_t tile, uint32_t 
           cbl = cd.color.z;
           dep = cd.depth;
         }
-        float qa[4] = {quad_bcast<0>(a), quad_bcast<1>(a), quad_bcast<2>(a), quad_bcast<3>(a)};
-        float qr[4] = {quad_bcast<0>(cr), quad_bcast<1>(cr), quad_bcast<2>(cr), quad_bcast<3>(cr)};
-        float qg[4] = {quad_bcast<0>(cgr), quad_bcast<1>(cgr), quad_bcast<2>(cgr),
-                       quad_bcast<3>(cgr)};
-        float qb[4] = {quad_bcast<0>(cbl), quad_bcast<1>(cbl), quad_bcast<2>(cbl),
-                       quad_bcast<3>(cbl)};
-        float qd[4] = {quad_bcast<0>(dep), quad_bcast<1>(dep), quad_bcast<2>(dep),
-                       quad_bcast<3>(dep)};
+        float qa[kLanes], qr[kLanes], qg[kLanes], qb[kLanes], qd[kLanes];
+        group_bcast_all<kLanes>(a, qa);
+        group_bcast_all<kLanes>(cr, qr);
+        group_bcast_all<kLanes>(cgr, qg);
+        group_bcast_all<kLanes>(cbl, qb);
+        group_bcast_all<kLanes>(dep, qd);
 #pragma unroll
-        for (int qq = 0; qq < 4; qq++) {
+        for (int qq = 0; qq < kLanes; qq++) {
           if (!v[qq] || done) continue;
           float alpha = qa[qq];
           if (alpha < (float)(1.0 / 255.0)) continue;
@@ -1715,7 +1741,7 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
     uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
     uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
-  constexpr bool kQuad = kEdge != 32;
+  constexpr bool kGroup = kEdge != 32;  // lane groups per pixel (edges 8, 16)
   constexpr uint32_t kSlabs = heavy_slabs(kEdge, kSplit);
   constexpr int kBatch = heavy_batch(kEdge, kSplit);
   __shared__ RasterLds<heavy_slab_pixels(kEdge, kSplit), kBatch> lds;
@@ -1765,11 +1791,12 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
         PixelSum s;
         uint32_t pix = threadIdx.x;  // pixel within the tile
         bool writer = threadIdx.x < (uint32_t)kPixels;
-        if constexpr (kQuad) {
-          s = blend_heavy_quad<kEdge, heavy_slab_rows(kEdge, kSplit), kBatch, kRecord>(
+        if constexpr (kGroup) {
+          constexpr int kLanes = heavy_lanes(kEdge, kSplit);
+          s = blend_heavy_group<kEdge, heavy_slab_rows(kEdge, kSplit), kLanes, kBatch, kRecord>(
               fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
-          pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> 2);
-          writer = (threadIdx.x & 3u) == 0u;
+          pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> (kLanes / 2));
+          writer = (threadIdx.x & (uint32_t)(kLanes - 1)) == 0u;
         } else {
           s = blend_tile_range<kEdge, kHeavyThreads, kBatch, kRecord>(
               fp, tile, cb, ce, sorted, splat, lds, x, y, active, record);
@@ -1796,7 +1823,7 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
 
 // Persistent grid of k_raster_heavy, a multiple of 8 blocks (at least 8):
 // whole-tile blocks (16 waves) one per CU, leaving room for the light-tile
-// kernel beside them; row-slab blocks (4 waves) five per CU, the waves per
+// kernel beside them; row-slab blocks (2 waves) ten per CU, the waves per
 // CU their registers allow (measured faster than leaving the light kernel
 // room: the slab units outnumber the CUs and gain most from occupancy).
 template <int kEdge, bool kSplit>
