@@ -765,9 +765,11 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   // Heavy units run as row slabs above 1.75 units per CU: below that, a
   // whole-tile block per unit already covers the CUs and slabs mostly
   // repeat the per-record staging. Measured on this GPU: the 1M/1080p frame
-  // (275 units) is 30% faster whole, mixed-resolution views of 567-574
-  // units 2-5% faster whole, dense culled 1080p views of 510-850 units
-  // 9-20% faster as slabs.
+  // (275 units) rasterizes its heavy units in 0.25 ms whole, 0.49 ms as
+  // slabs; batched culled 3M/1080p views run
+  // 1,264 frames/s with this threshold, 1,147 at 2.5 units per CU and 998
+  // at 3 (the 2M mixed-resolution batch gains 2.6% at 2.5, plain 3M views
+  // of 906-1,907 units do not move).
   p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 7 / 4
                                                                : ctx->cfg.heavy_split_units;
   // Training-set frames: chunking off, so every recorded alpha * T uses the
