@@ -1412,6 +1412,10 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
   uint32_t axis = fp.boundary_axis;
   float threshold = fp.transmittance_threshold;
   PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
+  constexpr int kChannels = 4 / kLanes;  // of r, g, b, depth, accumulated by this lane
+  float acc[kChannels];
+#pragma unroll
+  for (int h = 0; h < kChannels; h++) acc[h] = 0.0f;
   bool done = !active;
   // Per staged record: its ordinal two rounds ahead, its splat one round
   // ahead. Slots past the unit load the unit's last record (and are never
@@ -1447,7 +1451,7 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
       if (hit[e]) {
         L.mean[slot] = m[e];
         L.conic[slot] = c[e];
-        L.color[slot] = k[e];
+        L.color[slot] = make_float4(k[e].x, k[e].y, k[e].z, m[e].z);  // r, g, b, depth
         if (kRecord) L.gid[slot] = cg[e];
       }
       cm[e] = m[e];
@@ -1491,21 +1495,23 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
           mj = q == (uint32_t)qq ? j[qq] : mj;
           mv = q == (uint32_t)qq ? v[qq] : mv;
         }
-        float a = 0.0f, cr = 0.0f, cgr = 0.0f, cbl = 0.0f, dep = 0.0f;
-        if (mv) {
-          Candidate cd = eval_candidate(L, mj, pxc, pyc, vw, vh, axis);
-          a = cd.alpha;
-          cr = cd.color.x;
-          cgr = cd.color.y;
-          cbl = cd.color.z;
-          dep = cd.depth;
+        // This lane's channels of the group's candidates (indices of absent
+        // candidates stay inside the round; their values are not used).
+        float ch[kLanes][kChannels];
+#pragma unroll
+        for (int qq = 0; qq < kLanes; qq++) {
+          const float* src = reinterpret_cast<const float*>(&L.color[j[qq]]) + q * kChannels;
+#pragma unroll
+          for (int h = 0; h < kChannels; h++) ch[qq][h] = src[h];
         }
-        float qa[kLanes], qr[kLanes], qg[kLanes], qb[kLanes], qd[kLanes];
+        float a = 0.0f;
+        if (mv) {
+          float4 gm = L.mean[mj];
+          float power = pair_power(pxc, pyc, gm, L.conic[mj], vw, vh, axis);
+          a = gs_clampf(gm.w * gs_expf(power), 0.0f, 0.99f);
+        }
+        float qa[kLanes];
         group_bcast_all<kLanes>(a, qa);
-        group_bcast_all<kLanes>(cr, qr);
-        group_bcast_all<kLanes>(cgr, qg);
-        group_bcast_all<kLanes>(cbl, qb);
-        group_bcast_all<kLanes>(dep, qd);
 #pragma unroll
         for (int qq = 0; qq < kLanes; qq++) {
           if (!v[qq] || done) continue;
@@ -1513,10 +1519,8 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
           if (alpha < (float)(1.0 / 255.0)) continue;
           float contribution = s.t * alpha;
           if (kRecord && q == (uint32_t)qq) atomicMax(&record[L.gid[j[qq]]], gs_f2b(contribution));
-          s.r = s.r + qr[qq] * contribution;
-          s.g = s.g + qg[qq] * contribution;
-          s.b = s.b + qb[qq] * contribution;
-          s.d = s.d + qd[qq] * contribution;
+#pragma unroll
+          for (int h = 0; h < kChannels; h++) acc[h] = acc[h] + ch[qq][h] * contribution;
           s.blended++;
           s.t = s.t * (1.0f - alpha);
           if (s.t < threshold) {
@@ -1529,6 +1533,18 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
       s.checked += stop;  // pairs the reference's loop evaluates
     }
     if (__syncthreads_and(done)) break;
+  }
+  // Lane 0 of each group collects the channels of the other lanes.
+  if constexpr (kLanes == 4) {
+    s.r = acc[0];
+    s.g = group_bcast<4, 1>(acc[0]);
+    s.b = group_bcast<4, 2>(acc[0]);
+    s.d = group_bcast<4, 3>(acc[0]);
+  } else {
+    s.r = acc[0];
+    s.g = acc[1];
+    s.b = group_bcast<2, 1>(acc[0]);
+    s.d = group_bcast<2, 1>(acc[1]);
   }
   return s;
 }
