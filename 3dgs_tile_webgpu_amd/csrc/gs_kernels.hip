@@ -1218,14 +1218,53 @@ GS_DEV void gather_splat(const Splat* __restrict__ splat, uint32_t id, float4& m
   k = g.color;
 }
 
+// The candidate rows of mark_candidates under the plain boundary policy:
+// pair_power with its row terms hoisted, the same operations in the same
+// order; pixel centres step by 1.0f exactly (small integers + 0.5). With
+// kBranchFree every tested pixel takes an LDS atomic (an OR of 0 when it
+// fails), which saves the branch around it but costs LDS work per failing
+// pixel.
+template <int kEdge, bool kBranchFree, class Lds>
+GS_DEV void mark_rows(Lds& L, uint32_t bit, uint32_t word, const Rect& rc, int tx0, int ry0,
+                      float4 m, float4 c) {
+  const float cy2 = c.y * 2.0f;
+  for (int py = rc.y0; py <= rc.y1; py++) {
+    const float dy = ((float)py + 0.5f) - m.y;
+    const float qy = c.z * (dy * dy);
+    uint32_t* row = &L.mask[word][(uint32_t)((py - ry0) * kEdge)];
+    float pxf = (float)rc.x0 + 0.5f;
+    for (int i = rc.x0 - tx0; i <= rc.x1 - tx0; i++, pxf += 1.0f) {  // column in the tile
+      const float dx = pxf - m.x;
+      const float power = (c.x * (dx * dx) + cy2 * dx * dy + qy) * -0.5f;
+      const bool fails = power > 0.0f || power < -c.w;
+      if (kBranchFree)
+        atomicOr(&row[i], fails ? 0u : bit);
+      else if (!fails)
+        atomicOr(&row[i], bit);
+    }
+  }
+}
+
 // Set the candidate bits of the record in round slot `slot` for the pixels
 // of rc (absolute pixel bounds, inside the block's rows) that pass the
 // reference's test (TileRasterizer.ts:624-656); pixel p of the block is
-// (px - tx0) + (py - ry0) * E.
+// (px - tx0) + (py - ry0) * E. Under the plain policy a wave picks the
+// atomic form by its records' shape: an ellipse fills pi/4 sqrt(1 - r^2) of
+// its bounding rectangle (r^2 = c1^2 / (c0 c2)), so a wave holding a thin
+// tilted one (r^2 > 0.9, filling under a quarter) keeps the branch and the
+// rest go branch-free. Both forms set the same bits.
 template <int kEdge, class Lds>
 GS_DEV void mark_candidates(Lds& L, uint32_t slot, const Rect& rc, int tx0, int ry0, float4 m,
                             float4 c, float vw, float vh, uint32_t axis) {
   uint32_t bit = 1u << (slot & 31u), word = slot >> 5;
+  if (axis == 0) {
+    const bool thin = c.y * c.y > 0.9f * (c.x * c.z);
+    if (__builtin_amdgcn_ballot_w64(thin) != 0u)  // uniform over the wave
+      mark_rows<kEdge, false>(L, bit, word, rc, tx0, ry0, m, c);
+    else
+      mark_rows<kEdge, true>(L, bit, word, rc, tx0, ry0, m, c);
+    return;
+  }
   for (int py = rc.y0; py <= rc.y1; py++) {
     float pyf = (float)py + 0.5f;
     for (int px = rc.x0; px <= rc.x1; px++) {
