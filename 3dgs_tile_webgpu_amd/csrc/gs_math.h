@@ -113,6 +113,16 @@ GS_INLINE float gs_expf(float x) {
   return gs_expf_core(x);
 }
 
+// gs_expf for arguments in [-103.278929903431851103, 88.72283905206835] or
+// NaN, without the range branches: the same bits there. The rasterizer's
+// candidate powers lie in [-ln(255 a), 0], a the opacity (at most 65504 in
+// binary16), so inside.
+GS_INLINE float gs_expf_inrange(float x) {
+  const bool nan = x != x;
+  const float y = gs_expf_core(nan ? 0.0f : x);
+  return nan ? x : y;
+}
+
 // Cephes logf.
 GS_INLINE float gs_logf(float x) {
   if (x != x) return x;

@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-EXP, LOG, SIN, COS, ATAN, ATAN2, H2F, F2H, LOGPOS = range(9)
+EXP, LOG, SIN, COS, ATAN, ATAN2, H2F, F2H, LOGPOS, EXP_INRANGE = range(10)
 
 
 @pytest.fixture(scope="module")
@@ -59,6 +59,20 @@ def test_log_positive_normal(mathfn):
     assert ulps(got, mathfn(LOG, x).astype(np.float64)).max() <= 1.0
     x = np.arange(1, 512, dtype=np.float32)  # pixel radii
     assert ulps(mathfn(LOGPOS, x), np.log(x.astype(np.float64))).max() <= 1.0
+
+
+def test_exp_without_range_branches(mathfn):
+    """The rasterizer's exp (no overflow/underflow branches) gives gs_expf's
+    bits on its whole domain: every candidate power in [-17, 0] on a fine
+    grid, random arguments over the full range, its end points and NaN."""
+    lo, hi = np.float32(-103.278929903431851103), np.float32(88.72283905206835)
+    grid = np.arange(-17.0, 0.0, 1.0 / 4096, dtype=np.float64).astype(np.float32)
+    x = np.concatenate([grid, RNG.uniform(lo, hi, 200_000).astype(np.float32),
+                        np.array([lo, hi, 0.0, -0.0, np.nextafter(lo, 0), np.nextafter(hi, 0)],
+                                 np.float32)])
+    np.testing.assert_array_equal(mathfn(EXP_INRANGE, x).view(np.uint32),
+                                  mathfn(EXP, x).view(np.uint32))
+    assert np.isnan(mathfn(EXP_INRANGE, np.array([np.nan], np.float32))).all()
 
 
 @pytest.mark.parametrize("fn,ref", [(SIN, np.sin), (COS, np.cos)])
