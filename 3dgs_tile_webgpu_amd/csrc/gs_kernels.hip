@@ -1426,7 +1426,7 @@ GS_DEV uint32_t group_or(uint32_t v) {
 // for bit the one-lane result. Next nonzero mask words come from a per-pixel summary word, so no
 // lane scans empty words. The record ordinals of round r + 2 are loaded
 // while round r runs, the records of round r + 1 gathered. Needs cb < ce.
-template <int kEdge, int kSlabRows, int kLanes, int kBatch, bool kRecord>
+template <int kEdge, int kSlabRows, int kLanes, int kBatch, bool kRecord, bool kPeriodic>
 GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t slab,
                                   uint32_t cb, uint32_t ce, const uint2* __restrict__ sorted,
                                   const Splat* __restrict__ splat,
@@ -1448,7 +1448,8 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
   active = x < fp.width && y < fp.height;
   float pxc = (float)x + 0.5f, pyc = (float)y + 0.5f;
   float vw = fp.viewport[0], vh = fp.viewport[1];
-  uint32_t axis = fp.boundary_axis;
+  // The plain policy's instance carries no boundary arithmetic.
+  const uint32_t axis = kPeriodic ? fp.boundary_axis : 0u;
   float threshold = fp.transmittance_threshold;
   PixelSum s = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0u, 0u};
   constexpr int kChannels = 4 / kLanes;  // of r, g, b, depth, accumulated by this lane
@@ -1848,8 +1849,13 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
         bool writer = threadIdx.x < (uint32_t)kPixels;
         if constexpr (kGroup) {
           constexpr int kLanes = heavy_lanes(kEdge, kSplit);
-          s = blend_heavy_group<kEdge, heavy_slab_rows(kEdge, kSplit), kLanes, kBatch, kRecord>(
-              fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
+          constexpr int kRows = heavy_slab_rows(kEdge, kSplit);
+          if (fp.boundary_axis == 0u)
+            s = blend_heavy_group<kEdge, kRows, kLanes, kBatch, kRecord, false>(
+                fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
+          else
+            s = blend_heavy_group<kEdge, kRows, kLanes, kBatch, kRecord, true>(
+                fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
           pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> (kLanes / 2));
           writer = (threadIdx.x & (uint32_t)(kLanes - 1)) == 0u;
         } else {
