@@ -1181,7 +1181,7 @@ GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, 
   float4 gc = L.conic[j];
   float power = pair_power(pxc, pyc, gm, gc, vw, vh, axis);
   Candidate c;
-  c.alpha = gs_clampf(gm.w * gs_expf_inrange(power), 0.0f, 0.99f);
+  c.alpha = gs_clampf(gm.w * gs_expf_nonpos(power), 0.0f, 0.99f);
   c.depth = gm.z;
   c.color = L.color[j];
   return c;
@@ -1548,7 +1548,7 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
         if (mv) {
           float4 gm = L.mean[mj];
           float power = pair_power(pxc, pyc, gm, L.conic[mj], vw, vh, axis);
-          a = gs_clampf(gm.w * gs_expf_inrange(power), 0.0f, 0.99f);
+          a = gs_clampf(gm.w * gs_expf_nonpos(power), 0.0f, 0.99f);
         }
         float qa[kLanes];
         group_bcast_all<kLanes>(a, qa);

@@ -89,12 +89,12 @@ GS_INLINE float gs_exp2i(int e) {
   return gs_b2f(1u << (e + 149));
 }
 
-// Cephes expf core for x in [-103.3, 88.7]; no special-value handling.
-GS_INLINE float gs_expf_core(float x) {
+// Cephes expf reduction and polynomial: e^x = y 2^n.
+GS_INLINE float gs_expf_reduced(float x, int& n) {
   float z = gs_floorf(1.44269504088896341f * x + 0.5f);
   x = x - z * 0.693359375f;
   x = x - z * -2.12194440e-4f;
-  int n = (int)z;
+  n = (int)z;
   z = x * x;
   float y = 1.9875691500E-4f;
   y = y * x + 1.3981999507E-3f;
@@ -103,6 +103,13 @@ GS_INLINE float gs_expf_core(float x) {
   y = y * x + 1.6666665459E-1f;
   y = y * x + 5.0000001201E-1f;
   y = y * z + x + 1.0f;
+  return y;
+}
+
+// Cephes expf core for x in [-103.3, 88.7]; no special-value handling.
+GS_INLINE float gs_expf_core(float x) {
+  int n;
+  const float y = gs_expf_reduced(x, n);
   return gs_ldexpf(y, n);
 }
 
@@ -113,14 +120,15 @@ GS_INLINE float gs_expf(float x) {
   return gs_expf_core(x);
 }
 
-// gs_expf for arguments in [-103.278929903431851103, 88.72283905206835] or
-// NaN, without the range branches: the same bits there. The rasterizer's
-// candidate powers lie in [-ln(255 a), 0], a the opacity (at most 65504 in
-// binary16), so inside.
-GS_INLINE float gs_expf_inrange(float x) {
+// gs_expf for x in [-87, 0] or NaN, without the range branches: there n is
+// in [-126, 0], 2^n a normal binary32, and y 2^n one product with the same
+// single rounding as gs_ldexpf's two. The rasterizer's candidate powers lie
+// in [-ln(255 a), 0], a the opacity (at most 65504 in binary16), so inside.
+GS_INLINE float gs_expf_nonpos(float x) {
   const bool nan = x != x;
-  const float y = gs_expf_core(nan ? 0.0f : x);
-  return nan ? x : y;
+  int n;
+  const float y = gs_expf_reduced(nan ? 0.0f : x, n);
+  return nan ? x : y * gs_pow2i_normal(n);
 }
 
 // Cephes logf.

@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-EXP, LOG, SIN, COS, ATAN, ATAN2, H2F, F2H, LOGPOS, EXP_INRANGE = range(10)
+EXP, LOG, SIN, COS, ATAN, ATAN2, H2F, F2H, LOGPOS, EXP_NONPOS = range(10)
 
 
 @pytest.fixture(scope="module")
@@ -61,18 +61,21 @@ def test_log_positive_normal(mathfn):
     assert ulps(mathfn(LOGPOS, x), np.log(x.astype(np.float64))).max() <= 1.0
 
 
-def test_exp_without_range_branches(mathfn):
-    """The rasterizer's exp (no overflow/underflow branches) gives gs_expf's
-    bits on its whole domain: every candidate power in [-17, 0] on a fine
-    grid, random arguments over the full range, its end points and NaN."""
-    lo, hi = np.float32(-103.278929903431851103), np.float32(88.72283905206835)
+def test_exp_of_nonpositive_arguments(mathfn):
+    """The rasterizer's exp (no range branches, one scaling product) gives
+    gs_expf's bits on its whole domain [-87, 0]: every candidate power in
+    [-17, 0] on a fine grid, random arguments over the domain, its end points,
+    the arguments nearest each scaling step, and NaN."""
     grid = np.arange(-17.0, 0.0, 1.0 / 4096, dtype=np.float64).astype(np.float32)
-    x = np.concatenate([grid, RNG.uniform(lo, hi, 200_000).astype(np.float32),
-                        np.array([lo, hi, 0.0, -0.0, np.nextafter(lo, 0), np.nextafter(hi, 0)],
-                                 np.float32)])
-    np.testing.assert_array_equal(mathfn(EXP_INRANGE, x).view(np.uint32),
+    steps = (np.arange(-126, 1) - 0.5) * np.log(2.0)  # where n = floor(x log2 e + 1/2) steps
+    near = np.concatenate([np.nextafter(steps.astype(np.float32), -np.inf),
+                           steps.astype(np.float32), np.nextafter(steps.astype(np.float32), np.inf)])
+    near = near[(near >= -87.0) & (near <= 0.0)]
+    x = np.concatenate([grid, RNG.uniform(-87.0, 0.0, 200_000).astype(np.float32), near,
+                        np.array([-87.0, 0.0, -0.0, np.finfo(np.float32).tiny * -1], np.float32)])
+    np.testing.assert_array_equal(mathfn(EXP_NONPOS, x).view(np.uint32),
                                   mathfn(EXP, x).view(np.uint32))
-    assert np.isnan(mathfn(EXP_INRANGE, np.array([np.nan], np.float32))).all()
+    assert np.isnan(mathfn(EXP_NONPOS, np.array([np.nan], np.float32))).all()
 
 
 @pytest.mark.parametrize("fn,ref", [(SIN, np.sin), (COS, np.cos)])
