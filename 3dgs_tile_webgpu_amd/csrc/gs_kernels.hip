@@ -197,7 +197,111 @@ GS_DEV ProjGeom project_geometry(const FrameParams& fp, const SceneView& sc, uin
 
 // The exact tile count, hit record and SH colour of a candidate
 // (ProjectionStage.ts:600-649): visible if it hits a tile.
-GS_DEV ProjOut project_finish(const FrameParams& fp, const SceneView& sc, const ProjGeom& g) {
+// A visible Gaussian's bounding box in tiles (ProjectionStage.ts:590-618).
+struct TileBox {
+  int tmin_x, tmin_y, tmax_x, tmax_y;
+  GS_DEV uint32_t width() const { return (uint32_t)(tmax_x - tmin_x + 1); }
+  GS_DEV uint32_t height() const { return (uint32_t)(tmax_y - tmin_y + 1); }
+};
+constexpr uint32_t kMaskTiles = 32;  // boxes up to this many tiles keep a hit mask
+
+GS_DEV TileBox tile_box(const FrameParams& fp, const ProjGeom& g) {
+  float bminx = g.cx - g.rx, bminy = g.cy - g.ry, bmaxx = g.cx + g.rx, bmaxy = g.cy + g.ry;
+  float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
+  // x / ts as x * (1 / ts): the tile edge is a power of two, so both are the
+  // exact quotient rounded once, the same binary32 value.
+  float inv_ts = 1.0f / (float)fp.tile_size;
+  TileBox b;
+  b.tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx * inv_ts), 0.0f, max_tx));
+  b.tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy * inv_ts), 0.0f, max_ty));
+  b.tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx * inv_ts), 0.0f, max_tx));
+  b.tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy * inv_ts), 0.0f, max_ty));
+  if (fp.boundary_axis == 1) {
+    b.tmin_x = 0;
+    b.tmax_x = (int)fp.tiles_x - 1;
+  }
+  if (fp.boundary_axis == 2) {
+    b.tmin_y = 0;
+    b.tmax_y = (int)fp.tiles_y - 1;
+  }
+  return b;
+}
+
+// The box's tiles passing the exact tile test, in one thread; `mask` gets
+// the row-major hit bits of a box of at most kMaskTiles tiles.
+GS_DEV uint32_t count_box(const FrameParams& fp, const ProjGeom& g, const TileBox& b,
+                          uint32_t& mask) {
+  const float conic[3] = {g.conic[0], g.conic[1], g.conic[2]};
+  const float vw = fp.viewport[0], vh = fp.viewport[1], ts = (float)fp.tile_size;
+  const bool small = b.width() * b.height() <= kMaskTiles;
+  uint32_t count = 0u, bitpos = 0u;
+  mask = 0u;
+  for (int ty = b.tmin_y; ty <= b.tmax_y; ty++)
+    for (int tx = b.tmin_x; tx <= b.tmax_x; tx++, bitpos++)
+      if (tile_hit(g.cx, g.cy, conic, g.thr, g.rx, g.ry, tx, ty, vw, vh, fp.boundary_axis,
+                   ts)) {
+        count++;
+        if (small) mask |= 1u << bitpos;
+      }
+  return count;
+}
+
+// Tile i of a box `bw` tiles wide in row-major order: row = i / bw through
+// a binary32 reciprocal (exact for the quotients of tile grids), then one
+// integer correction either way.
+GS_DEV void box_tile(uint32_t i, uint32_t bw, float inv_bw, uint32_t& row, uint32_t& col) {
+  int r = (int)(((float)i + 0.5f) * inv_bw);
+  int c = (int)i - r * (int)bw;
+  if (c < 0) {
+    r--;
+    c += (int)bw;
+  } else if (c >= (int)bw) {
+    r++;
+    c -= (int)bw;
+  }
+  row = (uint32_t)r;
+  col = (uint32_t)c;
+}
+
+// Tile counts of the wave's boxes above kMaskTiles tiles (flagged `big`),
+// one box at a time over all 64 lanes: a thread walking a near-plane
+// Gaussian's thousands of tiles alone would hold its wave for all of them.
+// The count does not depend on the order the tiles are tested in. Needs
+// the whole wave.
+GS_DEV void count_big_boxes(const FrameParams& fp, bool big, const ProjGeom& g,
+                            const TileBox& b, uint32_t& count) {
+  const uint32_t lane = lane_id();
+  const float vw = fp.viewport[0], vh = fp.viewport[1], ts = (float)fp.tile_size;
+  for (uint64_t m = __ballot(big); m != 0ull; m &= m - 1ull) {
+    const int src = __builtin_ctzll(m);
+    const float cx = __shfl(g.cx, src, 64), cy = __shfl(g.cy, src, 64);
+    const float conic[3] = {__shfl(g.conic[0], src, 64), __shfl(g.conic[1], src, 64),
+                            __shfl(g.conic[2], src, 64)};
+    const float thr = __shfl(g.thr, src, 64), rx = __shfl(g.rx, src, 64),
+                ry = __shfl(g.ry, src, 64);
+    const int tx0 = __shfl(b.tmin_x, src, 64), ty0 = __shfl(b.tmin_y, src, 64);
+    const uint32_t bw = (uint32_t)__shfl((int)b.width(), src, 64);
+    const uint32_t n = bw * (uint32_t)__shfl((int)b.height(), src, 64);
+    const float inv_bw = 1.0f / (float)bw;
+    uint32_t c = 0u;
+    for (uint32_t i = lane; i < n; i += 64u) {
+      uint32_t row, col;
+      box_tile(i, bw, inv_bw, row, col);
+      if (tile_hit(cx, cy, conic, thr, rx, ry, tx0 + (int)col, ty0 + (int)row, vw, vh,
+                   fp.boundary_axis, ts))
+        c++;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += (uint32_t)__shfl_xor((int)c, d, 64);
+    if ((int)lane == src) count = c;
+  }
+}
+
+// The rest of ProjectionStage.ts:385-649 once the box's tiles are counted:
+// the hit record and the SH colour.
+GS_DEV ProjOut project_finish_counted(const FrameParams& fp, const SceneView& sc,
+                                      const ProjGeom& g, const TileBox& box, uint32_t count,
+                                      uint32_t mask) {
   ProjOut o;
   o.kind = kProjNone;
   o.count = 0u;
@@ -210,38 +314,10 @@ GS_DEV ProjOut project_finish(const FrameParams& fp, const SceneView& sc, const 
   const uint32_t slot = g.slot;
   const float thr = g.thr, rx = g.rx, ry = g.ry;
   const float conic[3] = {g.conic[0], g.conic[1], g.conic[2]};
-  float bminx = g.cx - rx, bminy = g.cy - ry, bmaxx = g.cx + rx, bmaxy = g.cy + ry;
-  float vw = fp.viewport[0], vh = fp.viewport[1];
-  uint32_t axis = fp.boundary_axis;
-  float max_tx = (float)((int)fp.tiles_x - 1), max_ty = (float)((int)fp.tiles_y - 1);
-  float ts = (float)fp.tile_size;
-  // x / ts as x * (1 / ts): the tile edge is a power of two, so both are the
-  // exact quotient rounded once, the same binary32 value.
-  float inv_ts = 1.0f / ts;
-  int tmin_x = gs_f2i(gs_clampf(gs_floorf(bminx * inv_ts), 0.0f, max_tx));
-  int tmin_y = gs_f2i(gs_clampf(gs_floorf(bminy * inv_ts), 0.0f, max_ty));
-  int tmax_x = gs_f2i(gs_clampf(gs_floorf(bmaxx * inv_ts), 0.0f, max_tx));
-  int tmax_y = gs_f2i(gs_clampf(gs_floorf(bmaxy * inv_ts), 0.0f, max_ty));
-  if (axis == 1) {
-    tmin_x = 0;
-    tmax_x = (int)fp.tiles_x - 1;
-  }
-  if (axis == 2) {
-    tmin_y = 0;
-    tmax_y = (int)fp.tiles_y - 1;
-  }
-  uint32_t count = 0;
-  uint32_t bw = (uint32_t)(tmax_x - tmin_x + 1), bh = (uint32_t)(tmax_y - tmin_y + 1);
-  bool small = bw * bh <= 32u;
-  uint32_t mask = 0u, bitpos = 0u;
-  for (int ty = tmin_y; ty <= tmax_y; ty++)
-    for (int tx = tmin_x; tx <= tmax_x; tx++, bitpos++)
-      if (tile_hit(g.cx, g.cy, conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
-        count++;
-        if (small) mask |= 1u << bitpos;
-      }
   if (count == 0) return o;
-  o.hits = make_uint4((uint32_t)tmin_x | ((uint32_t)tmin_y << 16),
+  const uint32_t bw = box.width(), bh = box.height();
+  const bool small = bw * bh <= kMaskTiles;
+  o.hits = make_uint4((uint32_t)box.tmin_x | ((uint32_t)box.tmin_y << 16),
                       small ? (bw | (bh << 8) | kHitMask) : 0u, mask, count);
   // SH colour (projectionHelpers.ts:118-176), direction = normalized
   // object-space offset from the camera.
@@ -314,6 +390,16 @@ GS_DEV ProjOut project_finish(const FrameParams& fp, const SceneView& sc, const 
   return o;
 }
 
+GS_DEV ProjOut project_finish(const FrameParams& fp, const SceneView& sc, const ProjGeom& g) {
+  TileBox b = {0, 0, -1, -1};
+  uint32_t count = 0u, mask = 0u;
+  if (g.kind == kProjVisible) {
+    b = tile_box(fp, g);
+    count = count_box(fp, g, b, mask);
+  }
+  return project_finish_counted(fp, sc, g, b, count, mask);
+}
+
 // The largest rgb8e8 exponent of the higher SH bands, minus the bias: the
 // log2 scale of the strongest view-dependent colour term (a learned-cull
 // feature; -127 without higher bands).
@@ -336,13 +422,26 @@ GS_DEV float sh_top_exponent(const FrameParams& fp, const SceneView& sc, uint32_
   return (float)(int)top - 127.0f;
 }
 
-// ProjectionStage.ts:385-649 for active ordinal gid. With kMode != plain it
-// also fills the 16 cull features of a visible Gaussian.
+// ProjectionStage.ts:385-649 for active ordinal gid (live lanes; every
+// lane of the wave must call it). Boxes above kMaskTiles tiles are counted
+// by the whole wave. With kMode != plain it also fills the 16 cull features
+// of a visible Gaussian.
 template <int kMode>
 GS_DEV ProjOut project_gaussian(const FrameParams& fp, const SceneView& sc, uint32_t gid,
-                                float feat[kCullIn]) {
-  ProjGeom g = project_geometry(fp, sc, gid);
-  ProjOut o = project_finish(fp, sc, g);
+                                bool live, float feat[kCullIn]) {
+  ProjGeom g;
+  g.kind = kProjNone;
+  if (live) g = project_geometry(fp, sc, gid);
+  TileBox b = {0, 0, -1, -1};
+  uint32_t count = 0u, mask = 0u;
+  bool big = false;
+  if (g.kind == kProjVisible) {
+    b = tile_box(fp, g);
+    big = b.width() * b.height() > kMaskTiles;
+    if (!big) count = count_box(fp, g, b, mask);
+  }
+  count_big_boxes(fp, big, g, b, count);
+  ProjOut o = project_finish_counted(fp, sc, g, b, count, mask);
   if (kMode != kModePlain && o.kind == kProjVisible)
     cull_features(feat, g.rx, g.ry, g.conic, g.opacity, g.depth, g.dir, g.local,
                   sh_top_exponent(fp, sc, g.slot), g.cx, g.cy, fp.inv_viewport[0],
@@ -382,9 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
     state->checked_heavy = 0ull;
     state->sort_error = 0u;
   }
-  if (gid >= fp.active_count) return;
+  const bool live = gid < fp.active_count;
+  if (__ballot(live) == 0ull) return;  // wave-uniform
   float feat[kCullIn];
-  ProjOut o = project_gaussian<kMode>(fp, sc, gid, feat);
+  ProjOut o = project_gaussian<kMode>(fp, sc, gid, live, feat);
+  if (!live) return;
   if constexpr (kMode == kModeCollect) {
     record[gid] = 0u;
     if (o.kind == kProjVisible)
@@ -941,6 +1042,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
 
 // intersections.ts:43-111: per depth rank, row-major bbox tiles, the same
 // exact tile test as the projection; sentinel (T, id) pads reserved slots.
+// Boxes without a hit mask (above kMaskTiles tiles) are emitted by the
+// whole wave, one box at a time in chunks of 64 tiles: lane l tests the
+// chunk's l-th tile and its key goes after the hits of the earlier tiles
+// (a ballot prefix), which is the row-major order of the serial walk.
 __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
                                                  const DeviceState* __restrict__ state,
                                                  const uint2* __restrict__ sorted_a,
@@ -950,16 +1055,22 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
                                                  const uint32_t* __restrict__ offsets,
                                                  uint2* __restrict__ records) {
   uint32_t rank = blockIdx.x * kBlock + threadIdx.x;
-  if (state->overflow || rank >= state->visible) return;
+  if (state->overflow) return;  // uniform
+  const bool live = rank < state->visible;
+  if (__ballot(live) == 0ull) return;  // wave-uniform
   uint32_t capacity = fp.capacity;
   // The depth sort's last output, or its last input if that pass was skipped.
   const uint2* sorted_depth = state->depth_in_place ? sorted_b : sorted_a;
-  uint32_t gid = sorted_depth[rank].y;
-  uint4 h = ordered_hits[rank];
-  uint32_t reserved = h.w;
-  uint32_t base = offsets[rank];
-  uint32_t local = 0;
-  if (h.y & kHitMask) {
+  uint32_t gid = 0u, reserved = 0u, base = 0u, local = 0u;
+  uint4 h = make_uint4(0u, 0u, 0u, 0u);
+  if (live) {
+    gid = sorted_depth[rank].y;
+    h = ordered_hits[rank];
+    reserved = h.w;
+    base = offsets[rank];
+  }
+  const bool masked = (h.y & kHitMask) != 0u;
+  if (live && masked) {
     // The projection's hits, in the same row-major order.
     uint32_t tx0 = h.x & 0xffffu, ty0 = h.x >> 16, bw = h.y & 0xffu;
     for (uint32_t m = h.z; m != 0u; m &= m - 1u, local++) {
@@ -967,37 +1078,77 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
       uint32_t tile = (ty0 + bit / bw) * fp.tiles_x + tx0 + bit % bw;
       if (base + local < capacity) records[base + local] = make_uint2(tile, gid);
     }
-  } else {
+  }
+  // Unmasked boxes: the tile footprint again from the projected mean and
+  // conic, emitted by the whole wave.
+  const bool big = live && !masked && reserved != 0u;
+  float cx = 0.0f, cy = 0.0f, thr = 0.0f, rx = 0.0f, ry = 0.0f;
+  float conic[3] = {0.0f, 0.0f, 0.0f};
+  int tmin_x = 0, tmin_y = 0, tmax_x = -1, tmax_y = -1;
+  if (big) {
     float4 mean = splat[gid].mean;
     float4 conic4 = splat[gid].conic;
     uint32_t packed = gs_f2u(conic4.w);
-    float rx = (float)(packed % 512u), ry = (float)(packed / 512u);
-    float conic[3] = {conic4.x, conic4.y, conic4.z};
-    float thr = gs_logf(mean.w * 255.0f);
+    rx = (float)(packed % 512u);
+    ry = (float)(packed / 512u);
+    conic[0] = conic4.x;
+    conic[1] = conic4.y;
+    conic[2] = conic4.z;
+    cx = mean.x;
+    cy = mean.y;
+    thr = gs_logf(mean.w * 255.0f);
     int max_tx = (int)fp.tiles_x - 1, max_ty = (int)fp.tiles_y - 1;
     float ts = (float)fp.tile_size, inv_ts = 1.0f / ts;  // a power of two: exact
-    int tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) * inv_ts)), 0), max_tx);
-    int tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) * inv_ts)), 0), max_ty);
-    int tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) * inv_ts)), 0), max_tx);
-    int tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) * inv_ts)), 0), max_ty);
-    uint32_t axis = fp.boundary_axis;
-    if (axis == 1) {
+    tmin_x = min(max(gs_f2i(gs_floorf((mean.x - rx) * inv_ts)), 0), max_tx);
+    tmin_y = min(max(gs_f2i(gs_floorf((mean.y - ry) * inv_ts)), 0), max_ty);
+    tmax_x = min(max(gs_f2i(gs_floorf((mean.x + rx) * inv_ts)), 0), max_tx);
+    tmax_y = min(max(gs_f2i(gs_floorf((mean.y + ry) * inv_ts)), 0), max_ty);
+    if (fp.boundary_axis == 1) {
       tmin_x = 0;
       tmax_x = max_tx;
     }
-    if (axis == 2) {
+    if (fp.boundary_axis == 2) {
       tmin_y = 0;
       tmax_y = max_ty;
     }
-    float vw = fp.viewport[0], vh = fp.viewport[1];
-    for (int ty = tmin_y; ty <= tmax_y; ty++)
-      for (int tx = tmin_x; tx <= tmax_x; tx++)
-        if (tile_hit(mean.x, mean.y, conic, thr, rx, ry, tx, ty, vw, vh, axis, ts)) {
-          if (local < reserved && base + local < capacity)
-            records[base + local] = make_uint2((uint32_t)ty * fp.tiles_x + (uint32_t)tx, gid);
-          local++;
-        }
   }
+  const uint32_t lane = lane_id();
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const float vw = fp.viewport[0], vh = fp.viewport[1], ts = (float)fp.tile_size;
+  for (uint64_t m = __ballot(big); m != 0ull; m &= m - 1ull) {
+    const int src = __builtin_ctzll(m);
+    const float scx = __shfl(cx, src, 64), scy = __shfl(cy, src, 64);
+    const float sconic[3] = {__shfl(conic[0], src, 64), __shfl(conic[1], src, 64),
+                             __shfl(conic[2], src, 64)};
+    const float sthr = __shfl(thr, src, 64), srx = __shfl(rx, src, 64),
+                sry = __shfl(ry, src, 64);
+    const int tx0 = __shfl(tmin_x, src, 64), ty0 = __shfl(tmin_y, src, 64);
+    const uint32_t bw = (uint32_t)(__shfl(tmax_x, src, 64) - tx0 + 1);
+    const uint32_t n = bw * (uint32_t)(__shfl(tmax_y, src, 64) - ty0 + 1);
+    const uint32_t sbase = (uint32_t)__shfl((int)base, src, 64);
+    const uint32_t sres = (uint32_t)__shfl((int)reserved, src, 64);
+    const uint32_t sgid = (uint32_t)__shfl((int)gid, src, 64);
+    const float inv_bw = 1.0f / (float)bw;
+    uint32_t run = 0u;  // hits of the box's earlier chunks
+    for (uint32_t i0 = 0; i0 < n; i0 += 64u) {
+      const uint32_t i = i0 + lane;
+      uint32_t row = 0u, col = 0u;
+      bool hit = false;
+      if (i < n) {
+        box_tile(i, bw, inv_bw, row, col);
+        hit = tile_hit(scx, scy, sconic, sthr, srx, sry, tx0 + (int)col, ty0 + (int)row, vw, vh,
+                       fp.boundary_axis, ts);
+      }
+      const uint64_t hb = __ballot(hit);
+      const uint32_t pos = run + (uint32_t)__popcll(hb & lt_mask);
+      if (hit && pos < sres && sbase + pos < capacity)
+        records[sbase + pos] =
+            make_uint2((uint32_t)(ty0 + (int)row) * fp.tiles_x + (uint32_t)(tx0 + (int)col), sgid);
+      run += (uint32_t)__popcll(hb);
+    }
+    if ((int)lane == src) local = run;
+  }
+  if (!live) return;
   for (uint32_t pad = local; pad < reserved; pad++)
     if (base + pad < capacity) records[base + pad] = make_uint2(fp.tile_count, gid);
 }
