@@ -53,6 +53,7 @@ class GsConfig(C.Structure):
         ("raster_batch", u32),
         ("batch_lanes", u32),
         ("heavy_split_units", u32),
+        ("heavy_fine_split_units", u32),
     ]
 
 

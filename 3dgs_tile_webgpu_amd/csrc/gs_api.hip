@@ -326,6 +326,7 @@ extern "C" void gs_config_default(gs_config* c) {
   c->raster_batch = 256;
   c->batch_lanes = 1;
   c->heavy_split_units = GS_SPLIT_AUTO;
+  c->heavy_fine_split_units = GS_SPLIT_AUTO;
 }
 
 // GaussianPass.ts:157-260 option validation.
@@ -772,6 +773,12 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   // of 906-1,907 units do not move).
   p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 7 / 4
                                                                : ctx->cfg.heavy_split_units;
+  // Row slabs have 8 rows up to 16 units per CU, 4 above: the 3M/1080p
+  // views (906-1,907 units) rasterize 9-13 % faster with 8, the 6M/4K
+  // frame (17,773 units) 25 % faster with 4.
+  p.fine_split_units = ctx->cfg.heavy_fine_split_units == GS_SPLIT_AUTO
+                           ? ctx->cu_count * 16
+                           : ctx->cfg.heavy_fine_split_units;
   // Training-set frames: chunking off, so every recorded alpha * T uses the
   // true transmittance in front of it.
   if (ctx->cull.collecting) p.chunk_size = 0;

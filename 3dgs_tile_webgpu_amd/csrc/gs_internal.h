@@ -62,6 +62,7 @@ struct FrameParams {
   uint32_t raster_stats;
   uint32_t tile_size;               // tile edge: 8, 16 or 32 (16 matches constants.ts:1)
   uint32_t split_units;             // heavy units above which 16x16 tiles run as row slabs
+  uint32_t fine_split_units;        // ... and above which those slabs have 4 rows, not 8
 };
 
 // Device-side counters of one frame (dispatch state + outputState).
@@ -77,7 +78,7 @@ struct DeviceState {
   unsigned long long checked_heavy;  // the part of `checked` in heavy units
   uint32_t sort_error;  // a look-back exceeded its wait bound: bit 0 in a
                         // radix pass, bit 1 in a scan
-  uint32_t heavy_split;  // this frame's heavy units run as row slabs
+  uint32_t heavy_split;  // this frame's heavy shape: 0 whole tiles, 1 8-row, 2 4-row slabs
   uint32_t light_units;  // row-slab frames: light tiles listed after the heavy units
   uint32_t heavy_next[8];  // per XCD: heavy work items taken beyond the first round
   uint32_t depth_in_place;  // the depth sort's last digit was one value for every key: its

@@ -1284,30 +1284,36 @@ struct PixelSum {
 //  - whole tiles: one block per unit, four lanes per pixel (256 threads at
 //    edge 8, 1024 at 16);
 //  - row slabs (edge 16, frames with more than FrameParams::split_units
-//    units): four blocks of 128 threads per unit, each covering 4 rows with
-//    two lanes per pixel. Such blocks share a CU ten at a time and each
-//    stops as soon as its own pixels are saturated, at the price of staging
-//    every record once per slab, which only pays when the units outnumber
-//    the CUs.
+//    units) with two lanes per pixel: two blocks of 256 threads per unit,
+//    each covering 8 rows, five to a CU; above fine_split_units units four
+//    blocks of 128 threads covering 4 rows, ten to a CU. Each slab stops as
+//    soon as its own pixels are saturated and a unit's work spreads over
+//    CUs, at the price of staging every record once per slab, which only
+//    pays when the units outnumber the CUs (and 4-row slabs only when they
+//    outnumber them many times over).
 // Edge 32 uses one block of 1024 threads per unit (one lane per pixel, 512
 // records per round).
 constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
 constexpr int kHeavyThreads = 1024;  // edge 32
 constexpr int log2_edge(int e) { return e == 8 ? 3 : (e == 16 ? 4 : 5); }
 constexpr int light_threads(int e, int batch) { return e * e > batch ? e * e : batch; }
-constexpr int heavy_slab_rows(int e, bool split) { return e == 16 && split ? 4 : e; }
-constexpr int heavy_slabs(int e, bool split) { return e / heavy_slab_rows(e, split); }
-constexpr int heavy_slab_pixels(int e, bool split) { return e * heavy_slab_rows(e, split); }
+// Heavy shapes (DeviceState::heavy_split; row slabs at edge 16 only).
+constexpr int kShapeWhole = 0, kShapeSlab8 = 1, kShapeSlab4 = 2;
+constexpr int heavy_slab_rows(int e, int shape) {
+  return e == 16 && shape != kShapeWhole ? (shape == kShapeSlab8 ? 8 : 4) : e;
+}
+constexpr int heavy_slabs(int e, int shape) { return e / heavy_slab_rows(e, shape); }
+constexpr int heavy_slab_pixels(int e, int shape) { return e * heavy_slab_rows(e, shape); }
 // Lanes per pixel at edges 8 and 16: four for whole tiles, whose frames
 // have about as many units as CUs and wait on the longest unit (more lanes,
 // shorter units); two for row slabs, whose frames have many more units
 // than CUs and are bound by total VALU work (the lanes of a group repeat
 // every blend, so fewer lanes do less of it).
-constexpr int heavy_lanes(int e, bool split) { return e == 16 && split ? 2 : 4; }
-constexpr int heavy_threads(int e, bool split) {
-  return e == 32 ? kHeavyThreads : heavy_lanes(e, split) * heavy_slab_pixels(e, split);
+constexpr int heavy_lanes(int e, int shape) { return e == 16 && shape != kShapeWhole ? 2 : 4; }
+constexpr int heavy_threads(int e, int shape) {
+  return e == 32 ? kHeavyThreads : heavy_lanes(e, shape) * heavy_slab_pixels(e, shape);
 }
-constexpr int heavy_batch(int e, bool split) { return e == 32 ? 512 : heavy_threads(e, split); }
+constexpr int heavy_batch(int e, int shape) { return e == 32 ? 512 : heavy_threads(e, shape); }
 
 template <int kPixels, int kBatch>
 struct RasterLds {
@@ -1897,7 +1903,9 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
       next += class_count[c];
     }
     state->heavy_units = min(next, unit_capacity);
-    state->heavy_split = next > fp.split_units ? 1u : 0u;
+    state->heavy_split = next <= fp.split_units
+                             ? (uint32_t)kShapeWhole
+                             : (uint32_t)(next <= fp.fine_split_units ? kShapeSlab8 : kShapeSlab4);
     for (int x = 0; x < 8; x++) state->heavy_next[x] = 0u;
   }
   __syncthreads();
@@ -1925,10 +1933,10 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
   }
 }
 
-// Heavy units: per chunk task or heavy tile, S = heavy_slabs(E, kSplit)
-// work items, one per row slab; the kSplit and whole-tile launches both
-// run and the one the frame did not choose (DeviceState::heavy_split, set
-// by k_heavy_units) exits at once. The grid is persistent (as many blocks
+// Heavy units: per chunk task or heavy tile, S = heavy_slabs(E, kShape)
+// work items, one per row slab; the launches of every shape run and those
+// the frame did not choose (DeviceState::heavy_split, set by
+// k_heavy_units) exit at once. The grid is persistent (as many blocks
 // as fit on the GPU at once, a multiple of 8). Workgroups go to the eight
 // XCDs in turn, so block b works for XCD x = b % 8: its items are
 // k = 0, 1, 2, ... of that XCD's sequence, where item k is unit
@@ -1939,8 +1947,8 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
 // tiles per item. A block starts with item b / 8 and then takes the XCD's
 // next unclaimed item (one atomic per item), so blocks that finish early
 // keep working.
-template <int kEdge, bool kSplit, bool kRecord>
-__global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
+template <int kEdge, int kShape, bool kRecord>
+__global__ __launch_bounds__(heavy_threads(kEdge, kShape)) void k_raster_heavy(
     const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
     const uint32_t* __restrict__ off, const Splat* __restrict__ splat,
     const uint2* __restrict__ units,
@@ -1949,11 +1957,12 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
     uint32_t* __restrict__ record) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr bool kGroup = kEdge != 32;  // lane groups per pixel (edges 8, 16)
-  constexpr uint32_t kSlabs = heavy_slabs(kEdge, kSplit);
-  constexpr int kBatch = heavy_batch(kEdge, kSplit);
-  __shared__ RasterLds<heavy_slab_pixels(kEdge, kSplit), kBatch> lds;
+  constexpr bool kSplit = kShape != kShapeWhole;  // row slabs, which also take the light tiles
+  constexpr uint32_t kSlabs = heavy_slabs(kEdge, kShape);
+  constexpr int kBatch = heavy_batch(kEdge, kShape);
+  __shared__ RasterLds<heavy_slab_pixels(kEdge, kShape), kBatch> lds;
   __shared__ uint32_t next_item;
-  if (kEdge == 16 && (state->heavy_split != 0u) != kSplit) return;
+  if (kEdge == 16 && state->heavy_split != (uint32_t)kShape) return;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
   const uint32_t n_heavy = state->heavy_units;
   const uint32_t n_light = kSplit ? state->light_units : 0u;
@@ -1999,15 +2008,15 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
         uint32_t pix = threadIdx.x;  // pixel within the tile
         bool writer = threadIdx.x < (uint32_t)kPixels;
         if constexpr (kGroup) {
-          constexpr int kLanes = heavy_lanes(kEdge, kSplit);
-          constexpr int kRows = heavy_slab_rows(kEdge, kSplit);
+          constexpr int kLanes = heavy_lanes(kEdge, kShape);
+          constexpr int kRows = heavy_slab_rows(kEdge, kShape);
           if (fp.boundary_axis == 0u)
             s = blend_heavy_group<kEdge, kRows, kLanes, kBatch, kRecord, false>(
                 fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
           else
             s = blend_heavy_group<kEdge, kRows, kLanes, kBatch, kRecord, true>(
                 fp, tile, slab, cb, ce, sorted, splat, lds, x, y, active, record);
-          pix = slab * heavy_slab_pixels(kEdge, kSplit) + (threadIdx.x >> (kLanes / 2));
+          pix = slab * heavy_slab_pixels(kEdge, kShape) + (threadIdx.x >> (kLanes / 2));
           writer = (threadIdx.x & (uint32_t)(kLanes - 1)) == 0u;
         } else {
           s = blend_tile_range<kEdge, kHeavyThreads, kBatch, kRecord>(
@@ -2035,12 +2044,13 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kSplit)) void k_raster_heavy(
 
 // Persistent grid of k_raster_heavy, a multiple of 8 blocks (at least 8):
 // whole-tile blocks (16 waves) one per CU, leaving room for the light-tile
-// kernel beside them; row-slab blocks (2 waves) ten per CU, the waves per
-// CU their registers allow (measured faster than leaving the light kernel
-// room: the slab units outnumber the CUs and gain most from occupancy).
-template <int kEdge, bool kSplit>
+// kernel beside them; row-slab blocks (4 or 2 waves) 20 waves per CU, the
+// waves their registers allow (measured faster than leaving the light
+// kernel room: the slab units outnumber the CUs and gain most from
+// occupancy).
+template <int kEdge, int kShape>
 inline uint32_t heavy_grid(uint32_t cu_count) {
-  uint32_t waves = (uint32_t)heavy_threads(kEdge, kSplit) / 64u;
+  uint32_t waves = (uint32_t)heavy_threads(kEdge, kShape) / 64u;
   uint32_t per_cu = waves >= 16u ? 1u : 20u / waves;
   uint32_t g = (cu_count * per_cu) & ~7u;
   return g < 8u ? 8u : g;
@@ -2134,18 +2144,23 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   hipError_t e = hipEventRecord(rs.fork, s);
   if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
   if (e != hipSuccess) return e;
-  // The whole-tile kernel, then (edge 16) the row-slab kernel: one of them
-  // exits at once, and does so before the other can fill the GPU.
+  // The whole-tile kernel, then (edge 16) the two row-slab kernels: all but
+  // the frame's shape exit at once, each before the next can fill the GPU.
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], rs.aux);
-  k_raster_heavy<kEdge, false, kRecord><<<heavy_grid<kEdge, false>(b.cu_count),
-                                           heavy_threads(kEdge, false), 0, rs.aux>>>(
-      fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials, b.partial_depth,
-      b.color_out, b.depth_out, b.record);
-  if constexpr (kEdge == 16)
-    k_raster_heavy<kEdge, true, kRecord><<<heavy_grid<kEdge, true>(b.cu_count),
-                                           heavy_threads(kEdge, true), 0, rs.aux>>>(
-        fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
-        b.partial_depth, b.color_out, b.depth_out, b.record);
+  k_raster_heavy<kEdge, kShapeWhole, kRecord>
+      <<<heavy_grid<kEdge, kShapeWhole>(b.cu_count), heavy_threads(kEdge, kShapeWhole), 0,
+         rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+                   b.partial_depth, b.color_out, b.depth_out, b.record);
+  if constexpr (kEdge == 16) {
+    k_raster_heavy<kEdge, kShapeSlab8, kRecord>
+        <<<heavy_grid<kEdge, kShapeSlab8>(b.cu_count), heavy_threads(kEdge, kShapeSlab8), 0,
+           rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+                     b.partial_depth, b.color_out, b.depth_out, b.record);
+    k_raster_heavy<kEdge, kShapeSlab4, kRecord>
+        <<<heavy_grid<kEdge, kShapeSlab4>(b.cu_count), heavy_threads(kEdge, kShapeSlab4), 0,
+           rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+                     b.partial_depth, b.color_out, b.depth_out, b.record);
+  }
   if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], rs.aux);
   e = hipEventRecord(rs.join, rs.aux);
   if (e != hipSuccess) return e;

@@ -48,6 +48,7 @@ DEFAULT_OPTIONS = {
 KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | {
     "fixedSize", "depthRange", "intersectionCapacity", "initialIntersectionCapacity",
     "maxIntersectionCapacity", "device", "dataMode", "batchLanes", "heavySplitUnits",
+    "heavyFineSplitUnits",
 }
 MAX_CAPACITY = 1 << 28
 
@@ -129,10 +130,11 @@ def validate_options(options: dict) -> dict:
     lanes = o.get("batchLanes", 1)
     if lanes != 1:
         raise N.GsRangeError(N.GS_E_RANGE, "batch lanes above 1 are disabled")
-    split = o.get("heavySplitUnits")
-    if split is not None and (not isinstance(split, int) or isinstance(split, bool)
-                              or not 0 <= split < 0xFFFFFFFF):
-        raise N.GsRangeError(N.GS_E_RANGE, "heavySplitUnits must be a non-negative integer")
+    for name in ("heavySplitUnits", "heavyFineSplitUnits"):
+        split = o.get(name)
+        if split is not None and (not isinstance(split, int) or isinstance(split, bool)
+                                  or not 0 <= split < 0xFFFFFFFF):
+            raise N.GsRangeError(N.GS_E_RANGE, f"{name} must be a non-negative integer")
     return o
 
 
@@ -159,6 +161,8 @@ def config_for(o: dict, policy: IntersectionCapacityPolicy) -> N.GsConfig:
     cfg.batch_lanes = int(o.get("batchLanes", 1))
     split = o.get("heavySplitUnits")
     cfg.heavy_split_units = 0xFFFFFFFF if split is None else int(split)
+    fine = o.get("heavyFineSplitUnits")
+    cfg.heavy_fine_split_units = 0xFFFFFFFF if fine is None else int(fine)
     return cfg
 
 

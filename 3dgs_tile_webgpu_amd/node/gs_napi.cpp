@@ -140,6 +140,8 @@ void read_config(napi_env env, napi_value o, gs_config* c) {
   c->radix_bits = (uint32_t)num(env, o, "radixBits", c->radix_bits);
   c->raster_batch = (uint32_t)num(env, o, "rasterBatch", c->raster_batch);
   c->heavy_split_units = (uint32_t)num(env, o, "heavySplitUnits", c->heavy_split_units);
+  c->heavy_fine_split_units =
+      (uint32_t)num(env, o, "heavyFineSplitUnits", c->heavy_fine_split_units);
   napi_value bg;
   if (get_prop(env, o, "background", &bg)) {
     for (uint32_t i = 0; i < 4; i++) {

@@ -30,9 +30,12 @@ export interface GaussianPassOptions {
   tileSize?: 8 | 16 | 32;
   radixBits?: 4 | 6 | 8;
   rasterBatch?: 128 | 256;
-  /** Heavy raster units above which 16-pixel tiles run as 4-row slabs
+  /** Heavy raster units above which 16-pixel tiles run as row slabs
    * (default: 1.75 per compute unit); never changes any output. */
   heavySplitUnits?: number;
+  /** Heavy raster units above which those slabs have 4 rows instead of 8
+   * (default: 16 per compute unit); never changes any output. */
+  heavyFineSplitUnits?: number;
   device?: number;
 }
 

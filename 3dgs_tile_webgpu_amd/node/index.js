@@ -36,7 +36,8 @@ const DEFAULTS = Object.freeze({
   rasterBatch: 256,
 });
 const KNOWN = new Set([...Object.keys(DEFAULTS), 'fixedSize', 'depthRange', 'intersectionCapacity',
-  'initialIntersectionCapacity', 'maxIntersectionCapacity', 'device', 'heavySplitUnits']);
+  'initialIntersectionCapacity', 'maxIntersectionCapacity', 'device', 'heavySplitUnits',
+  'heavyFineSplitUnits']);
 const BOUNDARY = { clip: 0, 'repeat-x': 1, 'repeat-y': 2 };
 const MAX_CAPACITY = 2 ** 28;
 const RESOURCE_NAMES = ['projected_mean', 'projected_conic', 'projected_color', 'visible_offsets',
@@ -110,9 +111,11 @@ function validateOptions(options) {
   if (![8, 16, 32].includes(o.tileSize) || ![4, 6, 8].includes(o.radixBits) || ![128, 256].includes(o.rasterBatch)) {
     throw new RangeError('tileSize must be 8, 16 or 32, radixBits 4, 6 or 8, rasterBatch 128 or 256');
   }
-  if (o.heavySplitUnits !== undefined &&
-      !(Number.isInteger(o.heavySplitUnits) && o.heavySplitUnits >= 0 && o.heavySplitUnits < 0xffffffff)) {
-    throw new RangeError('heavySplitUnits must be a non-negative integer');
+  for (const name of ['heavySplitUnits', 'heavyFineSplitUnits']) {
+    const v = o[name];
+    if (v !== undefined && !(Number.isInteger(v) && v >= 0 && v < 0xffffffff)) {
+      throw new RangeError(name + ' must be a non-negative integer');
+    }
   }
   return o;
 }
@@ -136,6 +139,7 @@ function nativeConfig(o, policy) {
     radixBits: o.radixBits,
     rasterBatch: o.rasterBatch,
     heavySplitUnits: o.heavySplitUnits === undefined ? 0xffffffff : o.heavySplitUnits,
+    heavyFineSplitUnits: o.heavyFineSplitUnits === undefined ? 0xffffffff : o.heavyFineSplitUnits,
   };
 }
 

@@ -85,8 +85,12 @@ typedef struct gs_config {
                                         is accepted at present (default 1) */
   uint32_t heavy_split_units;        /* heavy raster units (tiles above 1024
                                         records, chunk tasks) above which 16-pixel
-                                        tiles are rasterized as 4-row slabs;
+                                        tiles are rasterized as row slabs;
                                         GS_SPLIT_AUTO (default) = 1.75 per compute
+                                        unit; never changes any output */
+  uint32_t heavy_fine_split_units;   /* heavy raster units above which those
+                                        slabs have 4 rows instead of 8;
+                                        GS_SPLIT_AUTO (default) = 16 per compute
                                         unit; never changes any output */
 } gs_config;
 

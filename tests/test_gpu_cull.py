@@ -35,7 +35,7 @@ def test_collected_features_and_record_match_oracle(gs, oracle, gpu_available, s
     cam = w.camera()
     opts = {"intersectionCapacity": 1 << 21}
     if split is not None:
-        opts["heavySplitUnits"] = split  # the instrumented raster as 4-row slabs
+        opts["heavySplitUnits"] = split  # the instrumented raster as row slabs
     p = gs.gaussian_pass(cam, small, opts)
     cull = gs.LearnedCull(p, seed=1)
     cull.reserve(1 << 17)

@@ -92,6 +92,9 @@ def test_small_workload_bit_exact(gs, oracle, gpu_available, small):
     {"heavySplitUnits": 0},
     {"heavySplitUnits": 0, "rasterChunkSize": 256},
     {"heavySplitUnits": 0, "boundaryPolicy": "repeat-x"},
+    {"heavySplitUnits": 0, "heavyFineSplitUnits": 0},
+    {"heavySplitUnits": 0, "heavyFineSplitUnits": 0, "rasterChunkSize": 256},
+    {"heavySplitUnits": 0, "heavyFineSplitUnits": 0, "boundaryPolicy": "repeat-y"},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_option_variants_bit_exact(gs, oracle, gpu_available, small, options):
     w = gs.WORKLOADS["small"]
@@ -176,13 +179,15 @@ def test_graph_replay_and_recapture(gs, oracle, gpu_available, small):
     p.dispose()
 
 
-@pytest.mark.parametrize("split", [None, 0], ids=["auto", "slabs"])
-def test_1m_1080p_bit_exact(gs, oracle, gpu_available, split):
-    """Heavy units run as whole tiles here by default (275 units); with
-    heavySplitUnits=0 as 4-row slabs. Both must give the oracle's bits."""
+@pytest.mark.parametrize("opts", [{}, {"heavySplitUnits": 0},
+                                  {"heavySplitUnits": 0, "heavyFineSplitUnits": 0}],
+                         ids=["auto", "slabs8", "slabs4"])
+def test_1m_1080p_bit_exact(gs, oracle, gpu_available, opts):
+    """Heavy units run as whole tiles here by default (275 units); forced
+    to row slabs, as 8-row slabs (275 units is below the 4-row threshold)
+    or as 4-row ones. Every shape must give the oracle's bits."""
     w = gs.WORKLOADS["1m_1080p"]
     scene = gs.workload_scene("1m_1080p")
-    opts = {} if split is None else {"heavySplitUnits": split}
     p, ref = render_both(gs, oracle, scene, w.camera(), w.width, w.height, opts)
     assert ref.counts.chunk_tasks > 0  # dense cluster tiles take the chunk path
     assert_frame_matches(p, ref)

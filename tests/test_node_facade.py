@@ -76,6 +76,13 @@ CASES = [
     {"intersectionGrowthFactor": 1.0},
     {"intersectionHeadroom": 0.5},
     {"intersectionCapacity": 0},
+    {"heavySplitUnits": 0},
+    {"heavySplitUnits": -1},
+    {"heavySplitUnits": 1.5},
+    {"heavyFineSplitUnits": 0},
+    {"heavyFineSplitUnits": 4096},
+    {"heavyFineSplitUnits": -2},
+    {"heavyFineSplitUnits": 0xFFFFFFFF},
 ]
 
 
