@@ -129,8 +129,9 @@ class LearnedCull:
 
     def calibrate(self, cameras: list[PerspectiveCamera], sizes: list[tuple[int, int]],
                   target_psnr: float = 46.0,
-                  shifts=(-3.0, -2.5, -2.0, -1.75, -1.5, -1.25, -1.0, -0.75, -0.5, -0.25, 0.0,
-                          0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0)) -> tuple[float, list]:
+                  shifts=(-3.0, -2.5, -2.0, -1.875, -1.75, -1.625, -1.5, -1.375, -1.25, -1.125,
+                          -1.0, -0.75, -0.5, -0.25, 0.0, 0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0,
+                          8.0)) -> tuple[float, list]:
         """Move the network's output bias by the lowest shift (most culling)
         whose culled frames of these cameras (a BatchedPass owner) keep a
         mean PSNR of at least target_psnr against their plain frames. Returns
