@@ -172,8 +172,9 @@ struct FrameBuffers {
 };
 
 // Frame stages, in launch order; launch_frame can record an event before
-// each stage and one after the last (kStageCount + 1 events), then two on
-// the raster's second stream around the heavy-unit kernels (kMarks in all).
+// each stage and one after the last (kStageCount + 1 events), then four on
+// the raster's second stream: before the heavy-unit launches and after each
+// of the three (kMarks in all).
 enum Stage : int {
   kStageProject = 0,
   kStageVisible,
@@ -185,9 +186,9 @@ enum Stage : int {
   kStageRaster,
   kStageCount
 };
-constexpr int kMarkHeavyBegin = kStageCount + 1;
-constexpr int kMarkHeavyEnd = kStageCount + 2;
-constexpr int kMarks = kStageCount + 3;
+constexpr int kMarkHeavyBegin = kStageCount + 1;  // then one after each heavy shape's launch
+constexpr int kMarkHeavyShapes = 3;
+constexpr int kMarks = kMarkHeavyBegin + kMarkHeavyShapes + 1;
 
 // The raster's second branch: the heavy-unit kernels run on aux, forked
 // from the frame's stream at `fork` and joined back into it at `join`,

@@ -2151,17 +2151,21 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
       <<<heavy_grid<kEdge, kShapeWhole>(b.cu_count), heavy_threads(kEdge, kShapeWhole), 0,
          rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
                    b.partial_depth, b.color_out, b.depth_out, b.record);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 1], rs.aux);
   if constexpr (kEdge == 16) {
     k_raster_heavy<kEdge, kShapeSlab8, kRecord>
         <<<heavy_grid<kEdge, kShapeSlab8>(b.cu_count), heavy_threads(kEdge, kShapeSlab8), 0,
            rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
                      b.partial_depth, b.color_out, b.depth_out, b.record);
+  }
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], rs.aux);
+  if constexpr (kEdge == 16) {
     k_raster_heavy<kEdge, kShapeSlab4, kRecord>
         <<<heavy_grid<kEdge, kShapeSlab4>(b.cu_count), heavy_threads(kEdge, kShapeSlab4), 0,
            rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
                      b.partial_depth, b.color_out, b.depth_out, b.record);
   }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyEnd], rs.aux);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], rs.aux);
   e = hipEventRecord(rs.join, rs.aux);
   if (e != hipSuccess) return e;
   uint32_t light_blocks = b.max_tiles < kLightBlocks ? b.max_tiles : kLightBlocks;

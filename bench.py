@@ -308,7 +308,7 @@ def main():
         achieved = flops / (heavy_kernel_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                    "traffic": (profiled_traffic("gs::k_raster_heavy<16, false, false>")
+                    "traffic": (profiled_traffic("gs::k_raster_heavy<16, 0, false>")
                                 if args.workload == "1m_1080p" else None),
                     "traffic_source": "profiles/frame_1m_1080p_hbm_traffic.md (rocprofv3 "
                                       "FETCH_SIZE + WRITE_SIZE, copy-calibrated)",

@@ -270,8 +270,9 @@ int gs_device_image(gs_ctx* ctx, void** color, void** depth);
 int gs_profile_begin(gs_ctx* ctx, uint32_t max_frames);
 /* Synchronises; stage_ms receives the per-stage sums over the recorded
  * frames, frame_ms the sum of whole-frame times, heavy_ms the sum of the
- * heavy-unit raster kernel's times (events on its own stream), *frames the
- * count. */
+ * heavy-unit raster kernel's times (events on its own stream around each of
+ * its per-shape launches, the longest of which per frame is the one that
+ * ran; the others exit at once), *frames the count. */
 int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], double* frame_ms,
                    double* heavy_ms, uint32_t* frames);
 
