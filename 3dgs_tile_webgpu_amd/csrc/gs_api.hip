@@ -763,15 +763,15 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   p.output_depth = ctx->cfg.output_depth ? 1u : 0u;
   p.raster_stats = ctx->cfg.raster_stats ? 1u : 0u;
   p.tile_size = ts;
-  // Heavy units run as row slabs above 1.75 units per CU: below that, a
+  // Heavy units run as row slabs above 1.25 units per CU: below that, a
   // whole-tile block per unit already covers the CUs and slabs mostly
   // repeat the per-record staging. Measured on this GPU: the 1M/1080p frame
-  // (275 units) rasterizes its heavy units in 0.25 ms whole, 0.49 ms as
-  // slabs; batched culled 3M/1080p views run
-  // 1,264 frames/s with this threshold, 1,147 at 2.5 units per CU and 998
-  // at 3 (the 2M mixed-resolution batch gains 2.6% at 2.5, plain 3M views
-  // of 906-1,907 units do not move).
-  p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 7 / 4
+  // (275 units) rasterizes its heavy units in 0.21 ms whole, 0.30 ms as
+  // 8-row slabs; batched culled 3M/1080p views run 1,483 frames/s with this
+  // threshold, 1,458 at 1.75 units per CU and 1,430 at 2 (the 2M
+  // mixed-resolution batch and plain 3M views of 906-1,907 units do not
+  // move).
+  p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 5 / 4
                                                                : ctx->cfg.heavy_split_units;
   // Row slabs have 8 rows up to 16 units per CU, 4 above: the 3M/1080p
   // views (906-1,907 units) rasterize 9-13 % faster with 8, the 6M/4K

@@ -31,7 +31,7 @@ export interface GaussianPassOptions {
   radixBits?: 4 | 6 | 8;
   rasterBatch?: 128 | 256;
   /** Heavy raster units above which 16-pixel tiles run as row slabs
-   * (default: 1.75 per compute unit); never changes any output. */
+   * (default: 1.25 per compute unit); never changes any output. */
   heavySplitUnits?: number;
   /** Heavy raster units above which those slabs have 4 rows instead of 8
    * (default: 16 per compute unit); never changes any output. */

@@ -86,7 +86,7 @@ typedef struct gs_config {
   uint32_t heavy_split_units;        /* heavy raster units (tiles above 1024
                                         records, chunk tasks) above which 16-pixel
                                         tiles are rasterized as row slabs;
-                                        GS_SPLIT_AUTO (default) = 1.75 per compute
+                                        GS_SPLIT_AUTO (default) = 1.25 per compute
                                         unit; never changes any output */
   uint32_t heavy_fine_split_units;   /* heavy raster units above which those
                                         slabs have 4 rows instead of 8;
