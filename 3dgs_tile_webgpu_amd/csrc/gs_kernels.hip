@@ -390,16 +390,6 @@ GS_DEV ProjOut project_finish_counted(const FrameParams& fp, const SceneView& sc
   return o;
 }
 
-GS_DEV ProjOut project_finish(const FrameParams& fp, const SceneView& sc, const ProjGeom& g) {
-  TileBox b = {0, 0, -1, -1};
-  uint32_t count = 0u, mask = 0u;
-  if (g.kind == kProjVisible) {
-    b = tile_box(fp, g);
-    count = count_box(fp, g, b, mask);
-  }
-  return project_finish_counted(fp, sc, g, b, count, mask);
-}
-
 // The largest rgb8e8 exponent of the higher SH bands, minus the bias: the
 // log2 scale of the strongest view-dependent colour term (a learned-cull
 // feature; -127 without higher bands).
@@ -573,25 +563,42 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
          g.thr, g.rx, g.ry, g.dir[0], g.dir[1], g.dir[2]};
   }
   __syncthreads();
-  if (threadIdx.x < total) {
-    const CullSurvivor v = lds.surv[threadIdx.x];
+  // Waves holding survivors finish them, the lanes past the last survivor
+  // joining in the wave's count of large boxes.
+  if ((threadIdx.x & ~63u) < total) {  // wave-uniform
+    const bool live = threadIdx.x < total;
+    uint32_t sgid = 0u;
     ProjGeom s;
-    s.kind = kProjVisible;
-    s.slot = v.slot;
-    s.cx = v.cx;
-    s.cy = v.cy;
-    s.depth = v.depth;
-    s.opacity = v.opacity;
-    s.conic[0] = v.c0;
-    s.conic[1] = v.c1;
-    s.conic[2] = v.c2;
-    s.thr = v.thr;
-    s.rx = v.rx;
-    s.ry = v.ry;
-    s.dir[0] = v.d0;
-    s.dir[1] = v.d1;
-    s.dir[2] = v.d2;
-    store_projection(out, v.gid, project_finish(fp, sc, s));
+    s.kind = kProjNone;
+    if (live) {
+      const CullSurvivor v = lds.surv[threadIdx.x];
+      sgid = v.gid;
+      s.kind = kProjVisible;
+      s.slot = v.slot;
+      s.cx = v.cx;
+      s.cy = v.cy;
+      s.depth = v.depth;
+      s.opacity = v.opacity;
+      s.conic[0] = v.c0;
+      s.conic[1] = v.c1;
+      s.conic[2] = v.c2;
+      s.thr = v.thr;
+      s.rx = v.rx;
+      s.ry = v.ry;
+      s.dir[0] = v.d0;
+      s.dir[1] = v.d1;
+      s.dir[2] = v.d2;
+    }
+    TileBox box = {0, 0, -1, -1};
+    uint32_t count = 0u, mask = 0u;
+    bool big = false;
+    if (live) {
+      box = tile_box(fp, s);
+      big = box.width() * box.height() > kMaskTiles;
+      if (!big) count = count_box(fp, s, box, mask);
+    }
+    count_big_boxes(fp, big, s, box, count);
+    if (live) store_projection(out, sgid, project_finish_counted(fp, sc, s, box, count, mask));
   }
 }
 

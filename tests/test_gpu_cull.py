@@ -105,12 +105,24 @@ def test_gradient_matches_reference(gs, gpu_available, small):
     p.dispose()
 
 
-def test_cull_equals_plain_on_survivors(gs, oracle, gpu_available, small):
-    """With the cull on, the frame is the plain frame of the survivors."""
-    w = gs.WORKLOADS["small"]
-    cam = w.camera()
-    p = gs.gaussian_pass(cam, small, {"intersectionCapacity": 1 << 21})
-    p.render(800, 450)
+@pytest.mark.parametrize("scene_name", ["small", "anisotropic"])
+def test_cull_equals_plain_on_survivors(gs, oracle, gpu_available, small, scene_name):
+    """With the cull on, the frame is the plain frame of the survivors. The
+    anisotropic scene (a 200k draw of the 6M/4K distributions, near-plane
+    clusters) has survivors whose boxes exceed 32 tiles, which their wave
+    counts together."""
+    if scene_name == "small":
+        w = gs.WORKLOADS["small"]
+        cam = w.camera()
+        scene, width, height = small, 800, 450
+    else:
+        w = gs.WORKLOADS["6m_4k"]
+        scene = gs.workload_scene("6m_4k", count=200_000)
+        cam = gs.PerspectiveCamera(fov=60.0, aspect=960 / 540,
+                                   position=(0.0, 0.0, w.camera_distance))
+        width, height = 960, 540
+    p = gs.gaussian_pass(cam, scene, {"intersectionCapacity": 1 << 23})
+    p.render(width, height)
     plain_mean = p.read_debug("projected_mean")
     cull = gs.LearnedCull(p, seed=5)
     rng = np.random.default_rng(5)
@@ -119,8 +131,8 @@ def test_cull_equals_plain_on_survivors(gs, oracle, gpu_available, small):
     cull.set_params(params)
     # Standardise on the visible Gaussians' oracle features so the random net
     # spreads its predictions, then shift its output so half fall below.
-    f = gs.frame_for(cam, 800, 450, p.options)
-    ref_feats = oracle.run_frame(small, f, p.object_frames(), capacity=p.capacity, chunk_size=0,
+    f = gs.frame_for(cam, width, height, p.options)
+    ref_feats = oracle.run_frame(scene, f, p.object_frames(), capacity=p.capacity, chunk_size=0,
                                  cull_outputs=True)
     vis = ref_feats.get("projected_mean")[:, 3] > 0
     fv = ref_feats.get("cull_features")[vis]
@@ -129,14 +141,20 @@ def test_cull_equals_plain_on_survivors(gs, oracle, gpu_available, small):
     params[R.OFF_B3] = math.log(1e-3) - float(np.median(pred))  # median lands on the threshold
     cull.set_params(params)
     cull.enable(threshold=1e-3)
-    p.render(800, 450)
+    p.render(width, height)
     mean = p.read_debug("projected_mean")
     culled = (plain_mean[:, 3] > 0) & (mean[:, 3] == 0)
+    if scene_name == "anisotropic":  # survivors with boxes above 32 tiles (radius code)
+        code = p.read_debug("projected_conic")[:, 3]
+        kept = (mean[:, 3] > 0) & (code > 0)
+        rx, ry = code[kept] % 512, code[kept] // 512
+        assert ((2 * rx // 16 + 1) * (2 * ry // 16 + 1) > 32).sum() > 10
     frac = culled.sum() / max(1, (plain_mean[:, 3] > 0).sum())
     print(f"culled {culled.sum()} of {(plain_mean[:, 3] > 0).sum()} visible ({frac:.1%})")
     assert 0.2 < frac < 0.8
     survivors = np.flatnonzero(~culled).astype(np.uint32)
-    sub = gs.PackedScene(small.means, small.scales_opacity, small.rotations, small.sh, 3, survivors)
+    sub = gs.PackedScene(scene.means, scene.scales_opacity, scene.rotations, scene.sh, 3,
+                         survivors)
     ref = oracle.run_frame(sub, f, p.object_frames(), capacity=p.capacity, chunk_size=8192)
     np.testing.assert_array_equal(p.read_image().view(np.uint16), ref.get("image_f16"))
     assert p.read_state().intersections == ref.counts.count
