@@ -98,8 +98,10 @@ struct SceneView {
 // gather of a key (by active ordinal, in depth order within a tile, hence
 // scattered) reads one line instead of three: mean (cx, cy, depth,
 // opacity), conic (a, b, c, rX + 512 rY), colour (r, g, b, slot bits) and
-// aux.x = ln(255 opacity), the raster's per-Gaussian cutoff
-// (TileRasterizer.ts:579-581), computed once by the projection.
+// aux = (ln(255 opacity), the raster's per-Gaussian cutoff
+// (TileRasterizer.ts:579-581); the row and column terms and the shared
+// slack coefficient of the raster's support bound), computed once by the
+// projection.
 struct alignas(64) Splat {
   float4 mean, conic, color, aux;
 };

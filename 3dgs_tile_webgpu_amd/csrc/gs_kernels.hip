@@ -79,6 +79,7 @@ struct ProjOut {
   int kind;
   float4 mean, conic, color;
   float thr;  // ln(255 opacity)
+  float3 support;  // the raster's support bound terms (support_terms)
   uint32_t count;
   uint4 hits;  // HitRecord
 };
@@ -197,6 +198,41 @@ GS_DEV ProjGeom project_geometry(const FrameParams& fp, const SceneView& sc, uin
 
 // The exact tile count, hit record and SH colour of a candidate
 // (ProjectionStage.ts:600-649): visible if it hits a tile.
+// The per-Gaussian terms of the raster's support bound (record_rect): with
+// det = c0 c2 - c1^2, two_t = 2 thr (1 + 1e-6), g = 3e-7 and, for the rows,
+// n = det - g (c2 + |c1|) c0, the half-height h of the rows a block may
+// need obeys h^2 = two_t c0 / n + g (c0 + |c1|) c0 / n * O^2, O bounding
+// |dx| over the block's columns; columns likewise with c0 and c2 exchanged.
+// Returned: (rows' constant term, columns' constant term, the larger of the
+// two O^2 coefficients), each rounded once from binary64; an axis without a
+// bound (not positive definite, infinite cutoff, n <= 0) gets +inf, which
+// record_rect reads as "no narrowing".
+GS_DEV float3 support_terms(const float conic[3], float thr) {
+  const double g = 3.0e-7;
+  const double c0 = conic[0], c1a = fabs((double)conic[1]), c2 = conic[2];
+  const double det = c0 * c2 - (double)conic[1] * (double)conic[1];
+  const double T = thr;
+  const float inf = gs_b2f(0x7f800000u);
+  float3 t = make_float3(inf, inf, 0.0f);
+  if (!(det > 0.0 && c0 > 0.0 && c2 > 0.0 && T >= 0.0 && T < 1e30)) return t;
+  const double two_t = 2.0 * T * (1.0 + 1e-6);
+  const double n_rows = det - g * (c2 + c1a) * c0, n_cols = det - g * (c0 + c1a) * c2;
+  double b = 0.0;
+  if (n_rows > 0.0) {
+    t.x = (float)(two_t * c0 / n_rows);
+    b = fmax(b, g * (c0 + c1a) * c0 / n_rows);
+  }
+  if (n_cols > 0.0) {
+    t.y = (float)(two_t * c2 / n_cols);
+    b = fmax(b, g * (c2 + c1a) * c2 / n_cols);
+  }
+  // Rounded up so that the binary32 coefficient is never below the bound.
+  float bf = (float)b;
+  if ((double)bf < b) bf = gs_b2f(gs_f2b(bf) + 1u);
+  t.z = bf;
+  return t;
+}
+
 // A visible Gaussian's bounding box in tiles (ProjectionStage.ts:590-618).
 struct TileBox {
   int tmin_x, tmin_y, tmax_x, tmax_y;
@@ -386,6 +422,7 @@ GS_DEV ProjOut project_finish_counted(const FrameParams& fp, const SceneView& sc
   o.conic = make_float4(conic[0], conic[1], conic[2], rx + ry * 512.0f);
   o.color = make_float4(r, g_, b, gs_b2f(slot ^ 0x3f800000u));
   o.thr = thr;
+  o.support = support_terms(conic, thr);
   o.count = count;
   return o;
 }
@@ -446,7 +483,7 @@ GS_DEV void store_projection(const ProjectedView& out, uint32_t gid, const ProjO
     s->mean = o.mean;
     s->conic = o.conic;
     s->color = o.color;
-    s->aux = make_float4(o.thr, 0.0f, 0.0f, 0.0f);
+    s->aux = make_float4(o.thr, o.support.x, o.support.y, o.support.z);
     out.tile_counts[gid] = o.count;
     out.hits[gid] = o.hits;
   } else {
@@ -1225,43 +1262,24 @@ __global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
 // roundoffs), q >= (det / c2) dx^2 and S <= (c0 + |c1|) dx^2 + (c2 + |c1|)
 // dy^2; so over any set of rows with |dy| <= Y, q_hat > 2 thr whenever
 // dx^2 (det / c2 - g (c0 + |c1|)) > 2 thr + g (c2 + |c1|) Y^2. The same
-// bound with x and y exchanged limits rows. Evaluated in binary64 from the
-// binary32 operands; pixels left out lie at least one pixel beyond the
-// bound. Rows are bounded first (over the block's columns), then columns
-// over the rows that remain, so a block covering part of a tile can drop a
-// record before looking at its columns.
-struct SupportForm {
-  bool bounded;  // positive definite with a finite cutoff
-  double c0, c1a, c2, det, two_t;
-};
-GS_DEV SupportForm support_form(float c0, float c1, float c2, float thr) {
-  SupportForm f;
-  f.c0 = c0;
-  f.c1a = fabs((double)c1);
-  f.c2 = c2;
-  f.det = f.c0 * f.c2 - (double)c1 * (double)c1;
-  double T = thr;
-  f.bounded = f.det > 0.0 && f.c0 > 0.0 && f.c2 > 0.0 && T >= 0.0 && T < 1e30;
-  f.two_t = 2.0 * T * (1.0 + 1e-6);
-  return f;
-}
-// Narrows [p0, p1] along one axis (coefficient cm, centre pc) given that the
-// other axis (coefficient co, centre oc) only takes pixels o0..o1. With
-// hx^2 = (two_t + g (co + |c1|) O^2) co / (det - g (cm + |c1|) co): the
-// quotient and root are taken in binary32 (a few roundings, < 1e-6
-// relative) and widened by a 1e-5 relative margin.
-GS_DEV void support_axis(const SupportForm& f, double cm, double co, float pc, float oc, int o0,
-                         int o1, int& p0, int& p1) {
-  const double g = 3.0e-7;
-  double O = fmax(fabs(o0 + 0.5 - (double)oc), fabs(o1 + 0.5 - (double)oc)) + 1.0;
-  double n = f.det - g * (cm + f.c1a) * co;
-  if (n > 0.0) {
-    float h2 = (float)((f.two_t + g * (co + f.c1a) * O * O) * co) / (float)n;
-    double h = (double)gs_sqrtf(h2) * (1.0 + 1e-5) + 1e-6;
-    double lo = floor((double)pc - h - 0.5), hi = ceil((double)pc + h - 0.5);
-    if (lo > (double)p0) p0 = lo > (double)p1 ? p1 + 1 : (int)lo;
-    if (hi < (double)p1) p1 = hi < (double)p0 ? p0 - 1 : (int)hi;
-  }
+// bound with x and y exchanged limits rows. The per-Gaussian terms come
+// from the projection (support_terms, binary64 rounded once); here
+// h^2 = a + b O^2 and the root are taken in binary32 (a few roundings,
+// < 1e-6 relative) and widened by a 1e-5 relative margin, and pixels left
+// out lie at least one pixel beyond the bound (which also covers the
+// rounding of the pixel bounds). Rows are bounded first (over the block's
+// columns), then columns over the rows that remain, so a block covering
+// part of a tile can drop a record before looking at its columns.
+//
+// Narrows [p0, p1] along one axis (centre pc, constant term a) given that
+// the other axis (centre oc) only takes pixels o0..o1.
+GS_DEV void support_axis(float a, float b, float pc, float oc, int o0, int o1, int& p0,
+                         int& p1) {
+  const float O = gs_maxf(fabsf(((float)o0 + 0.5f) - oc), fabsf(((float)o1 + 0.5f) - oc)) + 1.0f;
+  const float h = gs_sqrtf(a + b * (O * O)) * (1.0f + 1e-5f) + 1e-6f;
+  const float lo = gs_floorf(pc - h - 0.5f), hi = gs_ceilf(pc + h - 0.5f);
+  if (lo > (float)p0) p0 = lo > (float)p1 ? p1 + 1 : (int)lo;
+  if (hi < (float)p1) p1 = hi < (float)p0 ? p0 - 1 : (int)hi;
 }
 
 // The reference's per-pair power (TileRasterizer.ts:624-656), exactly.
@@ -1356,29 +1374,29 @@ GS_DEV Candidate eval_candidate(const Lds& L, uint32_t j, float pxc, float pyc, 
 struct Rect {
   int x0, x1, y0, y1;
 };
-// The support rectangle (above) of the record with mean m and conic c
-// (c.w = ln(255 opacity)) within columns tx0..tx0+E-1 and rows ry0..ry1;
-// under the periodic boundary policies every pixel there is a candidate.
+// The support rectangle (above) of the record with mean m and support
+// terms sup = (ln(255 opacity), rows' term, columns' term, O^2 coefficient)
+// within columns tx0..tx0+E-1 and rows ry0..ry1; under the periodic
+// boundary policies every pixel there is a candidate.
 template <int kEdge>
-GS_DEV Rect record_rect(float4 m, float4 c, int tx0, int ry0, int ry1, uint32_t axis) {
+GS_DEV Rect record_rect(float4 m, float4 sup, int tx0, int ry0, int ry1, uint32_t axis) {
   Rect r = {tx0, tx0 + kEdge - 1, ry0, ry1};
   if (axis != 0) return r;
-  SupportForm f = support_form(c.x, c.y, c.z, c.w);
-  if (!f.bounded) return r;
-  support_axis(f, f.c2, f.c0, m.y, m.x, r.x0, r.x1, r.y0, r.y1);  // rows
+  support_axis(sup.y, sup.w, m.y, m.x, r.x0, r.x1, r.y0, r.y1);  // rows
   if (r.y0 > r.y1) return r;
-  support_axis(f, f.c0, f.c2, m.x, m.y, r.y0, r.y1, r.x0, r.x1);  // columns
+  support_axis(sup.z, sup.w, m.x, m.y, r.y0, r.y1, r.x0, r.x1);  // columns
   return r;
 }
 
 // Splat fields a raster round stages: mean, (conic.xyz, ln(255 opacity)),
-// colour.
+// colour, and the support terms.
 GS_DEV void gather_splat(const Splat* __restrict__ splat, uint32_t id, float4& m, float4& c,
-                         float4& k) {
+                         float4& k, float4& sup) {
   const Splat& g = splat[id];
   m = g.mean;
   float4 cn = g.conic;
-  c = make_float4(cn.x, cn.y, cn.z, g.aux.x);
+  sup = g.aux;
+  c = make_float4(cn.x, cn.y, cn.z, sup.x);
   k = g.color;
 }
 
@@ -1472,10 +1490,10 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
   // staged): unconditional loads land in the loop-carried registers, where
   // a conditional one makes the compiler wait for the load at the join.
   const uint32_t last = ce - 1u;
-  float4 m, c, k;
+  float4 m, c, k, sp;
   uint32_t cg = sorted[min(cb + t, last)].y;
   uint32_t cg_next = sorted[min(cb + kBatch + t, last)].y;
-  gather_splat(splat, cg, m, c, k);
+  gather_splat(splat, cg, m, c, k, sp);
   __syncthreads();
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
     for (uint32_t i = t; i < (uint32_t)(kMaskWords * kPixels); i += kThreads)
@@ -1484,7 +1502,7 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
     bool hit = stager && batch + t < ce;
     Rect crc = {0, -1, 0, -1};
     if (hit) {
-      crc = record_rect<kEdge>(m, c, tx0, ty0, ty0 + kEdge - 1, axis);
+      crc = record_rect<kEdge>(m, sp, tx0, ty0, ty0 + kEdge - 1, axis);
       hit = crc.x0 <= crc.x1 && crc.y0 <= crc.y1;
     }
     if (hit) {
@@ -1495,7 +1513,7 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
     }
     float4 cm = m, cc = c;
     cg = cg_next;
-    gather_splat(splat, cg, m, c, k);
+    gather_splat(splat, cg, m, c, k, sp);
     cg_next = sorted[min(batch + 2u * kBatch + t, last)].y;
     if (hit) mark_candidates<kEdge>(L, t, crc, tx0, ty0, cm, cc, vw, vh, axis);
     __syncthreads();
@@ -1628,12 +1646,12 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
   // is issued (to move its result at the join).
   const uint32_t last = ce - 1u;
   uint32_t cg[kPer], cg_next[kPer];
-  float4 m[kPer], c[kPer], k[kPer];
+  float4 m[kPer], c[kPer], k[kPer], sp[kPer];
 #pragma unroll
   for (int e = 0; e < kPer; e++) {
     uint32_t r = cb + e * kThreads + t;
     cg[e] = sorted[min(r, last)].y;
-    gather_splat(splat, cg[e], m[e], c[e], k[e]);
+    gather_splat(splat, cg[e], m[e], c[e], k[e], sp[e]);
     cg_next[e] = sorted[min(r + kBatch, last)].y;
   }
   for (uint32_t batch = cb; batch < ce; batch += kBatch) {
@@ -1649,7 +1667,7 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
       hit[e] = batch + slot < ce;
       crc[e] = {0, -1, 0, -1};
       if (hit[e]) {
-        crc[e] = record_rect<kEdge>(m[e], c[e], tx0, sy0, sy0 + kSlabRows - 1, axis);
+        crc[e] = record_rect<kEdge>(m[e], sp[e], tx0, sy0, sy0 + kSlabRows - 1, axis);
         hit[e] = crc[e].x0 <= crc[e].x1 && crc[e].y0 <= crc[e].y1;
       }
       if (hit[e]) {
@@ -1661,7 +1679,7 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
       cm[e] = m[e];
       cc[e] = c[e];
       cg[e] = cg_next[e];
-      gather_splat(splat, cg[e], m[e], c[e], k[e]);
+      gather_splat(splat, cg[e], m[e], c[e], k[e], sp[e]);
       cg_next[e] = sorted[min(batch + 2u * kBatch + slot, last)].y;
     }
 #pragma unroll
