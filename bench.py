@@ -133,16 +133,17 @@ def batched(gs, reps=10):
     return out
 
 
-def learned_cull(gs, train_views=512, eval_views=64, reps=3):
+def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     """Train the cull on orbit views of the 3M-Gaussian scene, calibrate its
-    output bias on 16 training views (mean PSNR >= 46.5 dB against plain,
-    1.5 dB above the 45 dB bar for the held-out views),
-    then time held-out views batched with and without it."""
+    output bias on validation views that are neither trained on nor timed
+    (mean PSNR >= 46 dB against plain, 1 dB above the 45 dB bar for the
+    held-out views), then time held-out views batched with and without it."""
     w = gs.WORKLOADS["3m_cull"]
     scene = gs.workload_scene("3m_cull")
     res = [(w.width, w.height)]
     train = gs.orbit_cameras(train_views, seed=5001, radius=w.orbit, resolutions=res)
     held = gs.orbit_cameras(eval_views, seed=5002, radius=w.orbit, resolutions=res)
+    valid = gs.orbit_cameras(calib_views, seed=5003, radius=w.orbit, resolutions=res)
     b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 24})
     cull = gs.LearnedCull(b, seed=3)
     cull.reserve(100_000_000)
@@ -160,8 +161,8 @@ def learned_cull(gs, train_views=512, eval_views=64, reps=3):
     loss = cull.train(5000, batch=65536, lr=1e-4, under_weight=300.0, seed=3)
     train_s = time.perf_counter() - t0
     progress("learned cull: calibrating and evaluating")
-    shift, rows = cull.calibrate([c for c, _ in train[:16]], [s for _, s in train[:16]],
-                                 target_psnr=46.5)
+    shift, rows = cull.calibrate([c for c, _ in valid], [s for _, s in valid],
+                                 target_psnr=46.0)
     cams, sizes = [c for c, _ in held], [s for _, s in held]
 
     def timed():
@@ -188,7 +189,7 @@ def learned_cull(gs, train_views=512, eval_views=64, reps=3):
     b.dispose()
     return {
         "workload": w.name, "gaussians": w.count, "resolution": f"{w.width}x{w.height}",
-        "train_views": train_views, "eval_views": eval_views, "threshold": 1e-4,
+        "train_views": train_views, "eval_views": eval_views, "calibration_views": calib_views, "threshold": 1e-4,
         "fps_plain_batched": round(len(cams) / t_plain, 1),
         "fps_cull_batched": round(len(cams) / t_cull, 1),
         "speedup": round(t_plain / t_cull, 3),
