@@ -7,8 +7,8 @@ default) are dropped before duplication and sort. Its training targets are
 recorded by the plain rasterizer itself (an instrumented frame raises a
 per-Gaussian record to every alpha * T it blends), over seeded orbit views;
 training runs the library's own forward and backward kernels with Adam.
-The plain pipeline is both the fallback (cull disabled) and the judge: the
-image loss of the cull is measured as PSNR against the plain frame.
+The plain pipeline is both the fallback (cull disabled) and the yardstick:
+the image loss of the cull is measured as PSNR against the plain frame.
 
 No counterpart in the reference; the nearest mechanisms there are the
 opacity and subpixel culls (ProjectionStage.ts:533, 572-587).
