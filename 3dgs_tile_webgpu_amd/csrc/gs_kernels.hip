@@ -1864,7 +1864,10 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
   if (kEdge == 16 && state->heavy_split != 0u) return;
   for (uint32_t tile = blockIdx.x; tile < fp.tile_count; tile += gridDim.x) {
     uint32_t begin = off[tile], end = off[tile + 1];
-    if (end - begin > kLightMax) continue;  // heavy unit (block-uniform)
+    // Heavy units and chunked tiles (chunk sizes below kLightMax make some
+    // tiles of at most kLightMax records chunked) are not light (block-uniform).
+    if (end - begin > kLightMax || (fp.chunk_size != 0u && end - begin > fp.chunk_size))
+      continue;
     light_tile<kEdge, kBatch, kRecord>(fp, state, tile, begin, end, sorted, splat, lds,
                                        color_out, depth_out, record);
   }

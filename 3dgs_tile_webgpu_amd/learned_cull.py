@@ -26,6 +26,10 @@ from .gaussian_pass import frame_for, object_frames_for
 
 PARAMS = 5313
 FEATURES = 16
+HIDDEN = 64
+# Feature indices (the order of include/gs.h).
+FEATURE_LOG_RADIUS = 0
+FEATURE_OPACITY = 4
 
 
 class LearnedCull:
@@ -135,7 +139,9 @@ class LearnedCull:
         """Move the network's output bias by the lowest shift (most culling)
         whose culled frames of these cameras (a BatchedPass owner) keep a
         mean PSNR of at least target_psnr against their plain frames. Returns
-        the shift and (shift, key fraction kept, mean PSNR, min PSNR) rows."""
+        the shift and (shift, key fraction kept, mean PSNR, min PSNR) rows.
+        When no shift reaches the target the shift is None, the parameters
+        are left as they were and the cull is disabled."""
         b = self.owner
         was = self.enabled
         self.disable()
@@ -144,7 +150,7 @@ class LearnedCull:
         keys = sum(s.intersections for s in b.read_states())
         p0 = self.params()
         rows = []
-        chosen = shifts[-1]
+        chosen = None
         for shift in shifts:
             p = p0.copy()
             p[PARAMS - 1] += shift
@@ -158,11 +164,18 @@ class LearnedCull:
                 chosen = shift
                 break
         p = p0.copy()
-        p[PARAMS - 1] += chosen
+        if chosen is not None:
+            p[PARAMS - 1] += chosen
         self.set_params(p)
-        if not was:
+        if not was or chosen is None:
             self.disable()
         return chosen, rows
+
+    def set_linear_score(self, feature: int, scale: float, bias: float) -> None:
+        """A non-learned cull on one feature (see linear_score_params), with
+        the current normalisation."""
+        mu, inv = self.normalization()
+        self.set_params(linear_score_params(feature, scale, bias, mu, inv))
 
     def predict(self, features: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(features, np.float32).reshape(-1, FEATURES)
@@ -181,6 +194,27 @@ class LearnedCull:
         N.check(N.lib().gs_cull_gradient(self._ctx, f.shape[0], f.ctypes.data, t.ctypes.data,
                                          under_weight, g.ctypes.data, C.byref(loss)), self._ctx)
         return g, loss.value
+
+
+def linear_score_params(feature: int, scale: float, bias: float, mu: np.ndarray,
+                        inv_sigma: np.ndarray) -> np.ndarray:
+    """Parameters under which the network outputs scale * f[feature] + bias
+    (f the raw feature; the bf16 rounding of the standardised input and of
+    the output weights aside). With scale > 0 the cull then drops the
+    Gaussians whose feature lies below (log(threshold) - bias) / scale: a
+    non-learned cull (an opacity or footprint bound) to compare the trained
+    network with. Two hidden units carry x and -x through the ReLUs."""
+    off_w2 = HIDDEN * FEATURES + HIDDEN
+    off_w3 = off_w2 + HIDDEN * HIDDEN + HIDDEN
+    p = np.zeros(PARAMS, np.float32)
+    p[0 * FEATURES + feature] = 1.0        # W1: hidden 0 = x, hidden 1 = -x
+    p[1 * FEATURES + feature] = -1.0
+    p[off_w2 + 0 * HIDDEN + 0] = 1.0       # W2: identity on the two units
+    p[off_w2 + 1 * HIDDEN + 1] = 1.0
+    p[off_w3 + 0] = scale / float(inv_sigma[feature])
+    p[off_w3 + 1] = -scale / float(inv_sigma[feature])
+    p[PARAMS - 1] = scale * float(mu[feature]) + bias
+    return p
 
 
 def psnr(a: np.ndarray, b: np.ndarray) -> float:

@@ -322,8 +322,8 @@ bool subpixel_has_sample(V2 center, V3 conic, float thr, V2 extent, uint32_t vx,
   return false;
 }
 
-// kernels/tileContribution.ts:271-313 (tile/ellipse test) inside
-// kernels/tileContribution.ts:339-367 (gaussian_boundary_tile_hit)
+// kernels/tileContribution.ts:16-59 (tileContributionWGSL: the tile/ellipse
+// test) inside kernels/tileContribution.ts:84-114 (gaussian_boundary_tile_hit)
 bool tile_hit(V2 center, V3 conic, float thr, V2 radius, int tx, int ty, V2 viewport,
               uint32_t axis, float ts = 16.0f) {
   V2 cc = center;
@@ -377,7 +377,7 @@ uint32_t count_tiles(V2 center, V3 conic, float thr, V2 radius, int minx, int mi
   return n;
 }
 
-// kernels/tileContribution.ts:321-337 (gaussian_boundary_delta)
+// kernels/tileContribution.ts:66-82 (gaussian_boundary_delta)
 V2 boundary_delta(V2 pixel, V2 center, V3 conic, V2 viewport, uint32_t axis) {
   V2 cc = center;
   if (axis == 1) cc.x = cc.x - gs_floorf(center.x / viewport.x) * viewport.x;
@@ -478,7 +478,7 @@ struct gs_oracle_result {
   std::vector<uint32_t> tile_counts, visible_offsets, ordered_counts,
       intersection_offsets, tile_offsets;
   std::vector<uint64_t> depth_records, emitted, sorted;
-  std::vector<float> image, depth_image, final_t;
+  std::vector<float> image, depth_image, final_t, depth_target;
   std::vector<uint16_t> image_half;
   // Learned-cull training data (scene.cull_outputs).
   std::vector<float> cull_features;
@@ -686,10 +686,46 @@ PixelAccum blend_range(const gs_oracle_result& r, uint32_t begin, uint32_t end,
   return a;
 }
 
+// pipeline/finalizeGaussianDepth.ts:18-68 for one pixel, with the default
+// depth node of nodes/GaussianContextNodes.ts:125-131 (encoding 0); the
+// other encodings are the custom nodes test/ScalarOutput.test.ts:61-96 sets.
+float finalize_depth(float weighted, float t, float alpha_threshold, float invalid,
+                     float near_plane, float far_plane, uint32_t encoding) {
+  float alpha = 1.0f - t;
+  float depth = 0.0f;
+  if (alpha > 0.0f) depth = weighted / alpha;  // never 0 / 0
+  auto finite = [](float v) { return v == v && gs_fabsf(v) <= kMaxF32; };
+  bool scalar_valid = alpha > 0.0f && finite(weighted) && finite(depth) && depth > 0.0f;
+  bool valid = scalar_valid && alpha > alpha_threshold;  // rasterOutputValid
+  float out = invalid;
+  if (valid) {
+    float encoded;
+    if (encoding == 0) {
+      float view_z = -depth;
+      encoded = gs_clampf((near_plane + view_z) * far_plane / ((far_plane - near_plane) * view_z),
+                          0.0f, 1.0f);
+    } else if (encoding == 1) {
+      encoded = depth;
+    } else if (encoding == 2) {
+      volatile float zero = 0.0f;
+      encoded = depth / zero;
+    } else {
+      encoded = alpha;
+    }
+    if (finite(encoded)) out = encoded;
+  }
+  return out;
+}
+
 // pipeline/TileRasterizer.ts:477-805 (direct and chunk targets) and :807-902
 // (ordered chunk composite), :930-964 (storeFinalPixel) for one tile.
-void raster_tile(gs_oracle_result& r, uint32_t tile, uint64_t& checked,
-                 uint64_t& blended, uint32_t& chunk_tasks) {
+// `checked` and `blended` follow the reference's counters (:617, :750-768):
+// each chunk task of a chunked tile blends its whole chunk from T = 1, so
+// every chunk is counted in full, even those the composite's early stop
+// never reads. Tiles above 1024 records or above the chunk size also add
+// their checked pairs to `checked_heavy`.
+void raster_tile(gs_oracle_result& r, uint32_t tile, uint64_t& checked, uint64_t& blended,
+                 uint64_t& checked_heavy, uint32_t& chunk_tasks) {
   const gs_oracle_scene& s = r.scene;
   uint32_t tiles_x = r.counts.tiles_x;
   uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
@@ -697,10 +733,12 @@ void raster_tile(gs_oracle_result& r, uint32_t tile, uint64_t& checked,
   uint32_t count = r.tile_offsets[tile + 1] - begin;
   uint32_t cs = s.raster_chunk_size;
   bool chunked = cs != 0 && count > cs;
+  bool heavy = chunked || count > 1024u;
   uint32_t chunks = chunked ? (count + cs - 1) / cs : 0;
   chunk_tasks += chunks;
   float bg_a = gs_clampf(s.background[3], 0.0f, 1.0f);
   uint32_t edge = tile_edge(s);
+  uint64_t tile_checked = 0;
   for (uint32_t ly = 0; ly < edge; ly++)
     for (uint32_t lx = 0; lx < edge; lx++) {
       uint32_t x = tx * edge + lx, y = ty * edge + ly;
@@ -712,20 +750,22 @@ void raster_tile(gs_oracle_result& r, uint32_t tile, uint64_t& checked,
         PixelAccum a = blend_range(r, begin, begin + count, px, py);
         c[0] = a.c[0]; c[1] = a.c[1]; c[2] = a.c[2];
         t = a.t; d = a.d;
-        checked += a.checked;
+        tile_checked += a.checked;
         blended += a.blended;
       } else {
+        bool stopped = false;
         for (uint32_t k = 0; k < chunks; k++) {
           uint32_t b = k * cs, e = std::min(b + cs, count);
           PixelAccum a = blend_range(r, begin + b, begin + e, px, py);
-          checked += a.checked;
+          tile_checked += a.checked;
           blended += a.blended;
+          if (stopped) continue;  // the composite has stopped reading partials
           c[0] = c[0] + a.c[0] * t;
           c[1] = c[1] + a.c[1] * t;
           c[2] = c[2] + a.c[2] * t;
           d = d + a.d * t;
           t = t * a.t;
-          if (t < s.transmittance_threshold) break;
+          if (t < s.transmittance_threshold) stopped = true;
         }
       }
       size_t o = ((size_t)y * r.width + x) * 4;
@@ -738,7 +778,13 @@ void raster_tile(gs_oracle_result& r, uint32_t tile, uint64_t& checked,
       }
       r.depth_image[(size_t)y * r.width + x] = d;
       r.final_t[(size_t)y * r.width + x] = t;
+      if (s.output_depth)
+        r.depth_target[(size_t)y * r.width + x] =
+            finalize_depth(d, t, s.depth_alpha_threshold, s.invalid_depth_value, s.near_plane,
+                           s.far_plane, 0u);
     }
+  checked += tile_checked;
+  if (heavy) checked_heavy += tile_checked;
 }
 
 }  // namespace
@@ -865,15 +911,19 @@ extern "C" gs_oracle_result* gs_oracle_run(const gs_oracle_scene* sp) {
   r.image_half.assign((size_t)r.width * r.height * 4, 0);
   r.depth_image.assign((size_t)r.width * r.height, 0.0f);
   r.final_t.assign((size_t)r.width * r.height, 1.0f);
+  if (s.output_depth)
+    r.depth_target.assign((size_t)r.width * r.height, s.invalid_depth_value);
   if (!r.counts.overflow) {
-    std::vector<uint64_t> ch(threads, 0), bl(threads, 0);
+    std::vector<uint64_t> ch(threads, 0), bl(threads, 0), hv(threads, 0);
     std::vector<uint32_t> tasks(threads, 0);
     parallel_for(threads, tiles, [&](size_t b, size_t e, uint32_t th) {
-      for (size_t t = b; t < e; t++) raster_tile(r, (uint32_t)t, ch[th], bl[th], tasks[th]);
+      for (size_t t = b; t < e; t++)
+        raster_tile(r, (uint32_t)t, ch[th], bl[th], hv[th], tasks[th]);
     });
     for (uint32_t i = 0; i < threads; i++) {
       r.counts.checked += ch[i];
       r.counts.blended += bl[i];
+      r.counts.checked_heavy += hv[i];
       r.counts.chunk_tasks += tasks[i];
     }
   }
@@ -923,6 +973,7 @@ extern "C" size_t gs_oracle_get(const gs_oracle_result* r, const char* name, voi
   if (n == "image_f16") return GS_VEC(image_half);
   if (n == "depth_f32") return GS_VEC(depth_image);
   if (n == "final_t") return GS_VEC(final_t);
+  if (n == "depth_target") return GS_VEC(depth_target);
   if (n == "cull_features") return GS_VEC(cull_features);
   if (n == "max_contribution") {
     size_t cnt = r->scene.cull_outputs ? r->scene.active_count : 0;
@@ -1105,6 +1156,19 @@ extern "C" void gs_oracle_decode_rotation(uint32_t packed, float out[4]) {
 extern "C" void gs_oracle_decode_sh_rgb8e8(uint32_t packed, float out[3]) {
   V3 v = decode_sh(packed);
   out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+extern "C" void gs_oracle_evaluate_sh(const uint32_t* words, uint32_t degree, const float dir[3],
+                                      float out[3]) {
+  V3 v = evaluate_sh(words, 0u, degree, V3{dir[0], dir[1], dir[2]});
+  out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+extern "C" float gs_oracle_finalize_depth(float weighted_depth, float transmittance,
+                                          float depth_alpha_threshold, float invalid_value,
+                                          float near_plane, float far_plane, uint32_t encoding) {
+  return finalize_depth(weighted_depth, transmittance, depth_alpha_threshold, invalid_value,
+                        near_plane, far_plane, encoding);
 }
 
 // Elementwise evaluation of the shared elementary functions, for the

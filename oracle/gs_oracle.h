@@ -49,6 +49,11 @@ typedef struct gs_oracle_scene {
   uint32_t cull_outputs;          /* also produce cull_features (n * 16 f32) and
                                      max_contribution (n u32: binary32 bits of each
                                      Gaussian's largest alpha * T) */
+  uint32_t output_depth;          /* also produce depth_target (W*H f32): the
+                                     finalized scalar of finalizeGaussianDepth.ts
+                                     with the default depth node */
+  float depth_alpha_threshold;    /* default 0.95 (GaussianPass.ts:243) */
+  float invalid_depth_value;      /* default 1 */
 } gs_oracle_scene;
 
 typedef struct gs_oracle_counts {
@@ -59,10 +64,15 @@ typedef struct gs_oracle_counts {
   uint32_t tiles_x, tiles_y;
   uint32_t sentinels;    /* padded records (must be 0) */
   uint32_t chunk_tasks;
-  uint64_t checked;      /* raster (pixel, Gaussian) pairs evaluated */
+  uint64_t checked;      /* raster (pixel, Gaussian) pairs evaluated: the
+                            reference's `checked` counter, every chunk task
+                            of a chunked tile counted in full */
   uint64_t blended;      /* pairs that blended */
   double stage_seconds[8]; /* project, visible, depth sort, scan, emit,
                               tile sort, offsets, raster */
+  uint64_t checked_heavy; /* the part of `checked` in tiles above 1024
+                             records or above the chunk size (the split the
+                             GPU's heavy-unit counter uses) */
 } gs_oracle_counts;
 
 typedef struct gs_oracle_result gs_oracle_result;
@@ -74,7 +84,8 @@ void gs_oracle_counts_of(const gs_oracle_result* r, gs_oracle_counts* out);
  * Names: projected_mean, projected_conic, projected_color, tile_counts,
  * visible_offsets, depth_records, ordered_counts, intersection_offsets,
  * emitted_records, sorted_records, tile_offsets, image_f32, image_f16,
- * depth_f32 (per-pixel weighted view depth), final_t.
+ * depth_f32 (per-pixel weighted view depth), final_t, depth_target
+ * (finalized depth, with output_depth).
  * Returns the byte size, or 0 if unknown; with dst == NULL only the size. */
 size_t gs_oracle_get(const gs_oracle_result* r, const char* name, void* dst,
                      size_t bytes);
@@ -117,6 +128,17 @@ void gs_oracle_boundary_delta(const float pixel[2], const float center[2],
 void gs_oracle_decode_scale_opacity(uint32_t x, uint32_t y, float out[4]);
 void gs_oracle_decode_rotation(uint32_t packed, float out[4]);
 void gs_oracle_decode_sh_rgb8e8(uint32_t packed, float out[3]);
+/* evaluate_gaussian_sh_rgb8e8 for one Gaussian's (deg+1)^2 words and a unit
+ * direction: clamp(sum + 0.5, 0, 1) per channel. */
+void gs_oracle_evaluate_sh(const uint32_t* words, uint32_t degree, const float dir[3],
+                           float out[3]);
+/* finalizeGaussianDepth for one pixel. encoding 0 is the default depth node
+ * (clamp((near + viewZ) far / ((far - near) viewZ), 0, 1), viewZ = -depth),
+ * 1 the raw scalar (rasterOutputDepth), 2 a non-finite encoding
+ * (rasterOutputDepth / 0), 3 the accumulated alpha. */
+float gs_oracle_finalize_depth(float weighted_depth, float transmittance,
+                               float depth_alpha_threshold, float invalid_value,
+                               float near_plane, float far_plane, uint32_t encoding);
 /* Elementwise shared elementary functions (accuracy tests). */
 void gs_oracle_math(int fn, const float* a, const float* b, float* out, size_t n);
 

@@ -29,6 +29,7 @@ class OracleScene(C.Structure):
         ("antialias_compensated", u32), ("depth_mode", u32), ("raster_chunk_size", u32),
         ("capacity", u32), ("transmittance_threshold", f32), ("background", f32 * 4),
         ("threads", u32), ("tile_size", u32), ("cull_outputs", u32),
+        ("output_depth", u32), ("depth_alpha_threshold", f32), ("invalid_depth_value", f32),
     ]
 
 
@@ -37,6 +38,7 @@ class OracleCounts(C.Structure):
         ("visible", u32), ("total", u32), ("count", u32), ("overflow", u32),
         ("tiles_x", u32), ("tiles_y", u32), ("sentinels", u32), ("chunk_tasks", u32),
         ("checked", C.c_uint64), ("blended", C.c_uint64), ("stage_seconds", C.c_double * 8),
+        ("checked_heavy", C.c_uint64),
     ]
 
 
@@ -78,6 +80,10 @@ def lib():
         h.gs_oracle_decode_scale_opacity.argtypes = [u32, u32, P]
         h.gs_oracle_decode_rotation.argtypes = [u32, P]
         h.gs_oracle_decode_sh_rgb8e8.argtypes = [u32, P]
+        h.gs_oracle_evaluate_sh.restype = None
+        h.gs_oracle_evaluate_sh.argtypes = [P, u32, P, P]
+        h.gs_oracle_finalize_depth.restype = f32
+        h.gs_oracle_finalize_depth.argtypes = [f32, f32, f32, f32, f32, f32, u32]
         _LIB = h
     return _LIB
 
@@ -97,6 +103,7 @@ class OracleResult:
         "depth_records": (np.uint32, 2), "emitted_records": (np.uint32, 2),
         "sorted_records": (np.uint32, 2), "image_f32": (np.float32, 4),
         "image_f16": (np.uint16, 4), "depth_f32": (np.float32, 0), "final_t": (np.float32, 0),
+        "depth_target": (np.float32, 0),
         "cull_features": (np.float32, 16), "max_contribution": (np.uint32, 0),
     }
 
@@ -118,7 +125,7 @@ class OracleResult:
             assert got == size
         if name.startswith("image") :
             out = out.reshape(self.height, self.width, 4)
-        elif name in ("depth_f32", "final_t"):
+        elif name in ("depth_f32", "final_t", "depth_target"):
             out = out.reshape(self.height, self.width)
         return out
 
@@ -135,7 +142,8 @@ class OracleResult:
 
 def run_frame(scene, frame, objects, *, capacity, subpixel=True, compensated=True,
               depth_mode=0, chunk_size=8192, transmittance=1e-4, background=(0, 0, 0, 0),
-              threads=0, tile_size=16, cull_outputs=False) -> OracleResult:
+              threads=0, tile_size=16, cull_outputs=False, output_depth=False,
+              depth_alpha_threshold=0.95, invalid_depth_value=1.0) -> OracleResult:
     """Run the oracle on a PackedScene with the exact f32 uniforms of `frame`
     (a GsFrame) and `objects` ((k, 40) f32 object table)."""
     keep = [np.ascontiguousarray(scene.means, np.float32),
@@ -170,7 +178,26 @@ def run_frame(scene, frame, objects, *, capacity, subpixel=True, compensated=Tru
     s.threads = threads
     s.tile_size = tile_size
     s.cull_outputs = 1 if cull_outputs else 0
+    s.output_depth = 1 if output_depth else 0
+    s.depth_alpha_threshold = depth_alpha_threshold
+    s.invalid_depth_value = invalid_depth_value
     h = lib().gs_oracle_run(C.byref(s))
     if not h:
         raise ValueError("oracle rejected the scene")
     return OracleResult(h, frame.width, frame.height, keep)
+
+
+def evaluate_sh(words, degree: int, direction) -> np.ndarray:
+    """evaluate_gaussian_sh_rgb8e8 of one Gaussian: clamp(sum + 0.5, 0, 1)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.empty(3, np.float32)
+    lib().gs_oracle_evaluate_sh(w.ctypes.data, degree, d.ctypes.data, out.ctypes.data)
+    return out
+
+
+def finalize_depth(weighted, transmittance, *, alpha_threshold=0.95, invalid=1.0, near=0.1,
+                   far=1000.0, encoding=0) -> float:
+    """finalizeGaussianDepth for one pixel (encodings: see gs_oracle.h)."""
+    return float(lib().gs_oracle_finalize_depth(weighted, transmittance, alpha_threshold, invalid,
+                                                near, far, encoding))

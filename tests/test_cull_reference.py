@@ -8,6 +8,7 @@ projected outputs.
 """
 from __future__ import annotations
 
+import importlib
 import math
 
 import numpy as np
@@ -123,3 +124,22 @@ def test_oracle_features_consistent_with_projection(gs, oracle):
     assert np.all(rec[vis] <= np.minimum(pm[vis, 3], 0.99) + 1e-7)
     assert np.all(rec[~vis] == 0)
     assert (rec[vis] > 0).mean() > 0.3
+
+
+@pytest.mark.parametrize("feature,scale,bias", [(4, 10.0, -12.0), (0, 3.0, -5.0)])
+def test_linear_score_params_compute_the_linear_score(gs, feature, scale, bias):
+    """The non-learned baseline cull's parameters make the network (the
+    numpy reference at the kernels' bf16 rounding points) output
+    scale * feature + bias, to bf16 precision of the standardised input."""
+    lc = importlib.import_module("3dgs_tile_webgpu_amd.learned_cull")
+    rng = np.random.default_rng(8)
+    mu = rng.normal(0, 1, 16).astype(np.float32)
+    inv = np.exp(rng.normal(0, 0.5, 16)).astype(np.float32)
+    feats = (mu + rng.normal(0, 1, (500, 16)) / inv).astype(np.float32)
+    p = lc.linear_score_params(feature, scale, bias, mu, inv)
+    y = R.forward(p, mu, inv, feats)["y"]
+    want = scale * feats[:, feature].astype(np.float64) + bias
+    x = np.abs((feats[:, feature] - mu[feature]) * inv[feature])
+    # two bf16 roundings (input, output weight), unit roundoff 2^-8 each
+    tol = scale / inv[feature] * (x * 2.0**-7 + 1e-6) + 1e-4 * np.abs(want) + 1e-5
+    assert (np.abs(y - want) <= tol).all()

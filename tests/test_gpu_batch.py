@@ -117,3 +117,44 @@ def test_autotune_picks_a_shape_and_stays_exact(gs, oracle, gpu_available, small
         np.testing.assert_array_equal(b.read_image(v).view(np.uint16), ref.get("image_f16"))
         p.dispose()
     b.dispose()
+
+
+def test_c4_batch_at_configured_size(gs, oracle, gpu_available):
+    """The batched workload at its configured size: 2M Gaussians, 64 seeded
+    orbit cameras (radius 12-25) at resolutions drawn from 854x480,
+    1280x720, 1920x1080 and 2560x1440, in one gs_render_batch. Every view
+    ends without overflow or wait errors, publishes the same key count as a
+    single-camera pass of it and the same image bits; one view of each
+    resolution is checked bit for bit against the oracle."""
+    w = gs.WORKLOADS["2m_batch"]
+    scene = gs.workload_scene("2m_batch")
+    views = gs.orbit_cameras(w.views, seed=777, radius=w.orbit, resolutions=gs.BATCH_RESOLUTIONS)
+    cams, sizes = [c for c, _ in views], [s for _, s in views]
+    assert len(cams) == 64 and set(sizes) == set(gs.BATCH_RESOLUTIONS)
+    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22})
+    b.render(cams, sizes)
+    states = b.read_states()
+    assert all(s.overflow == 0 and s.error_flags == 0 for s in states)
+    p = gs.gaussian_pass(cams[0], scene, {"intersectionCapacity": 1 << 23})
+    checked = {}
+    for v, (cam, (x, y)) in enumerate(views):
+        p.camera = cam
+        p.render(x, y)
+        st = p.read_state()
+        assert st.overflow == 0 and st.error_flags == 0
+        assert (states[v].visible, states[v].intersections) == (st.visible, st.intersections), v
+        np.testing.assert_array_equal(b.read_image(v).view(np.uint16),
+                                      p.read_image().view(np.uint16), err_msg=f"view {v}")
+        if (x, y) not in checked:
+            checked[(x, y)] = v
+            frame = gs.frame_for(cam, x, y, p.options)
+            ref = oracle.run_frame(scene, frame, p.object_frames(), capacity=p.capacity)
+            assert ref.counts.count == st.intersections
+            np.testing.assert_array_equal(p.read_debug("sorted_records"), ref.get("sorted_records"))
+            np.testing.assert_array_equal(b.read_image(v).view(np.uint16), ref.get("image_f16"),
+                                          err_msg=f"view {v} ({x}x{y}) against the oracle")
+    print("keys per view:", [s.intersections for s in states][:8], "...",
+          "total", sum(s.intersections for s in states), "oracle views", checked)
+    assert len(checked) == 4
+    p.dispose()
+    b.dispose()

@@ -27,7 +27,10 @@ def oracle_for(gs, oracle, p, scene, camera, width, height, objects=None):
                             depth_mode=0 if o["depthSortMode"] == "float32" else 1,
                             chunk_size=o["rasterChunkSize"] or 0,
                             transmittance=o["rasterTransmittanceThreshold"],
-                            background=o["background"], tile_size=o["tileSize"])
+                            background=o["background"], tile_size=o["tileSize"],
+                            output_depth=bool(o["outputDepth"]),
+                            depth_alpha_threshold=o["depthAlphaThreshold"],
+                            invalid_depth_value=o["invalidDepthValue"])
 
 
 def render_both(gs, oracle, scene, camera, width, height, options=None, capacity=1 << 22,
