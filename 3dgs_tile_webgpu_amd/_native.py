@@ -195,6 +195,9 @@ SIGNATURES = {
     "gs_profile_begin": (C.c_int, [P, u32]),
     "gs_profile_end": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double), C.POINTER(u32)]),
+    "gs_render_noalloc": (C.c_int, [C.POINTER(GsConfig), C.POINTER(GsDataDesc), u32,
+                                    C.POINTER(GsFrame), C.POINTER(GsImage), P,
+                                    C.POINTER(GsState)]),
     "gs_encode_compact": (None, [C.c_size_t, P, P, P, P, P]),
     "gs_encode_quat_oct101012": (u32, [P]),
     "gs_pack_sh_rgb8e8": (u32, [P]),

@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -330,7 +331,7 @@ extern "C" void gs_config_default(gs_config* c) {
 }
 
 // GaussianPass.ts:157-260 option validation.
-static int validate_config(const gs_config* c, std::string& msg) {
+static int validate_config(const gs_config* c, const char*& msg) {
   if (c->depth_sort_mode > GS_DEPTH_PACKED16) {
     msg = "depthSortMode must be float32 or packed16";
     return GS_E_TYPE;
@@ -390,7 +391,7 @@ extern "C" int gs_create(const gs_config* config, gs_ctx** out) {
   gs_config c;
   if (config) c = *config;
   else gs_config_default(&c);
-  std::string msg;
+  const char* msg = nullptr;
   int rc = validate_config(&c, msg);
   if (rc) return rc;
   int devices = 0;
@@ -666,20 +667,27 @@ extern "C" int gs_resize_batch_intersections(gs_ctx* ctx, uint32_t view, uint32_
 namespace {
 
 // TiledGaussianPipeline.ts:286-300 checks on one camera.
+int check_frame(uint32_t ts, const gs_frame* f, const char*& msg) {
+  msg = nullptr;
+  if (f->width < 1 || f->height < 1) msg = "empty render size";
+  else if (div_up(f->width, ts) > 65535 || div_up(f->height, ts) > 65535)
+    msg = "Render size exceeds the tile dispatch limit";
+  else if (!(f->near_plane > 0.0f && f->far_plane > f->near_plane && finite(f->far_plane)))
+    msg = "depthRange must be positive and increasing";
+  else if (!(f->filter_variance >= 0.0f && finite(f->filter_variance)))
+    msg = "filterVariance must be finite and nonnegative";
+  else if (f->boundary_axis > GS_BOUNDARY_REPEAT_Y)
+    msg = "unknown boundaryPolicy";
+  else
+    for (int i = 0; i < 16; i++)
+      if (!finite(f->projection[i])) msg = "projection must be finite";
+  return msg ? GS_E_RANGE : GS_OK;
+}
+
 int validate_frame(gs_ctx* ctx, const gs_frame* f) {
-  if (f->width < 1 || f->height < 1) return fail(ctx, GS_E_RANGE, "empty render size");
-  uint32_t ts = ctx->cfg.tile_size;
-  if (div_up(f->width, ts) > 65535 || div_up(f->height, ts) > 65535)
-    return fail(ctx, GS_E_RANGE, "Render size exceeds the tile dispatch limit");
-  if (!(f->near_plane > 0.0f && f->far_plane > f->near_plane && finite(f->far_plane)))
-    return fail(ctx, GS_E_RANGE, "depthRange must be positive and increasing");
-  if (!(f->filter_variance >= 0.0f && finite(f->filter_variance)))
-    return fail(ctx, GS_E_RANGE, "filterVariance must be finite and nonnegative");
-  if (f->boundary_axis > GS_BOUNDARY_REPEAT_Y)
-    return fail(ctx, GS_E_RANGE, "unknown boundaryPolicy");
-  for (int i = 0; i < 16; i++)
-    if (!finite(f->projection[i])) return fail(ctx, GS_E_RANGE, "projection must be finite");
-  return GS_OK;
+  const char* msg = nullptr;
+  int rc = check_frame(ctx->cfg.tile_size, f, msg);
+  return rc ? fail(ctx, rc, msg) : GS_OK;
 }
 
 // rasterChunks.ts:73-78 (maxRasterChunkTasks): every chunked tile holds
@@ -724,44 +732,41 @@ size_t view_stride(uint32_t object_count) {
   return gs::kCameraBytes + (obj + 255) / 256 * 256;
 }
 
-// Launch description of view v of `set` on lane l. All lane buffers are
-// already sized.
-void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const Lane& l,
-                void* color_dst, void* depth_dst, FrameBuffers& b) {
-  memset(&b, 0, sizeof(b));
-  uint32_t ts = ctx->cfg.tile_size;
+// The frame uniforms of one camera (FrameUniforms.ts:41-58 plus the stage
+// constants) under configuration c on a device with cu_count compute units.
+void fill_params(const gs_config& c, uint32_t cu_count, const gs_frame* f,
+                 uint32_t active_count, uint32_t object_count, uint32_t sh_degree,
+                 bool has_active, uint32_t capacity, const FrameCamera* camera, FrameParams& p) {
+  memset(&p, 0, sizeof(p));
+  uint32_t ts = c.tile_size;
   uint32_t tiles_x = div_up(f->width, ts), tiles_y = div_up(f->height, ts);
-  uint32_t tiles = tiles_x * tiles_y;
-  uint32_t capacity = set.capacity[v];
-  const unsigned char* dyn = (const unsigned char*)set.dynamic.ptr + (size_t)v * set.stride;
-  FrameParams& p = b.params;
-  p.camera = (const FrameCamera*)dyn;
+  p.camera = camera;
   p.viewport[0] = (float)f->width;
   p.viewport[1] = (float)f->height;
   p.inv_viewport[0] = 1.0f / p.viewport[0];
   p.inv_viewport[1] = 1.0f / p.viewport[1];
   p.filter_variance = f->filter_variance;
-  p.transmittance_threshold = ctx->cfg.transmittance_threshold;
-  memcpy(p.background, ctx->cfg.background, sizeof(p.background));
-  p.depth_alpha_threshold = ctx->cfg.depth_alpha_threshold;
-  p.invalid_depth_value = ctx->cfg.invalid_depth_value;
+  p.transmittance_threshold = c.transmittance_threshold;
+  memcpy(p.background, c.background, sizeof(p.background));
+  p.depth_alpha_threshold = c.depth_alpha_threshold;
+  p.invalid_depth_value = c.invalid_depth_value;
   p.boundary_axis = f->boundary_axis;
-  p.active_count = ctx->active_count;
+  p.active_count = active_count;
   p.tiles_x = tiles_x;
   p.tiles_y = tiles_y;
-  p.tile_count = tiles;
+  p.tile_count = tiles_x * tiles_y;
   p.width = f->width;
   p.height = f->height;
   p.capacity = capacity;
-  p.object_count = ctx->object_count;
-  p.sh_degree = ctx->sh_degree;
-  p.depth_mode = ctx->cfg.depth_sort_mode;
-  p.antialias_compensated = ctx->cfg.antialias_mode == GS_AA_COMPENSATED ? 1u : 0u;
-  p.subpixel_culling = ctx->cfg.subpixel_sample_culling ? 1u : 0u;
-  p.chunk_size = ctx->cfg.raster_chunk_size;
-  p.has_active_slots = ctx->has_active ? 1u : 0u;
-  p.output_depth = ctx->cfg.output_depth ? 1u : 0u;
-  p.raster_stats = ctx->cfg.raster_stats ? 1u : 0u;
+  p.object_count = object_count;
+  p.sh_degree = sh_degree;
+  p.depth_mode = c.depth_sort_mode;
+  p.antialias_compensated = c.antialias_mode == GS_AA_COMPENSATED ? 1u : 0u;
+  p.subpixel_culling = c.subpixel_sample_culling ? 1u : 0u;
+  p.chunk_size = c.raster_chunk_size;
+  p.has_active_slots = has_active ? 1u : 0u;
+  p.output_depth = c.output_depth ? 1u : 0u;
+  p.raster_stats = c.raster_stats ? 1u : 0u;
   p.tile_size = ts;
   // Heavy units run as row slabs above 1.25 units per CU: below that, a
   // whole-tile block per unit already covers the CUs and slabs mostly
@@ -771,14 +776,26 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   // threshold, 1,458 at 1.75 units per CU and 1,430 at 2 (the 2M
   // mixed-resolution batch and plain 3M views of 906-1,907 units do not
   // move).
-  p.split_units = ctx->cfg.heavy_split_units == GS_SPLIT_AUTO ? ctx->cu_count * 5 / 4
-                                                               : ctx->cfg.heavy_split_units;
+  p.split_units = c.heavy_split_units == GS_SPLIT_AUTO ? cu_count * 5 / 4 : c.heavy_split_units;
   // Row slabs have 8 rows up to 16 units per CU, 4 above: the 3M/1080p
   // views (906-1,907 units) rasterize 9-13 % faster with 8, the 6M/4K
   // frame (17,773 units) 25 % faster with 4.
-  p.fine_split_units = ctx->cfg.heavy_fine_split_units == GS_SPLIT_AUTO
-                           ? ctx->cu_count * 16
-                           : ctx->cfg.heavy_fine_split_units;
+  p.fine_split_units =
+      c.heavy_fine_split_units == GS_SPLIT_AUTO ? cu_count * 16 : c.heavy_fine_split_units;
+}
+
+// Launch description of view v of `set` on lane l. All lane buffers are
+// already sized.
+void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const Lane& l,
+                void* color_dst, void* depth_dst, FrameBuffers& b) {
+  memset(&b, 0, sizeof(b));
+  uint32_t ts = ctx->cfg.tile_size;
+  uint32_t tiles = div_up(f->width, ts) * div_up(f->height, ts);
+  uint32_t capacity = set.capacity[v];
+  const unsigned char* dyn = (const unsigned char*)set.dynamic.ptr + (size_t)v * set.stride;
+  FrameParams& p = b.params;
+  fill_params(ctx->cfg, ctx->cu_count, f, ctx->active_count, ctx->object_count, ctx->sh_degree,
+              ctx->has_active, capacity, (const FrameCamera*)dyn, p);
   // Training-set frames: chunking off, so every recorded alpha * T uses the
   // true transmittance in front of it.
   if (ctx->cull.collecting) p.chunk_size = 0;
@@ -1384,6 +1401,225 @@ extern "C" int gs_autotune(gs_ctx* ctx, uint32_t n, const gs_frame* frames, uint
   return GS_OK;
 }
 
+
+// ------------------------------------------------------ no-allocation path ----
+//
+// gs_render_noalloc renders one frame like gs_render, for callers that cannot
+// allocate (code under HIP graph capture, tight serving loops): the scene and
+// the output images are the caller's device buffers, and every working array
+// of the frame is carved out of one file-scope device array, `scratch`. Each
+// GPU the library runs on holds its own copy of it from module load until the
+// process ends. Calls on one GPU share that copy: calls that overlap on one
+// GPU (on different streams, or from different threads) race.
+
+namespace {
+
+static_assert(GS_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size is a multiple of 256");
+static __device__ __align__(256) unsigned char scratch[GS_NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` per device: the host-side name is only a
+// stand-in, so kernels get the address hipGetSymbolAddress returns for the
+// current device, taken once per device on its first call.
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int scratch_address(int device, unsigned char** out) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return GS_E_RANGE;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return GS_E_DEVICE;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = (unsigned char*)g_scratch_ptr[device];
+  return GS_OK;
+}
+
+// Offsets of one frame's working arrays inside the scratch (256-byte
+// aligned), as the context path sizes them for one lane.
+struct NoAllocLayout {
+  size_t state, camera, pmean, splat, tile_counts, hits, ordered_hits, visible_offsets, depth_a,
+      depth_b, ordered_counts, intersection_offsets, records_a, records_b, hist, partials,
+      tile_offsets, chunk_first, tasks, partials_px, partial_depth;
+  size_t total;
+  uint32_t sort_parts, scan_parts, task_capacity, unit_capacity;
+};
+
+NoAllocLayout noalloc_layout(const gs_config& c, uint32_t n, uint32_t capacity, uint32_t tiles) {
+  NoAllocLayout L;
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    size_t off = at;
+    at += (std::max<size_t>(bytes, 16) + 255) / 256 * 256;
+    return off;
+  };
+  uint32_t m = std::max(n, capacity);
+  L.sort_parts = std::max<uint32_t>(1u, div_up(m, gs::kSortPartition));
+  L.scan_parts = div_up(m, gs::kScanPartition);
+  L.task_capacity = task_capacity_for(c, capacity);
+  L.unit_capacity = unit_capacity_for(L.task_capacity, capacity, tiles);
+  size_t px = (size_t)c.tile_size * c.tile_size;
+  L.state = take(sizeof(DeviceState));
+  L.camera = take(gs::kCameraBytes);
+  L.pmean = take((size_t)n * 16);
+  L.splat = take((size_t)n * sizeof(gs::Splat));
+  L.tile_counts = take((size_t)n * 4);
+  L.hits = take((size_t)n * 16);
+  L.ordered_hits = take((size_t)n * 16);
+  L.visible_offsets = take((size_t)n * 4);
+  L.depth_a = take((size_t)n * 8);
+  L.depth_b = take((size_t)n * 8);
+  L.ordered_counts = take((size_t)n * 4);
+  L.intersection_offsets = take((size_t)n * 4);
+  L.records_a = take((size_t)capacity * 8);
+  L.records_b = take((size_t)capacity * 8);
+  L.hist = take(((size_t)8 * 256 + (size_t)2 * L.sort_parts * 256) * 4);
+  L.partials = take((size_t)8 * (L.scan_parts + 1));
+  L.tile_offsets = take(((size_t)tiles + 1) * 4);
+  L.chunk_first = take(((size_t)tiles + 1) * 4);
+  L.tasks = take((size_t)L.unit_capacity * 8);
+  L.partials_px = take((size_t)L.task_capacity * px * 16);
+  L.partial_depth = take((size_t)L.task_capacity * px * 4);
+  L.total = at;
+  return L;
+}
+
+// The frame's camera uniforms (passed by value, so a captured launch keeps
+// them) into the scratch, and the parts of the scratch that must start at
+// zero: the frame state and the scan look-back status words.
+__global__ __launch_bounds__(256) void k_noalloc_begin(FrameCamera cam, FrameCamera* cam_out,
+                                                        uint32_t* state_words,
+                                                        uint32_t state_count,
+                                                        unsigned long long* scan_status,
+                                                        uint32_t scan_count) {
+  uint32_t t = threadIdx.x;
+  if (t == 0) *cam_out = cam;
+  for (uint32_t i = t; i < state_count; i += blockDim.x) state_words[i] = 0u;
+  for (uint32_t i = t; i < scan_count; i += blockDim.x) scan_status[i] = 0ull;
+}
+
+}  // namespace
+
+extern "C" int gs_render_noalloc(const gs_config* config, const gs_data_desc* data,
+                                 uint32_t object_count, const gs_frame* frame,
+                                 const gs_image* out, void* stream, gs_state* state) {
+  if (!config || !data || !frame || !out) return GS_E_RANGE;
+  const char* msg = nullptr;  // checks shared with the context path; no message is kept
+  int rc = validate_config(config, msg);
+  if (rc) return rc;
+  const gs_config& c = *config;
+  if (data->geometry_format != GS_GEOMETRY_COMPACT || data->sh_format != GS_SH_RGB8E8 ||
+      data->memory != GS_MEMORY_DEVICE || out->memory != GS_MEMORY_DEVICE)
+    return GS_E_TYPE;
+  if (data->sh_degree > 3 || data->active_count > data->count) return GS_E_RANGE;
+  if (data->active_count > 0 &&
+      (!data->means || !data->scales_opacity || !data->rotations || !data->sh))
+    return GS_E_RANGE;
+  if (!out->color || (c.output_depth && !out->depth)) return GS_E_RANGE;
+  if (object_count > 0 && !frame->object_frames) return GS_E_RANGE;
+  if ((rc = check_frame(c.tile_size, frame, msg))) return rc;
+  uint32_t tiles = div_up(frame->width, c.tile_size) * div_up(frame->height, c.tile_size);
+  uint32_t n = data->active_count, capacity = c.intersection_capacity;
+  NoAllocLayout L = noalloc_layout(c, n, capacity, tiles);
+  if (L.total > (size_t)GS_NOALLOC_SCRATCH_BYTES) return GS_E_RANGE;
+  if (hipSetDevice(c.device) != hipSuccess) {
+    (void)hipGetLastError();
+    return GS_E_DEVICE;
+  }
+  unsigned char* base = nullptr;
+  if ((rc = scratch_address(c.device, &base))) return rc;
+  int cus = 0;
+  uint32_t cu_count = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) ==
+          hipSuccess &&
+      cus > 0)
+    cu_count = (uint32_t)cus;
+  FrameBuffers b;
+  memset(&b, 0, sizeof(b));
+  FrameCamera* camera = (FrameCamera*)(base + L.camera);
+  fill_params(c, cu_count, frame, n, object_count, data->sh_degree, data->active_slots != nullptr,
+              capacity, camera, b.params);
+  b.state = (DeviceState*)(base + L.state);
+  b.scene.means = (const float4*)data->means;
+  b.scene.scales_opacity = (const uint2*)data->scales_opacity;
+  b.scene.rotations = data->rotations;
+  b.scene.sh = data->sh;
+  b.scene.active_slots = data->active_slots;
+  b.scene.objects = frame->object_frames;
+  b.proj.mean = (float4*)(base + L.pmean);
+  b.proj.splat = (gs::Splat*)(base + L.splat);
+  b.proj.tile_counts = (uint32_t*)(base + L.tile_counts);
+  b.proj.hits = (uint4*)(base + L.hits);
+  b.ordered_hits = (uint4*)(base + L.ordered_hits);
+  b.visible_offsets = (uint32_t*)(base + L.visible_offsets);
+  b.depth_a = (uint2*)(base + L.depth_a);
+  b.depth_b = (uint2*)(base + L.depth_b);
+  b.ordered_counts = (uint32_t*)(base + L.ordered_counts);
+  b.intersection_offsets = (uint32_t*)(base + L.intersection_offsets);
+  b.records_a = (uint2*)(base + L.records_a);
+  b.records_b = (uint2*)(base + L.records_b);
+  b.tile_offsets = (uint32_t*)(base + L.tile_offsets);
+  b.chunk_first = (uint32_t*)(base + L.chunk_first);
+  b.tasks = (uint2*)(base + L.tasks);
+  b.chunk_partials = (float4*)(base + L.partials_px);
+  b.partial_depth = (float*)(base + L.partial_depth);
+  b.task_capacity = L.task_capacity;
+  b.unit_capacity = L.unit_capacity;
+  b.hist = (uint32_t*)(base + L.hist);
+  b.sort_parts = L.sort_parts;
+  b.partials = (uint32_t*)(base + L.partials);
+  b.color_out = (uint16_t*)out->color;
+  b.depth_out = c.output_depth ? (float*)out->depth : nullptr;
+  b.max_n = n;
+  b.max_k = capacity;
+  b.max_tiles = tiles;
+  b.radix_bits = c.radix_bits;
+  b.raster_batch = c.raster_batch;
+  b.cu_count = cu_count;
+  b.depth_passes = div_up(c.depth_sort_mode == GS_DEPTH_FLOAT32 ? 32u : 16u, c.radix_bits);
+  b.tile_passes = std::max<uint32_t>(1u, div_up(bits_for(tiles), c.radix_bits));
+  FrameCamera cam;
+  memset(&cam, 0, sizeof(cam));
+  memcpy(cam.projection, frame->projection, sizeof(cam.projection));
+  cam.near_plane = frame->near_plane;
+  cam.far_plane = frame->far_plane;
+  cam.attempt = frame->attempt;
+  cam.reset_valid = frame->reset_valid;
+  hipStream_t s = (hipStream_t)stream;
+  k_noalloc_begin<<<1, 256, 0, s>>>(cam, camera, (uint32_t*)b.state,
+                                    (uint32_t)(sizeof(DeviceState) / 4),
+                                    (unsigned long long*)b.partials, L.scan_parts + 1);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = gs::launch_frame(b, s, gs::RasterStreams{nullptr, nullptr, nullptr},
+                                            nullptr);
+  if (e != hipSuccess) return GS_E_DEVICE;
+  if (state) {
+    DeviceState d;
+    e = hipMemcpyAsync(&d, b.state, sizeof(d), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return GS_E_DEVICE;
+    memset(state, 0, sizeof(*state));
+    state->attempt = d.out_attempt;
+    state->version = d.out_version;
+    state->requested = d.out_demand;
+    state->overflow = d.out_overflow;
+    state->visible = d.visible;
+    state->intersections = d.count;
+    state->capacity = capacity;
+    state->tiles_x = b.params.tiles_x;
+    state->tiles_y = b.params.tiles_y;
+    state->raster_checked = d.checked;
+    state->raster_blended = d.blended;
+    state->raster_checked_heavy = d.checked_heavy;
+    state->error_flags = d.sort_error;
+  }
+  return GS_OK;
+}
 
 // ------------------------------------------------------------ learned cull ----
 

@@ -2169,38 +2169,48 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
   DeviceState* st = b.state;
   k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
                                    b.unit_capacity);
-  hipError_t e = hipEventRecord(rs.fork, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
-  if (e != hipSuccess) return e;
+  // Without a second stream (rs.aux null) the heavy kernels run on s, ahead
+  // of the light-tile kernel, and no event is recorded.
+  const hipStream_t aux = rs.aux ? rs.aux : s;
+  hipError_t e = hipSuccess;
+  if (rs.aux) {
+    e = hipEventRecord(rs.fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
+    if (e != hipSuccess) return e;
+  }
   // The whole-tile kernel, then (edge 16) the two row-slab kernels: all but
   // the frame's shape exit at once, each before the next can fill the GPU.
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], rs.aux);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
   k_raster_heavy<kEdge, kShapeWhole, kRecord>
       <<<heavy_grid<kEdge, kShapeWhole>(b.cu_count), heavy_threads(kEdge, kShapeWhole), 0,
-         rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+         aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
                    b.partial_depth, b.color_out, b.depth_out, b.record);
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 1], rs.aux);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 1], aux);
   if constexpr (kEdge == 16) {
     k_raster_heavy<kEdge, kShapeSlab8, kRecord>
         <<<heavy_grid<kEdge, kShapeSlab8>(b.cu_count), heavy_threads(kEdge, kShapeSlab8), 0,
-           rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+           aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
                      b.partial_depth, b.color_out, b.depth_out, b.record);
   }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], rs.aux);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], aux);
   if constexpr (kEdge == 16) {
     k_raster_heavy<kEdge, kShapeSlab4, kRecord>
         <<<heavy_grid<kEdge, kShapeSlab4>(b.cu_count), heavy_threads(kEdge, kShapeSlab4), 0,
-           rs.aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
+           aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
                      b.partial_depth, b.color_out, b.depth_out, b.record);
   }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], rs.aux);
-  e = hipEventRecord(rs.join, rs.aux);
-  if (e != hipSuccess) return e;
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], aux);
+  if (rs.aux) {
+    e = hipEventRecord(rs.join, rs.aux);
+    if (e != hipSuccess) return e;
+  }
   uint32_t light_blocks = b.max_tiles < kLightBlocks ? b.max_tiles : kLightBlocks;
   k_raster_light<kEdge, kBatch, kRecord><<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(
       fp, st, sorted, b.tile_offsets, b.proj.splat, b.color_out, b.depth_out, b.record);
-  e = hipStreamWaitEvent(s, rs.join, 0);
-  if (e != hipSuccess) return e;
+  if (rs.aux) {
+    e = hipStreamWaitEvent(s, rs.join, 0);
+    if (e != hipSuccess) return e;
+  }
   if (fp.chunk_size != 0)
     k_composite<kEdge><<<b.max_tiles, kEdge * kEdge, 0, s>>>(
         fp, st, b.tile_offsets, b.chunk_first, b.chunk_partials, b.partial_depth, b.color_out,

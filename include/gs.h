@@ -331,6 +331,32 @@ int gs_cull_eval(gs_ctx* ctx, uint32_t n, const float* features, float* out);
 int gs_cull_gradient(gs_ctx* ctx, uint32_t n, const float* features, const float* targets,
                      float under_weight, float* grad, float* loss);
 
+/* One frame without allocating (no counterpart in the reference): for
+ * callers that cannot call hipMalloc -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation. It renders what gs_render
+ * renders, with the same kernels and bits, but takes the scene
+ * (data->memory == GS_MEMORY_DEVICE; active_slots, when given, must hold
+ * slots below data->count), this camera's object table
+ * (frame->object_frames: object_count * 40 f32 in device memory) and the
+ * output images (out->memory == GS_MEMORY_DEVICE; depth with
+ * config->output_depth) from the caller, and keeps every working array of
+ * the frame in one library-owned device array of GS_NOALLOC_SCRATCH_BYTES
+ * per GPU, held from library load until the process ends. config supplies
+ * the options, the device and the intersection capacity
+ * (config->intersection_capacity); the frame's working set, about
+ * 144 B per active Gaussian + 16 B per intersection of capacity + 20 B per
+ * tile pixel per raster chunk task, must fit the array, else GS_E_RANGE.
+ * Kernels are enqueued on `stream` (a hipStream_t; NULL is the null stream)
+ * and nothing is allocated or freed. With state == NULL the call is
+ * asynchronous (and can be captured); otherwise it synchronises the stream
+ * and fills *state. An overflowing frame draws nothing. Every call on one
+ * GPU uses the same array, which keeps the last call's contents: calls that
+ * overlap on one GPU race. Returns GS_OK or GS_E_RANGE / GS_E_TYPE /
+ * GS_E_DEVICE (no message is kept: there is no context). */
+#define GS_NOALLOC_SCRATCH_BYTES (32u << 20)
+int gs_render_noalloc(const gs_config* config, const gs_data_desc* data, uint32_t object_count,
+                      const gs_frame* frame, const gs_image* out, void* stream, gs_state* state);
+
 #ifdef __cplusplus
 }
 #endif
