@@ -25,6 +25,15 @@ from .gaussian_pass import (IntersectionCapacityPolicy, config_for, frame_for,
 from .scene import PackedScene
 
 
+def check_object_tables(tables: list[np.ndarray]) -> None:
+    """Every camera of a batch brings its own object table (40 floats per
+    object); the library reads as many rows from each as the first has, so
+    they must all have that many."""
+    rows = {t.shape[0] for t in tables}
+    if len(rows) > 1:
+        raise N.GsRangeError(N.GS_E_RANGE, "every camera's object table must have the same rows")
+
+
 class BatchedPass:
     """A batch of cameras rendered together on one GPU."""
 
@@ -48,15 +57,16 @@ class BatchedPass:
         """The gs_frame array for the batch (one object table per camera)."""
         if len(cameras) != len(sizes) or not cameras:
             raise N.GsRangeError(N.GS_E_RANGE, "one size per camera is required")
+        tables = [np.ascontiguousarray(object_frames_for(cam) if objects is None else objects[i],
+                                       np.float32).reshape(-1, 40)
+                  for i, cam in enumerate(cameras)]
+        check_object_tables(tables)
         self.attempt += 1
         arr = (N.GsFrame * len(cameras))()
-        self._objects = []
+        self._objects = tables
         for i, (cam, (w, h)) in enumerate(zip(cameras, sizes)):
             f = frame_for(cam, w, h, self.options, self.attempt)
-            table = object_frames_for(cam) if objects is None else objects[i]
-            table = np.ascontiguousarray(table, np.float32).reshape(-1, 40)
-            self._objects.append(table)
-            f.object_frames = table.ctypes.data
+            f.object_frames = tables[i].ctypes.data
             arr[i] = f
         if len(self._objects):
             N.check(N.lib().gs_set_objects(self._ctx, self._objects[0].shape[0],

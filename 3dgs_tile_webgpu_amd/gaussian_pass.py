@@ -45,12 +45,26 @@ DEFAULT_OPTIONS = {
     "radixBits": 8,
     "rasterBatch": 256,
 }
-KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | {
+# Options of the reference that take no part in this path's arithmetic:
+# accepted (and validated where the reference validates them) for drop-in
+# use. radixBackend picks a WebGPU sort kernel (the sort here is always the
+# library's own, with identical order); colorSpace only tags the output
+# texture; profileKernels asks for timestamp queries (gs_profile_begin/end
+# is the counterpart).
+ACCEPTED_OPTIONS = {"redrawStrategy", "radixBackend", "dataMode", "colorSpace", "profileKernels",
+                    "intersectionReadbackInterval"}
+KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | ACCEPTED_OPTIONS | {
     "fixedSize", "depthRange", "intersectionCapacity", "initialIntersectionCapacity",
-    "maxIntersectionCapacity", "device", "dataMode", "batchLanes", "heavySplitUnits",
+    "maxIntersectionCapacity", "device", "batchLanes", "heavySplitUnits",
     "heavyFineSplitUnits",
 }
 MAX_CAPACITY = 1 << 28
+
+
+def _f32(v: float) -> float:
+    """Math.fround: the nearest binary32 value (inf beyond its range)."""
+    with np.errstate(over="ignore"):
+        return float(np.float32(v))
 
 
 class IntersectionCapacityPolicy:
@@ -79,6 +93,9 @@ class IntersectionCapacityPolicy:
             raise N.GsRangeError(N.GS_E_RANGE, "intersectionGrowthFactor must exceed 1")
         if not math.isfinite(self.headroom) or self.headroom < 1:
             raise N.GsRangeError(N.GS_E_RANGE, "intersectionHeadroom must be at least 1")
+        self.interval = options.get("intersectionReadbackInterval", 100)
+        if not math.isfinite(self.interval) or self.interval < 0:
+            raise N.GsRangeError(N.GS_E_RANGE, "intersectionReadbackInterval must be nonnegative")
 
     def next(self, current: int, demand: int) -> int | None:
         """Next capacity after an overflow with `demand` records, or None."""
@@ -89,40 +106,63 @@ class IntersectionCapacityPolicy:
 
 
 def validate_options(options: dict) -> dict:
-    """GaussianPass.ts:157-260 checks for the supported option subset."""
+    """GaussianPass.ts:157-260 and the render-domain setters (:386-466) for
+    the supported option subset, with the reference's error classes: every
+    value check is a RangeError there, and so a GsRangeError (ValueError)
+    here. depthSortMode has no run-time check in the reference (its type
+    system rejects other values); an unsupported variant or an unknown
+    option is a GsTypeError (TypeError) here."""
     unknown = set(options) - KNOWN_OPTIONS
     if unknown:
         raise N.GsTypeError(N.GS_E_TYPE, f"unsupported options: {sorted(unknown)}")
     o = {**DEFAULT_OPTIONS, **options}
     if o["depthSortMode"] not in ("float32", "packed16"):
         raise N.GsTypeError(N.GS_E_TYPE, "depthSortMode must be float32 or packed16")
-    if o["antialiasMode"] not in ("compensated", "classic"):
-        raise N.GsTypeError(N.GS_E_TYPE, "antialiasMode must be compensated or classic")
-    if o["boundaryPolicy"] not in N.GS_BOUNDARY:
-        raise N.GsRangeError(N.GS_E_RANGE, "boundaryPolicy must be clip, repeat-x or repeat-y")
-    t = o["rasterTransmittanceThreshold"]
+    if o["antialiasMode"] not in ("compensated", "classic"):  # GaussianPass.ts:161-165
+        raise N.GsRangeError(N.GS_E_RANGE, 'antialiasMode must be either "compensated" or "classic"')
+    if o.get("redrawStrategy", "always") not in ("always", "auto", "never"):  # :166-174
+        raise N.GsRangeError(N.GS_E_RANGE, 'redrawStrategy must be "always", "auto", or "never"')
+    if o.get("radixBackend", "auto") not in ("auto", "subgroup", "workgroup"):  # radixBackend.ts:7-15
+        raise N.GsRangeError(N.GS_E_RANGE, 'radixBackend must be "auto", "subgroup", or "workgroup"')
+    if o.get("dataMode", "managed") not in ("managed", "external"):  # :212-213
+        raise N.GsRangeError(N.GS_E_RANGE, 'dataMode must be "managed" or "external"')
+    if o["boundaryPolicy"] not in N.GS_BOUNDARY:  # :462-466
+        raise N.GsRangeError(N.GS_E_RANGE, 'boundaryPolicy must be "clip", "repeat-x", or "repeat-y"')
+    t = o["rasterTransmittanceThreshold"]  # :246-255
     if not (isinstance(t, (int, float)) and math.isfinite(t) and 0 < t < 1):
         raise N.GsRangeError(N.GS_E_RANGE, "rasterTransmittanceThreshold must be finite and in (0, 1)")
-    fv = o["filterVariance"]
-    if not (math.isfinite(fv) and fv >= 0):
+    fv = o["filterVariance"]  # :436-441
+    if not (isinstance(fv, (int, float)) and math.isfinite(fv) and fv >= 0
+            and math.isfinite(_f32(fv))):
         raise N.GsRangeError(N.GS_E_RANGE, "filterVariance must be finite and nonnegative")
+    iv = o["invalidDepthValue"]  # :449-455 (Infinity is allowed)
+    if not isinstance(iv, (int, float)) or math.isnan(iv) or (math.isfinite(iv)
+                                                              and not math.isfinite(_f32(iv))):
+        raise N.GsRangeError(N.GS_E_RANGE,
+                             "invalidDepthValue must be representable as float32 and cannot be NaN")
     bg = o["background"]
     if len(bg) != 4 or not all(math.isfinite(v) for v in bg):
         raise N.GsRangeError(N.GS_E_RANGE, "background must be four finite numbers")
-    cs = o["rasterChunkSize"]
-    if cs is not None and (not isinstance(cs, int) or cs < 256 or cs % 256 != 0):
+    cs = o["rasterChunkSize"]  # rasterChunks.ts:80-94
+    if cs is not None and (not isinstance(cs, int) or isinstance(cs, bool) or cs < 256
+                           or cs % 256 != 0):
         raise N.GsRangeError(N.GS_E_RANGE, "rasterChunkSize must be a multiple of 256 and at least 256")
-    da = o["depthAlphaThreshold"]
-    if not (0 <= da <= 1):
-        raise N.GsRangeError(N.GS_E_RANGE, "depthAlphaThreshold must be in [0, 1]")
-    fs = o.get("fixedSize")
+    da = o["depthAlphaThreshold"]  # :236-242
+    if not (isinstance(da, (int, float)) and 0 <= da <= 1):
+        raise N.GsRangeError(N.GS_E_RANGE, "depthAlphaThreshold must be finite and in [0, 1]")
+    fs = o.get("fixedSize")  # :386-395
     if fs is not None:
         w, h = fs["width"], fs["height"]
-        if not (isinstance(w, int) and isinstance(h, int) and w >= 1 and h >= 1):
-            raise N.GsRangeError(N.GS_E_RANGE, "fixedSize must be positive integers")
-    dr = o.get("depthRange")
-    if dr is not None and not (0 < dr[0] < dr[1] and math.isfinite(dr[1])):
-        raise N.GsRangeError(N.GS_E_RANGE, "depthRange must be positive and increasing")
+        if not all(isinstance(v, int) and not isinstance(v, bool) and 1 <= v <= 2**53 - 1
+                   for v in (w, h)):
+            raise N.GsRangeError(N.GS_E_RANGE, "fixedSize width/height must be positive safe integers")
+    dr = o.get("depthRange")  # :410-423
+    if dr is not None:
+        n, f = float(dr[0]), float(dr[1])
+        if not (math.isfinite(n) and math.isfinite(f) and n > 0 and _f32(n) > 0 and f > n
+                and math.isfinite(_f32(f)) and _f32(n) < _f32(f)):
+            raise N.GsRangeError(N.GS_E_RANGE,
+                                 "depthRange must contain finite positive increasing float32 values")
     if (o["tileSize"] not in (8, 16, 32) or o["radixBits"] not in (4, 6, 8)
             or o["rasterBatch"] not in (128, 256)):
         raise N.GsRangeError(N.GS_E_RANGE,
@@ -242,6 +282,16 @@ class GaussianPass:
         self.latest_attempt = 0
         self.width = self.height = 0
         self.data = None
+        # redrawStrategy (GaussianPass.ts:159-174, 684-693, 776-790, 1187+):
+        # "always" encodes every frame; "auto" only when the frame's inputs
+        # (size, depth range, projection, camera pose, object table, data)
+        # changed or the pass was invalidated; "never" only when invalidated.
+        self.redraw_strategy = self.options.get("redrawStrategy", "always")
+        self.render_count = 0       # frames encoded (renderCount)
+        self.cache_hit_count = 0    # frames that reused the last output (cacheHitCount)
+        self._dirty = True
+        self._snapshot = None
+        self.allocation_error: str | None = None  # set when growth failed (allocation-failed)
         self.update_data(data)
 
     # -- data --------------------------------------------------------------
@@ -249,11 +299,17 @@ class GaussianPass:
         """TiledGaussianPipeline.updateData (TiledGaussianPipeline.ts:203-218)."""
         self._keep = upload_scene(self._ctx, data)
         self.data = data
+        self._dirty = True
 
     def update_active(self, start: int, slots: np.ndarray, active_count: int) -> None:
         s = np.ascontiguousarray(slots, np.uint32)
         N.check(N.lib().gs_update_active(self._ctx, start, s.size, s.ctypes.data if s.size else None,
                                          active_count), self._ctx)
+        self._dirty = True
+
+    def invalidate(self) -> None:
+        """Force the next render to encode a frame (GaussianPass.ts:604-606)."""
+        self._dirty = True
 
     # -- frames ------------------------------------------------------------
     def frame_struct(self, width: int, height: int, attempt: int, reset_valid: bool = False) -> N.GsFrame:
@@ -274,17 +330,40 @@ class GaussianPass:
             width, height = (fs["width"], fs["height"]) if fs else (self.width, self.height)
         if not width or not height:
             raise N.GsRangeError(N.GS_E_RANGE, "render size is required")
+        if self.allocation_error is not None:
+            return  # GaussianPass.ts:683: nothing is encoded after a failed allocation
+        table = np.ascontiguousarray(self.object_frames() if objects is None else objects,
+                                     np.float32).reshape(-1, 40)
+        if (width, height) != (self.width, self.height):
+            self._dirty = True  # a resized target holds no valid frame
+        snapshot = None
+        if self.redraw_strategy == "auto":
+            f0 = self.frame_struct(width, height, 0)
+            snapshot = (width, height, f0.near_plane, f0.far_plane, bytes(f0.projection),
+                        table.tobytes())
+            if snapshot != self._snapshot:
+                self._dirty = True
+        if self.redraw_strategy != "always" and not self._dirty:
+            self.cache_hit_count += 1
+            return
         self.width, self.height = width, height
-        self.set_objects(self.object_frames() if objects is None else objects)
+        self.set_objects(table)
         self.latest_attempt += 1
         f = self.frame_struct(width, height, self.latest_attempt)
         N.check(N.lib().gs_render(self._ctx, C.byref(f), None), self._ctx)
+        self.render_count += 1
+        self._dirty = False
+        self._snapshot = snapshot
         if retry_overflow:
             st = self.read_state()
             if st.overflow:
                 nxt = self.policy.next(self.capacity, st.requested)
                 if nxt is not None and nxt > self.capacity:
-                    self.resize_intersections(nxt)
+                    try:
+                        self.resize_intersections(nxt)
+                    except N.GsError as e:  # GaussianPass.ts:1010-1017 (failAllocation)
+                        self.allocation_error = str(e)
+                        return
                     self.latest_attempt += 1
                     f = self.frame_struct(width, height, self.latest_attempt)
                     N.check(N.lib().gs_render(self._ctx, C.byref(f), None), self._ctx)
@@ -319,13 +398,16 @@ class GaussianPass:
     def get_output_status(self) -> OutputStatus:
         """getOutputStatus (GaussianPass.ts:989-1008), synchronous form."""
         st = self.read_state()
-        if st.attempt == 0:
+        if self.allocation_error is not None:
+            state = "allocation-failed"
+        elif st.attempt == 0:
             state = "awaiting-first-result"
         elif st.overflow:
             state = "budget-exceeded" if self.policy.next(self.capacity, st.requested) is None else "overflow-pending"
         else:
             state = "ready"
-        return OutputStatus(state, st.version != 0 and not st.overflow, st.version, st.attempt,
+        valid = st.version != 0 and not st.overflow and self.allocation_error is None
+        return OutputStatus(state, valid, st.version, st.attempt,
                             self.latest_attempt, st.requested, self.capacity, self.policy.maximum,
                             self.growth_count)
 

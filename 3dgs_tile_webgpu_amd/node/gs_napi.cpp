@@ -80,20 +80,31 @@ bool typed_prop(napi_env env, napi_value obj, const char* name, napi_typedarray_
 }
 
 // A context handle is an External over this box; destroy() empties it so
-// the device buffers go before garbage collection runs the finalizer.
+// the device buffers go before garbage collection runs the finalizer. It
+// also keeps the object count of the last setObjects: a frame's own object
+// table must have exactly that many rows (the library copies that many).
 struct Handle {
   gs_ctx* ctx;
+  uint32_t object_count;
 };
 
-gs_ctx* ctx_arg(napi_env env, napi_value v) {
+Handle* handle_arg(napi_env env, napi_value v) {
   void* p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     throw_status(env, GS_E_TYPE, "expected a context handle");
     return nullptr;
   }
-  gs_ctx* ctx = ((Handle*)p)->ctx;
-  if (!ctx) throw_status(env, GS_E_STATE, "the pass has been disposed");
-  return ctx;
+  Handle* h = (Handle*)p;
+  if (!h->ctx) {
+    throw_status(env, GS_E_STATE, "the pass has been disposed");
+    return nullptr;
+  }
+  return h;
+}
+
+gs_ctx* ctx_arg(napi_env env, napi_value v) {
+  Handle* h = handle_arg(env, v);
+  return h ? h->ctx : nullptr;
 }
 
 void finalize_ctx(napi_env, void* data, void*) {
@@ -185,7 +196,7 @@ napi_value create(napi_env env, napi_callback_info info) {
     return throw_status(env, rc, msg);
   }
   napi_value ext;
-  napi_create_external(env, new Handle{ctx}, finalize_ctx, nullptr, &ext);
+  napi_create_external(env, new Handle{ctx, 0u}, finalize_ctx, nullptr, &ext);
   return ext;
 }
 
@@ -227,13 +238,15 @@ napi_value set_data(napi_env env, napi_callback_info info) {
 napi_value set_objects(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
-  gs_ctx* ctx = ctx_arg(env, argv[0]);
-  if (!ctx) return nullptr;
+  Handle* h = handle_arg(env, argv[0]);
+  if (!h) return nullptr;
   void* p;
   size_t n;
   if (!typed(env, argv[1], napi_float32_array, &p, &n) || n % 40)
     return throw_status(env, GS_E_TYPE, "object frames must be a Float32Array of 40 per object");
-  return check(env, ctx, gs_set_objects(ctx, (uint32_t)(n / 40), (const float*)p));
+  int rc = gs_set_objects(h->ctx, (uint32_t)(n / 40), (const float*)p);
+  if (rc == GS_OK) h->object_count = (uint32_t)(n / 40);
+  return check(env, h->ctx, rc);
 }
 
 napi_value object_frame(napi_env env, napi_callback_info info) {
@@ -254,7 +267,10 @@ napi_value object_frame(napi_env env, napi_callback_info info) {
   return out;
 }
 
-bool read_frame(napi_env env, napi_value o, gs_frame* f, std::vector<float>* objects) {
+// A frame object; `objects` receives a copy of its own object table, which
+// must hold object_count rows of 40 floats (the count of setObjects).
+bool read_frame(napi_env env, napi_value o, gs_frame* f, std::vector<float>* objects,
+                uint32_t object_count) {
   memset(f, 0, sizeof(*f));
   void* p;
   size_t n;
@@ -272,6 +288,11 @@ bool read_frame(napi_env env, napi_value o, gs_frame* f, std::vector<float>* obj
   f->attempt = (uint32_t)num(env, o, "attempt", 1);
   f->reset_valid = (uint32_t)num(env, o, "resetValid", 0);
   if (objects && typed_prop(env, o, "objectFrames", napi_float32_array, &p, &n)) {
+    if (n != (size_t)object_count * 40) {
+      throw_status(env, GS_E_RANGE,
+                   "objectFrames must hold 40 floats for each object of setObjects");
+      return false;
+    }
     objects->assign((const float*)p, (const float*)p + n);
     f->object_frames = objects->data();
   }
@@ -281,29 +302,39 @@ bool read_frame(napi_env env, napi_value o, gs_frame* f, std::vector<float>* obj
 napi_value render(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
-  gs_ctx* ctx = ctx_arg(env, argv[0]);
-  if (!ctx) return nullptr;
+  Handle* h = handle_arg(env, argv[0]);
+  if (!h) return nullptr;
   gs_frame f;
   std::vector<float> objects;
-  if (!read_frame(env, argv[1], &f, &objects)) return nullptr;
-  return check(env, ctx, gs_render(ctx, &f, nullptr));
+  if (!read_frame(env, argv[1], &f, &objects, h->object_count)) return nullptr;
+  return check(env, h->ctx, gs_render(h->ctx, &f, nullptr));
+}
+
+// The gs_frame array of a JS array of frame objects (object tables copied).
+bool read_frames(napi_env env, napi_value arr, uint32_t object_count,
+                 std::vector<gs_frame>* frames, std::vector<std::vector<float>>* objects) {
+  uint32_t n = 0;
+  napi_get_array_length(env, arr, &n);
+  frames->resize(n);
+  objects->resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    if (!read_frame(env, e, &(*frames)[i], &(*objects)[i], object_count)) return false;
+  }
+  return true;
 }
 
 napi_value render_batch(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   args(env, info, 2, argv);
-  gs_ctx* ctx = ctx_arg(env, argv[0]);
-  if (!ctx) return nullptr;
-  uint32_t n = 0;
-  napi_get_array_length(env, argv[1], &n);
-  std::vector<gs_frame> frames(n);
-  std::vector<std::vector<float>> objects(n);
-  for (uint32_t i = 0; i < n; i++) {
-    napi_value e;
-    napi_get_element(env, argv[1], i, &e);
-    if (!read_frame(env, e, &frames[i], &objects[i])) return nullptr;
-  }
-  return check(env, ctx, gs_render_batch(ctx, n, frames.data(), nullptr));
+  Handle* h = handle_arg(env, argv[0]);
+  if (!h) return nullptr;
+  std::vector<gs_frame> frames;
+  std::vector<std::vector<float>> objects;
+  if (!read_frames(env, argv[1], h->object_count, &frames, &objects)) return nullptr;
+  return check(env, h->ctx,
+               gs_render_batch(h->ctx, (uint32_t)frames.size(), frames.data(), nullptr));
 }
 
 napi_value state_object(napi_env env, const gs_state& s) {
@@ -320,6 +351,7 @@ napi_value state_object(napi_env env, const gs_state& s) {
   set_num(env, o, "tilesY", s.tiles_y);
   set_num(env, o, "rasterChecked", (double)s.raster_checked);
   set_num(env, o, "rasterBlended", (double)s.raster_blended);
+  set_num(env, o, "rasterCheckedHeavy", (double)s.raster_checked_heavy);
   set_num(env, o, "errorFlags", s.error_flags);
   return o;
 }
@@ -442,6 +474,302 @@ napi_value set_tuning(napi_env env, napi_callback_info info) {
   return check(env, ctx, gs_set_tuning(ctx, &t));
 }
 
+// A new Float32Array of n elements; *data receives its storage.
+napi_value new_f32(napi_env env, size_t n, float** data) {
+  napi_value buffer, out;
+  void* p = nullptr;
+  napi_create_arraybuffer(env, n * sizeof(float), &p, &buffer);
+  napi_create_typedarray(env, napi_float32_array, n, buffer, 0, &out);
+  *data = (float*)p;
+  return out;
+}
+
+double arg_num(napi_env env, napi_value v, double fallback) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  double d = fallback;
+  if (t == napi_number) napi_get_value_double(env, v, &d);
+  return d;
+}
+
+// updateActive(ctx, start, slots: Uint32Array, newActiveCount)
+napi_value update_active(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  args(env, info, 4, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  void* p = nullptr;
+  size_t n = 0;
+  if (!typed(env, argv[2], napi_uint32_array, &p, &n))
+    return throw_status(env, GS_E_TYPE, "slots must be a Uint32Array");
+  double start = arg_num(env, argv[1], -1), active = arg_num(env, argv[3], -1);
+  if (!(start >= 0 && start <= 4294967295.0 && active >= 0 && active <= 4294967295.0))
+    return throw_status(env, GS_E_RANGE, "start and activeCount must be non-negative integers");
+  return check(env, ctx, gs_update_active(ctx, (uint32_t)start, (uint32_t)n, (const uint32_t*)p,
+                                          (uint32_t)active));
+}
+
+napi_value get_tuning(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  gs_tuning t;
+  int rc = gs_get_tuning(ctx, &t);
+  if (rc) return check(env, ctx, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "tileSize", t.tile_size);
+  set_num(env, o, "radixBits", t.radix_bits);
+  set_num(env, o, "rasterBatch", t.raster_batch);
+  return o;
+}
+
+// autotune(ctx, frames, reps) -> {best: tuning, table: [{tileSize, radixBits, rasterBatch, ms}]}
+napi_value autotune(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  args(env, info, 3, argv);
+  Handle* h = handle_arg(env, argv[0]);
+  if (!h) return nullptr;
+  std::vector<gs_frame> frames;
+  std::vector<std::vector<float>> objects;
+  if (!read_frames(env, argv[1], h->object_count, &frames, &objects)) return nullptr;
+  double reps = arg_num(env, argv[2], 3);
+  if (!(reps >= 1 && reps <= 1e6)) return throw_status(env, GS_E_RANGE, "reps must be positive");
+  gs_tuning best;
+  gs_tune_entry table[18];
+  uint32_t count = 0;
+  int rc = gs_autotune(h->ctx, (uint32_t)frames.size(), frames.data(), (uint32_t)reps, &best,
+                       table, 18, &count);
+  if (rc) return check(env, h->ctx, rc);
+  napi_value o, b, arr;
+  napi_create_object(env, &o);
+  napi_create_object(env, &b);
+  set_num(env, b, "tileSize", best.tile_size);
+  set_num(env, b, "radixBits", best.radix_bits);
+  set_num(env, b, "rasterBatch", best.raster_batch);
+  napi_set_named_property(env, o, "best", b);
+  napi_create_array_with_length(env, count, &arr);
+  for (uint32_t i = 0; i < count && i < 18; i++) {
+    napi_value e;
+    napi_create_object(env, &e);
+    set_num(env, e, "tileSize", table[i].tuning.tile_size);
+    set_num(env, e, "radixBits", table[i].tuning.radix_bits);
+    set_num(env, e, "rasterBatch", table[i].tuning.raster_batch);
+    set_num(env, e, "ms", table[i].ms);
+    napi_set_element(env, arr, i, e);
+  }
+  napi_set_named_property(env, o, "table", arr);
+  return o;
+}
+
+// ---- learned cull (gs_cull_*) ----
+
+napi_value cull_init(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  double seed = arg_num(env, argv[1], 0);
+  if (!(seed >= 0 && seed <= 9007199254740991.0))
+    return throw_status(env, GS_E_RANGE, "seed must be a non-negative safe integer");
+  return check(env, ctx, gs_cull_init(ctx, (uint64_t)seed));
+}
+
+napi_value cull_set_params(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  void* p = nullptr;
+  size_t n = 0;
+  if (!typed(env, argv[1], napi_float32_array, &p, &n) || n != GS_CULL_PARAMS)
+    return throw_status(env, GS_E_TYPE, "params must be a Float32Array(5313)");
+  return check(env, ctx, gs_cull_set_params(ctx, (const float*)p));
+}
+
+napi_value cull_get_params(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  float* data = nullptr;
+  napi_value out = new_f32(env, GS_CULL_PARAMS, &data);
+  int rc = gs_cull_get_params(ctx, data);
+  if (rc) return check(env, ctx, rc);
+  return out;
+}
+
+napi_value cull_set_normalization(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  args(env, info, 3, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  void *mu = nullptr, *inv = nullptr;
+  size_t n1 = 0, n2 = 0;
+  if (!typed(env, argv[1], napi_float32_array, &mu, &n1) || n1 != GS_CULL_FEATURES ||
+      !typed(env, argv[2], napi_float32_array, &inv, &n2) || n2 != GS_CULL_FEATURES)
+    return throw_status(env, GS_E_TYPE, "mu and invSigma must be Float32Array(16)");
+  return check(env, ctx, gs_cull_set_normalization(ctx, (const float*)mu, (const float*)inv));
+}
+
+napi_value cull_get_normalization(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  float *mu = nullptr, *inv = nullptr;
+  napi_value a = new_f32(env, GS_CULL_FEATURES, &mu), b = new_f32(env, GS_CULL_FEATURES, &inv);
+  int rc = gs_cull_get_normalization(ctx, mu, inv);
+  if (rc) return check(env, ctx, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "mu", a);
+  napi_set_named_property(env, o, "invSigma", b);
+  return o;
+}
+
+napi_value cull_enable(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  args(env, info, 3, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  bool on = false;
+  napi_get_value_bool(env, argv[1], &on);
+  return check(env, ctx, gs_cull_enable(ctx, on ? 1 : 0, (float)arg_num(env, argv[2], 1e-4)));
+}
+
+napi_value cull_reserve(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  double cap = arg_num(env, argv[1], -1);
+  if (!(cap >= 0 && cap <= 4294967295.0))
+    return throw_status(env, GS_E_RANGE, "capacity must be a non-negative integer");
+  return check(env, ctx, gs_cull_reserve(ctx, (uint32_t)cap));
+}
+
+// cullCollect(ctx, frame, rate, seed) -> state
+napi_value cull_collect(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  args(env, info, 4, argv);
+  Handle* h = handle_arg(env, argv[0]);
+  if (!h) return nullptr;
+  gs_frame f;
+  std::vector<float> objects;
+  if (!read_frame(env, argv[1], &f, &objects, h->object_count)) return nullptr;
+  gs_state st;
+  int rc = gs_cull_collect(h->ctx, &f, (float)arg_num(env, argv[2], 0.05),
+                           (uint32_t)arg_num(env, argv[3], 0), &st);
+  if (rc) return check(env, h->ctx, rc);
+  return state_object(env, st);
+}
+
+napi_value cull_dataset_size(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  uint32_t n = 0;
+  int rc = gs_cull_dataset_size(ctx, &n);
+  if (rc) return check(env, ctx, rc);
+  napi_value r;
+  napi_create_uint32(env, n, &r);
+  return r;
+}
+
+// cullReadDataset(ctx, first, n) -> {features: Float32Array(n * 16), targets: Float32Array(n)}
+napi_value cull_read_dataset(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  args(env, info, 3, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  double first = arg_num(env, argv[1], -1), n = arg_num(env, argv[2], -1);
+  if (!(first >= 0 && n >= 0 && first + n <= 4294967295.0))
+    return throw_status(env, GS_E_RANGE, "first and n must be non-negative integers");
+  float *f = nullptr, *t = nullptr;
+  napi_value a = new_f32(env, (size_t)n * GS_CULL_FEATURES, &f), b = new_f32(env, (size_t)n, &t);
+  int rc = gs_cull_read_dataset(ctx, (uint32_t)first, (uint32_t)n, f, t);
+  if (rc) return check(env, ctx, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "features", a);
+  napi_set_named_property(env, o, "targets", b);
+  return o;
+}
+
+napi_value cull_fit_normalization(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  return check(env, ctx, gs_cull_fit_normalization(ctx));
+}
+
+// cullTrain(ctx, steps, batch, lr, underWeight, seed) -> last loss
+napi_value cull_train(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  args(env, info, 6, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  double steps = arg_num(env, argv[1], -1), batch = arg_num(env, argv[2], 65536);
+  double seed = arg_num(env, argv[5], 0);
+  if (!(steps >= 0 && steps <= 4294967295.0 && batch >= 1 && batch <= 4294967295.0 &&
+        seed >= 0 && seed <= 9007199254740991.0))
+    return throw_status(env, GS_E_RANGE, "steps, batch and seed must be non-negative integers");
+  float loss = 0.0f;
+  int rc = gs_cull_train(ctx, (uint32_t)steps, (uint32_t)batch, (float)arg_num(env, argv[3], 2e-3),
+                         (float)arg_num(env, argv[4], 4.0), (uint64_t)seed, &loss);
+  if (rc) return check(env, ctx, rc);
+  napi_value r;
+  napi_create_double(env, loss, &r);
+  return r;
+}
+
+// cullEval(ctx, features: Float32Array(n * 16)) -> Float32Array(n)
+napi_value cull_eval(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  void* p = nullptr;
+  size_t n = 0;
+  if (!typed(env, argv[1], napi_float32_array, &p, &n) || n % GS_CULL_FEATURES)
+    return throw_status(env, GS_E_TYPE, "features must be a Float32Array of 16 per sample");
+  float* out = nullptr;
+  napi_value r = new_f32(env, n / GS_CULL_FEATURES, &out);
+  if (n == 0) return r;
+  int rc = gs_cull_eval(ctx, (uint32_t)(n / GS_CULL_FEATURES), (const float*)p, out);
+  if (rc) return check(env, ctx, rc);
+  return r;
+}
+
+// cullGradient(ctx, features, targets, underWeight) -> {grad: Float32Array(5313), loss}
+napi_value cull_gradient(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  args(env, info, 4, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  void *f = nullptr, *t = nullptr;
+  size_t nf = 0, nt = 0;
+  if (!typed(env, argv[1], napi_float32_array, &f, &nf) || nf % GS_CULL_FEATURES ||
+      !typed(env, argv[2], napi_float32_array, &t, &nt) || nt != nf / GS_CULL_FEATURES)
+    return throw_status(env, GS_E_TYPE,
+                        "features (16 per sample) and targets (1 per sample) must be Float32Arrays");
+  float* g = nullptr;
+  napi_value grad = new_f32(env, GS_CULL_PARAMS, &g);
+  float loss = 0.0f;
+  int rc = gs_cull_gradient(ctx, (uint32_t)nt, (const float*)f, (const float*)t,
+                            (float)arg_num(env, argv[3], 4.0), g, &loss);
+  if (rc) return check(env, ctx, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "grad", grad);
+  set_num(env, o, "loss", loss);
+  return o;
+}
+
 napi_value last_error(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   args(env, info, 1, argv);
@@ -485,6 +813,23 @@ NAPI_MODULE_INIT() {
       {"capacityNext", nullptr, capacity_next, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"setTuning", nullptr, set_tuning, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"lastError", nullptr, last_error, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"updateActive", nullptr, update_active, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"getTuning", nullptr, get_tuning, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"autotune", nullptr, autotune, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullInit", nullptr, cull_init, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullSetParams", nullptr, cull_set_params, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullGetParams", nullptr, cull_get_params, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullSetNormalization", nullptr, cull_set_normalization, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullGetNormalization", nullptr, cull_get_normalization, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullEnable", nullptr, cull_enable, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullReserve", nullptr, cull_reserve, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullCollect", nullptr, cull_collect, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullDatasetSize", nullptr, cull_dataset_size, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullReadDataset", nullptr, cull_read_dataset, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullFitNormalization", nullptr, cull_fit_normalization, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullTrain", nullptr, cull_train, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullEval", nullptr, cull_eval, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullGradient", nullptr, cull_gradient, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
   };
   napi_define_properties(env, exports, sizeof p / sizeof *p, p);
   return exports;

@@ -5,6 +5,7 @@
 export type DepthSortMode = 'float32' | 'packed16';
 export type AntialiasMode = 'compensated' | 'classic';
 export type GaussianBoundaryPolicy = 'clip' | 'repeat-x' | 'repeat-y';
+export type GaussianPassRedrawStrategy = 'always' | 'auto' | 'never';
 
 export interface GaussianPassOptions {
   depthSortMode?: DepthSortMode;
@@ -26,6 +27,14 @@ export interface GaussianPassOptions {
   maxIntersectionCapacity?: number;
   intersectionGrowthFactor?: number;
   intersectionHeadroom?: number;
+  intersectionReadbackInterval?: number;
+  /** When frames are encoded (GaussianPass.ts:159-174); default "always". */
+  redrawStrategy?: GaussianPassRedrawStrategy;
+  /** Accepted for drop-in use; no part in this path's arithmetic. */
+  radixBackend?: 'auto' | 'subgroup' | 'workgroup';
+  dataMode?: 'managed' | 'external';
+  colorSpace?: string;
+  profileKernels?: boolean;
   /** Launch shape; only 16 gives tile keys comparable with the reference. */
   tileSize?: 8 | 16 | 32;
   radixBits?: 4 | 6 | 8;
@@ -68,7 +77,7 @@ export interface GaussianPassStats {
 }
 
 export type GaussianPassOutputState =
-  'awaiting-first-result' | 'ready' | 'overflow-pending' | 'budget-exceeded';
+  'awaiting-first-result' | 'ready' | 'overflow-pending' | 'budget-exceeded' | 'allocation-failed';
 
 export interface GaussianPassOutputStatus {
   state: GaussianPassOutputState;
@@ -94,6 +103,8 @@ export interface NativeState {
   tilesY: number;
   rasterChecked: number;
   rasterBlended: number;
+  /** The part of rasterChecked in heavy units (tiles above 1024 records, chunk tasks). */
+  rasterCheckedHeavy: number;
   errorFlags: number;
 }
 
@@ -112,9 +123,19 @@ export class GaussianPass {
   readonly invalidDepthValue: number;
   readonly boundaryPolicy: GaussianBoundaryPolicy;
   readonly depthRange: [number, number] | null;
+  readonly redrawStrategy: GaussianPassRedrawStrategy;
+  /** Frames encoded. */
   readonly renderCount: number;
+  /** Frames that reused the last output (redrawStrategy "auto" / "never"). */
+  readonly cacheHitCount: number;
   camera: CameraLike;
   updateData(data: GaussianDataBuffers): void;
+  /** Active slots [start, start + slots.length) and the new active count. */
+  updateActive(start: number, slots: Uint32Array, activeCount: number): void;
+  /** Force the next render to encode a frame. */
+  invalidate(): void;
+  /** The error of a failed capacity growth; no frame is encoded after it. */
+  readonly allocationError: string | null;
   setFixedSize(width: number, height: number): this;
   render(width?: number, height?: number): void;
   resizeIntersections(capacity: number): void;
@@ -137,9 +158,47 @@ export class GaussianBatch {
   render(cameras: CameraLike[], sizes: [number, number][]): void;
   readStates(): NativeState[];
   readImage(view: number): Uint16Array;
-  setTuning(tuning: { tileSize: number; radixBits: number; rasterBatch: number }): void;
+  setTuning(tuning: LaunchShape): void;
+  getTuning(): LaunchShape;
+  /** Times the 18 launch shapes on these views and keeps the fastest. */
+  autotune(cameras: CameraLike[], sizes: [number, number][], reps?: number):
+    { best: LaunchShape; table: (LaunchShape & { ms: number })[] };
   dispose(): void;
 }
+
+export interface LaunchShape { tileSize: number; radixBits: number; rasterBatch: number }
+
+export interface CalibrationRow { shift: number; keysKept: number; psnrMean: number; psnrMin: number }
+
+/** The learned cull of a pass or batch (no counterpart in the reference). */
+export class LearnedCull {
+  constructor(owner: GaussianPass | GaussianBatch, seed?: number);
+  threshold: number;
+  readonly enabled: boolean;
+  reserve(capacity: number): void;
+  collect(camera: CameraLike, width: number, height: number, rate: number, seed: number): NativeState;
+  datasetSize(): number;
+  readDataset(first?: number, n?: number): { features: Float32Array; targets: Float32Array };
+  fitNormalization(): void;
+  train(steps: number, opts?: { batch?: number; lr?: number; underWeight?: number; seed?: number }): number;
+  /** W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3 (5313 floats). */
+  params(): Float32Array;
+  setParams(params: Float32Array): void;
+  normalization(): { mu: Float32Array; invSigma: Float32Array };
+  setNormalization(mu: Float32Array, invSigma: Float32Array): void;
+  /** Predicted log max alpha * T for raw feature rows (16 floats each). */
+  predict(features: Float32Array): Float32Array;
+  gradient(features: Float32Array, targets: Float32Array, underWeight?: number):
+    { grad: Float32Array; loss: number };
+  enable(threshold?: number): void;
+  disable(): void;
+  calibrate(cameras: CameraLike[], sizes: [number, number][], targetPsnr?: number,
+            shifts?: number[]): { shift: number | null; rows: CalibrationRow[] };
+}
+
+/** PSNR of two rgba16float images (raw half bits) over rgb. */
+export function psnr(a: Uint16Array, b: Uint16Array): number;
+export function checkObjectTables(tables: Float32Array[]): void;
 
 export function gaussianPass(camera: CameraLike, data: GaussianDataBuffers,
                              options?: GaussianPassOptions): GaussianPass;

@@ -35,9 +35,16 @@ const DEFAULTS = Object.freeze({
   radixBits: 8,
   rasterBatch: 256,
 });
-const KNOWN = new Set([...Object.keys(DEFAULTS), 'fixedSize', 'depthRange', 'intersectionCapacity',
-  'initialIntersectionCapacity', 'maxIntersectionCapacity', 'device', 'heavySplitUnits',
-  'heavyFineSplitUnits']);
+// Options of the reference that take no part in this path's arithmetic,
+// accepted (and validated where the reference validates them) for drop-in
+// use: radixBackend picks a WebGPU sort kernel (the library's sort gives the
+// same order), colorSpace tags the output texture, profileKernels asks for
+// timestamp queries.
+const ACCEPTED = ['redrawStrategy', 'radixBackend', 'dataMode', 'colorSpace', 'profileKernels',
+  'intersectionReadbackInterval'];
+const KNOWN = new Set([...Object.keys(DEFAULTS), ...ACCEPTED, 'fixedSize', 'depthRange',
+  'intersectionCapacity', 'initialIntersectionCapacity', 'maxIntersectionCapacity', 'device',
+  'heavySplitUnits', 'heavyFineSplitUnits']);
 const BOUNDARY = { clip: 0, 'repeat-x': 1, 'repeat-y': 2 };
 const MAX_CAPACITY = 2 ** 28;
 const RESOURCE_NAMES = ['projected_mean', 'projected_conic', 'projected_color', 'visible_offsets',
@@ -69,6 +76,8 @@ class IntersectionCapacityPolicy {
     this.headroom = dflt(options.intersectionHeadroom, 1.25);
     if (!Number.isFinite(this.growth) || this.growth <= 1) throw new RangeError('intersectionGrowthFactor must exceed 1');
     if (!Number.isFinite(this.headroom) || this.headroom < 1) throw new RangeError('intersectionHeadroom must be at least 1');
+    this.interval = dflt(options.intersectionReadbackInterval, 100);
+    if (!Number.isFinite(this.interval) || this.interval < 0) throw new RangeError('intersectionReadbackInterval must be nonnegative');
   }
 
   // The capacity after an overflow with `demand` records, or null when the
@@ -78,18 +87,39 @@ class IntersectionCapacityPolicy {
   }
 }
 
-// GaussianPass.ts:157-260 checks for the supported option subset.
+// GaussianPass.ts:157-260 and the render-domain setters (:386-466) for the
+// supported option subset, with the reference's error classes (every value
+// check there is a RangeError). depthSortMode has no run-time check in the
+// reference (its types reject other values) and unknown options are
+// rejected: both are TypeErrors here.
 function validateOptions(options) {
   for (const k of Object.keys(options)) {
     if (!KNOWN.has(k)) throw new TypeError(`unsupported option: ${k}`);
   }
   const o = { ...DEFAULTS, ...options };
+  const oneOf = (v, d, allowed) => allowed.includes(v === undefined ? d : v);
   if (o.depthSortMode !== 'float32' && o.depthSortMode !== 'packed16') throw new TypeError('depthSortMode must be float32 or packed16');
-  if (o.antialiasMode !== 'compensated' && o.antialiasMode !== 'classic') throw new TypeError('antialiasMode must be compensated or classic');
-  if (!(o.boundaryPolicy in BOUNDARY)) throw new RangeError('boundaryPolicy must be clip, repeat-x or repeat-y');
+  if (o.antialiasMode !== 'compensated' && o.antialiasMode !== 'classic') {
+    throw new RangeError('antialiasMode must be either "compensated" or "classic"');
+  }
+  if (!oneOf(o.redrawStrategy, 'always', ['always', 'auto', 'never'])) {
+    throw new RangeError('redrawStrategy must be "always", "auto", or "never"');
+  }
+  if (!oneOf(o.radixBackend, 'auto', ['auto', 'subgroup', 'workgroup'])) {
+    throw new RangeError('radixBackend must be "auto", "subgroup", or "workgroup"');
+  }
+  if (!oneOf(o.dataMode, 'managed', ['managed', 'external'])) throw new RangeError('dataMode must be "managed" or "external"');
+  if (!(o.boundaryPolicy in BOUNDARY)) throw new RangeError('boundaryPolicy must be "clip", "repeat-x", or "repeat-y"');
   const t = o.rasterTransmittanceThreshold;
   if (!(Number.isFinite(t) && t > 0 && t < 1)) throw new RangeError('rasterTransmittanceThreshold must be finite and in (0, 1)');
-  if (!(Number.isFinite(o.filterVariance) && o.filterVariance >= 0)) throw new RangeError('filterVariance must be finite and nonnegative');
+  const fv = o.filterVariance;
+  if (!(Number.isFinite(fv) && fv >= 0 && Number.isFinite(Math.fround(fv)))) {
+    throw new RangeError('filterVariance must be finite and nonnegative');
+  }
+  const iv = o.invalidDepthValue;
+  if (typeof iv !== 'number' || Number.isNaN(iv) || (Number.isFinite(iv) && !Number.isFinite(Math.fround(iv)))) {
+    throw new RangeError('invalidDepthValue must be representable as float32 and cannot be NaN');
+  }
   if (!Array.isArray(o.background) || o.background.length !== 4 || !o.background.every(Number.isFinite)) {
     throw new RangeError('background must be four finite numbers');
   }
@@ -97,16 +127,20 @@ function validateOptions(options) {
   if (cs !== null && (!Number.isInteger(cs) || cs < 256 || cs % 256 !== 0)) {
     throw new RangeError('rasterChunkSize must be a multiple of 256 and at least 256');
   }
-  if (!(o.depthAlphaThreshold >= 0 && o.depthAlphaThreshold <= 1)) throw new RangeError('depthAlphaThreshold must be in [0, 1]');
+  const da = o.depthAlphaThreshold;
+  if (!(Number.isFinite(da) && da >= 0 && da <= 1)) throw new RangeError('depthAlphaThreshold must be finite and in [0, 1]');
   if (o.fixedSize !== undefined) {
     const { width, height } = o.fixedSize;
-    if (!(Number.isInteger(width) && Number.isInteger(height) && width >= 1 && height >= 1)) {
-      throw new RangeError('fixedSize must be positive integers');
+    if (!(isSafeInt(width) && isSafeInt(height) && width >= 1 && height >= 1)) {
+      throw new RangeError('fixedSize width/height must be positive safe integers');
     }
   }
-  if (o.depthRange !== undefined) {
+  if (o.depthRange !== undefined && o.depthRange !== null) {
     const [n, f] = o.depthRange;
-    if (!(n > 0 && n < f && Number.isFinite(f))) throw new RangeError('depthRange must be positive and increasing');
+    if (!(Number.isFinite(n) && Number.isFinite(f) && n > 0 && Math.fround(n) > 0 && f > n &&
+          Number.isFinite(Math.fround(f)) && Math.fround(n) < Math.fround(f))) {
+      throw new RangeError('depthRange must contain finite positive increasing float32 values');
+    }
   }
   if (![8, 16, 32].includes(o.tileSize) || ![4, 6, 8].includes(o.radixBits) || ![128, 256].includes(o.rasterBatch)) {
     throw new RangeError('tileSize must be 8, 16 or 32, radixBits 4, 6 or 8, rasterBatch 128 or 256');
@@ -149,6 +183,20 @@ function objectFrames(camera, world = null, visible = true) {
   return native.objectFrame(Float64Array.from(camera.matrixWorld.elements), w, visible);
 }
 
+// Every camera of a batch brings its own object table (40 floats per
+// object); the library reads as many rows from each as setObjects set, so
+// they must all have the first one's length.
+function checkObjectTables(tables) {
+  const n = tables.length ? tables[0].length : 0;
+  for (const t of tables) {
+    if (!(t instanceof Float32Array) || t.length % 40 !== 0 || t.length !== n) {
+      throw new RangeError("every camera's object table must have the same rows");
+    }
+  }
+}
+
+const sameArray = (a, b) => a !== null && b !== null && a.length === b.length && a.every((v, i) => v === b[i]);
+
 class GaussianPass {
   constructor(camera, data, options = {}) {
     this.options = validateOptions(options);
@@ -158,11 +206,23 @@ class GaussianPass {
     this._capacity = this.policy.initial;
     this.growthCount = 0;
     this.latestAttempt = 0;
+    // redrawStrategy (GaussianPass.ts:159-174, 684-693, 776-790, 1187+):
+    // "always" encodes every frame; "auto" only when the frame's inputs
+    // (size, depth range, projection, camera pose, data) changed or the pass
+    // was invalidated; "never" only when invalidated.
+    this.redrawStrategy = dflt(this.options.redrawStrategy, 'always');
     this.renderCount = 0;
+    this.cacheHitCount = 0;
+    this._dirty = true;
+    this._snapshot = null;
+    this.allocationError = null;  // set when growth failed (allocation-failed)
     this.width = 0;
     this.height = 0;
     this.updateData(data);
   }
+
+  // Force the next render to encode a frame (GaussianPass.ts:604-606).
+  invalidate() { this._dirty = true; }
 
   get intersectionCapacity() { return this._capacity; }
   get filterVariance() { return this.options.filterVariance; }
@@ -174,6 +234,14 @@ class GaussianPass {
   updateData(data) {
     native.setData(this._ctx, data);
     this.data = data;
+    this._dirty = true;
+  }
+
+  // Active-slot update of [start, start + slots.length)
+  // (GaussianData.ts:170-232; ProjectionStage.updateActiveSlots :298-329).
+  updateActive(start, slots, activeCount) {
+    native.updateActive(this._ctx, start, slots, activeCount);
+    this._dirty = true;
   }
 
   setFixedSize(width, height) {
@@ -207,16 +275,37 @@ class GaussianPass {
       height = fixed ? fixed.height : this.height;
     }
     if (!width || !height) throw new RangeError('render size is required');
+    if (this.allocationError !== null) return;  // GaussianPass.ts:683
+    const table = objectFrames(this.camera);
+    if (width !== this.width || height !== this.height) this._dirty = true;
+    let snapshot = null;
+    if (this.redrawStrategy === 'auto') {
+      const range = this.options.depthRange;
+      snapshot = [width, height, range ? range[0] : this.camera.near, range ? range[1] : this.camera.far,
+        ...Float32Array.from(this.camera.projectionMatrix.elements), ...table];
+      if (!sameArray(snapshot, this._snapshot)) this._dirty = true;
+    }
+    if (this.redrawStrategy !== 'always' && !this._dirty) {
+      this.cacheHitCount++;
+      return;
+    }
     this.width = width;
     this.height = height;
-    native.setObjects(this._ctx, objectFrames(this.camera));
+    native.setObjects(this._ctx, table);
     native.render(this._ctx, this._frame(width, height));
     this.renderCount++;
+    this._dirty = false;
+    this._snapshot = snapshot;
     const st = native.readState(this._ctx);
     if (st.overflow) {
       const next = this.policy.next(this._capacity, st.requested);
       if (next !== null && next > this._capacity) {
-        this.resizeIntersections(next);
+        try {
+          this.resizeIntersections(next);
+        } catch (e) {  // GaussianPass.ts:1010-1017 (failAllocation)
+          this.allocationError = String(e);
+          return;
+        }
         native.render(this._ctx, this._frame(width, height));
       }
     }
@@ -251,11 +340,12 @@ class GaussianPass {
   getOutputStatus() {
     const st = native.readState(this._ctx);
     let state = 'ready';
-    if (st.attempt === 0) state = 'awaiting-first-result';
+    if (this.allocationError !== null) state = 'allocation-failed';
+    else if (st.attempt === 0) state = 'awaiting-first-result';
     else if (st.overflow) state = this.policy.next(this._capacity, st.requested) === null ? 'budget-exceeded' : 'overflow-pending';
     return {
       state,
-      valid: st.version !== 0 && !st.overflow,
+      valid: st.version !== 0 && !st.overflow && this.allocationError === null,
       version: st.version,
       confirmedAttempt: st.attempt,
       latestAttempt: this.latestAttempt,
@@ -326,13 +416,19 @@ class GaussianBatch {
     }));
   }
 
-  render(cameras, sizes) {
+  _prepare(cameras, sizes) {
     if (cameras.length !== sizes.length || cameras.length === 0) {
       throw new RangeError('one size per camera is required');
     }
     const frames = this._frames(cameras, sizes);
+    checkObjectTables(frames.map((f) => f.objectFrames));
     // The object table's size comes from the context; each view brings its own frames.
     native.setObjects(this._ctx, frames[0].objectFrames);
+    return frames;
+  }
+
+  render(cameras, sizes) {
+    const frames = this._prepare(cameras, sizes);
     this.sizes = sizes.map((s) => [s[0], s[1]]);
     while (this.capacities.length < frames.length) this.capacities.push(this.policy.initial);
     for (let round = 0; round < 8; round++) {
@@ -362,9 +458,155 @@ class GaussianBatch {
 
   setTuning(tuning) { native.setTuning(this._ctx, tuning); }
 
+  getTuning() { return native.getTuning(this._ctx); }
+
+  // Times every launch shape (tile 8/16/32 x digits 4/6/8 x raster batch
+  // 128/256) on these views, keeps the fastest and returns {best, table}.
+  autotune(cameras, sizes, reps = 3) {
+    const frames = this._prepare(cameras, sizes);
+    this.sizes = sizes.map((s) => [s[0], s[1]]);
+    const result = native.autotune(this._ctx, frames, reps);
+    this.capacities = native.readBatchState(this._ctx, frames.length).map((st) => st.capacity);
+    return result;
+  }
+
   dispose() {
     if (this._ctx) native.destroy(this._ctx);
     this._ctx = null;
+  }
+}
+
+// PSNR of two rgba16float images (raw half bits) over their rgb channels.
+function halfToFloat(h) {
+  const s = h & 0x8000 ? -1 : 1;
+  const e = (h >> 10) & 31;
+  const m = h & 1023;
+  if (e === 0) return s * m * 2 ** -24;
+  if (e === 31) return m ? NaN : s * Infinity;
+  return s * (1 + m / 1024) * 2 ** (e - 15);
+}
+
+function psnr(a, b) {
+  let sum = 0;
+  let n = 0;
+  for (let i = 0; i < a.length; i++) {
+    if (i % 4 === 3) continue;
+    const d = halfToFloat(a[i]) - halfToFloat(b[i]);
+    sum += d * d;
+    n++;
+  }
+  const mse = sum / n;
+  return mse === 0 ? Infinity : 10 * Math.log10(1 / mse);
+}
+
+const CULL_PARAMS = 5313;
+
+// The learned cull of one pass or batch (no counterpart in the reference;
+// its nearest mechanisms are the opacity and subpixel culls,
+// ProjectionStage.ts:533, 572-587): a 16-64-64-1 MLP on the matrix cores,
+// fused into projection, trained by the library's own kernels on targets
+// the plain rasterizer records. The plain path is its fallback (disable())
+// and the yardstick of its image loss (calibrate()).
+class LearnedCull {
+  constructor(owner, seed = 0) {
+    this.owner = owner;
+    this.threshold = 1e-4;
+    this.enabled = false;
+    native.cullInit(owner._ctx, seed);
+  }
+
+  reserve(capacity) { native.cullReserve(this.owner._ctx, capacity); }
+
+  // Records one plain frame of `camera` and appends a hashed fraction `rate`
+  // of its visible Gaussians; an overflowing frame grows the owner's
+  // capacity and is recorded again.
+  collect(camera, width, height, rate, seed) {
+    const ctx = this.owner._ctx;
+    const table = objectFrames(camera);
+    native.setObjects(ctx, table);
+    const range = this.owner.options.depthRange;
+    for (let i = 0; i < 6; i++) {
+      const st = native.cullCollect(ctx, {
+        projection: Float32Array.from(camera.projectionMatrix.elements), width, height,
+        near: range ? range[0] : camera.near, far: range ? range[1] : camera.far,
+        filterVariance: this.owner.options.filterVariance,
+        boundaryAxis: BOUNDARY[this.owner.options.boundaryPolicy], attempt: 1, objectFrames: table,
+      }, rate, seed);
+      if (!st.overflow) return st;
+      const next = this.owner.policy.next(st.capacity, st.requested);
+      if (next === null) throw new RangeError('intersection demand exceeds maximum');
+      native.resizeIntersections(ctx, next);
+      if (this.owner._capacity !== undefined) this.owner._capacity = next;
+    }
+    throw new Error('collect frame kept overflowing');
+  }
+
+  datasetSize() { return native.cullDatasetSize(this.owner._ctx); }
+  readDataset(first = 0, n = undefined) {
+    const count = n === undefined ? this.datasetSize() - first : n;
+    return native.cullReadDataset(this.owner._ctx, first, count);
+  }
+  fitNormalization() { native.cullFitNormalization(this.owner._ctx); }
+  train(steps, { batch = 65536, lr = 2e-3, underWeight = 4.0, seed = 0 } = {}) {
+    return native.cullTrain(this.owner._ctx, steps, batch, lr, underWeight, seed);
+  }
+  params() { return native.cullGetParams(this.owner._ctx); }
+  setParams(p) { native.cullSetParams(this.owner._ctx, p); }
+  normalization() { return native.cullGetNormalization(this.owner._ctx); }
+  setNormalization(mu, invSigma) { native.cullSetNormalization(this.owner._ctx, mu, invSigma); }
+  predict(features) { return native.cullEval(this.owner._ctx, features); }
+  gradient(features, targets, underWeight = 4.0) {
+    return native.cullGradient(this.owner._ctx, features, targets, underWeight);
+  }
+
+  enable(threshold = undefined) {
+    if (threshold !== undefined) this.threshold = threshold;
+    native.cullEnable(this.owner._ctx, true, this.threshold);
+    this.enabled = true;
+    if (this.owner.invalidate) this.owner.invalidate();
+  }
+
+  disable() {
+    native.cullEnable(this.owner._ctx, false, this.threshold);
+    this.enabled = false;
+    if (this.owner.invalidate) this.owner.invalidate();
+  }
+
+  // Moves the output bias by the lowest shift (most culling) whose culled
+  // frames of these cameras (a GaussianBatch owner) keep a mean PSNR of at
+  // least targetPsnr against their plain frames. Returns {shift, rows}; shift
+  // is null (parameters unchanged, cull disabled) when no shift reaches it.
+  calibrate(cameras, sizes, targetPsnr = 46, shifts = [-3, -2.5, -2, -1.875, -1.75, -1.625, -1.5,
+    -1.375, -1.25, -1.125, -1, -0.75, -0.5, -0.25, 0, 0.5, 1, 1.5, 2, 3, 4, 6, 8]) {
+    const b = this.owner;
+    const was = this.enabled;
+    this.disable();
+    b.render(cameras, sizes);
+    const plain = cameras.map((c, v) => b.readImage(v));
+    const keys = b.readStates().reduce((a, st) => a + st.intersections, 0);
+    const p0 = this.params();
+    const rows = [];
+    let chosen = null;
+    for (const shift of shifts) {
+      const p = Float32Array.from(p0);
+      p[CULL_PARAMS - 1] += shift;
+      this.setParams(p);
+      this.enable();
+      b.render(cameras, sizes);
+      const kept = b.readStates().reduce((a, st) => a + st.intersections, 0) / Math.max(1, keys);
+      const ps = cameras.map((c, v) => psnr(b.readImage(v), plain[v]));
+      const mean = ps.reduce((a, x) => a + x, 0) / ps.length;
+      rows.push({ shift, keysKept: kept, psnrMean: mean, psnrMin: Math.min(...ps) });
+      if (mean >= targetPsnr) {
+        chosen = shift;
+        break;
+      }
+    }
+    const p = Float32Array.from(p0);
+    if (chosen !== null) p[CULL_PARAMS - 1] += chosen;
+    this.setParams(p);
+    if (!was || chosen === null) this.disable();
+    return { shift: chosen, rows };
   }
 }
 
@@ -373,6 +615,6 @@ function gaussianPass(camera, data, options = {}) {
 }
 
 module.exports = {
-  gaussianPass, GaussianPass, GaussianBatch, IntersectionCapacityPolicy, validateOptions, objectFrames,
-  DEFAULTS, native,
+  gaussianPass, GaussianPass, GaussianBatch, IntersectionCapacityPolicy, LearnedCull, validateOptions,
+  objectFrames, checkObjectTables, psnr, DEFAULTS, native,
 };

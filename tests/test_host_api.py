@@ -58,7 +58,7 @@ def test_capacity_rule_c_matches_python(gs):
 
 @pytest.mark.parametrize("opts,err", [
     ({"depthSortMode": "float16"}, TypeError),
-    ({"antialiasMode": "msaa"}, TypeError),
+    ({"antialiasMode": "msaa"}, ValueError),  # RangeError, GaussianPass.ts:161-165
     ({"rasterTransmittanceThreshold": 0.0}, ValueError),
     ({"rasterTransmittanceThreshold": 1.0}, ValueError),
     ({"rasterChunkSize": 128}, ValueError),

@@ -239,3 +239,106 @@ def test_node_batch_matches_single_renders(gs, gpu_available, tmp_path):
     assert got["same"] == [True, True]
     assert all(s["overflow"] == 0 and s["errorFlags"] == 0 and s["visible"] > 0 for s in got["states"])
     assert got["threw"] == -5  # a disposed handle is GS_E_STATE
+
+
+def test_short_object_tables_are_rejected_before_the_library(gs):
+    """Every camera's object table must have the rows the library reads
+    (40 floats per object of setObjects); both host mirrors refuse a batch
+    whose tables differ, before any native call reads them."""
+    import importlib
+    batch = importlib.import_module("3dgs_tile_webgpu_amd.batch")
+    one, two = np.zeros((1, 40), np.float32), np.zeros((2, 40), np.float32)
+    batch.check_object_tables([two, two])
+    with pytest.raises(gs.GsRangeError):
+        batch.check_object_tables([two, one])
+    got = run_node("""
+      const t = (n) => new Float32Array(40 * n);
+      const out = [[t(2), t(2)], [t(2), t(1)], [t(1), new Float32Array(39)]].map((tables) => {
+        try { facade.checkObjectTables(tables); return 'ok'; } catch (e) { return e.constructor.name; }
+      });
+      console.log(JSON.stringify(out));""")
+    assert got == ["ok", "RangeError", "RangeError"]
+
+
+@pytest.mark.gpu
+def test_node_host_contract(gs, gpu_available, tmp_path):
+    """From JavaScript: a short per-frame object table is a RangeError from
+    the addon (not an over-read); redrawStrategy "auto" reuses a clean frame;
+    updateActive shrinks the active list; the batch autotuner returns the 18
+    shapes; the state carries the heavy-unit counter."""
+    scene = gs.workload_scene("small", count=20_000)
+    cam = gs.WORKLOADS["small"].camera()
+    payload = {"scene": write_scene(tmp_path, scene), "camera": camera_json(gs, cam)}
+    print("node host contract: small 20K", flush=True)
+    got = run_node(LOAD_SCENE + """
+      const c = camera(input.camera);
+      const pass = facade.gaussianPass(c, data, {redrawStrategy: 'auto', rasterStats: true,
+                                                 intersectionCapacity: 1 << 20});
+      pass.render(320, 180);
+      pass.render(320, 180);
+      const counts = [pass.renderCount, pass.cacheHitCount];
+      const st = pass.readState();
+      const frame = {projection: Float32Array.from(c.projectionMatrix.elements), width: 320,
+                     height: 180, near: c.near, far: c.far, objectFrames: new Float32Array(20)};
+      let short = null;
+      try { facade.native.render(pass._ctx, frame); } catch (e) { short = e.constructor.name; }
+      const slots = new Uint32Array(5000).map((v, i) => i);
+      pass.updateActive(0, slots, 5000);
+      pass.render(320, 180);
+      const visibleAfter = pass.readState().visible;
+      pass.dispose();
+      const batch = new facade.GaussianBatch(data, {initialIntersectionCapacity: 1 << 20});
+      const tuned = batch.autotune([c, c], [[320, 180], [256, 144]], 1);
+      batch.dispose();
+      console.log(JSON.stringify({counts, st, short, visibleAfter, tuned}));""", payload)
+    assert got["counts"] == [1, 1]
+    assert got["short"] == "RangeError"
+    st = got["st"]
+    assert st["rasterChecked"] > st["rasterCheckedHeavy"] >= 0 and st["visible"] > 5000
+    assert 0 < got["visibleAfter"] <= 5000
+    assert len(got["tuned"]["table"]) == 18 and got["tuned"]["best"]["tileSize"] in (8, 16, 32)
+
+
+@pytest.mark.gpu
+def test_node_trains_calibrates_and_enables_the_cull(gs, gpu_available, tmp_path):
+    """A Node script drives the learned cull end to end on the dense
+    3M-Gaussian scene at 1920x1080: collects training frames, fits the
+    normalisation, trains, calibrates the bias on 8 validation views (mean
+    PSNR >= 46 dB) and enables it; on 8 held-out views the culled frames
+    remove keys while keeping a mean PSNR >= 44 dB against the plain ones
+    (8 views vary by several dB around the validation mean)."""
+    scene = gs.workload_scene("3m_cull")
+    views = gs.orbit_cameras(112, seed=41, radius=(12.0, 25.0), resolutions=[(1920, 1080)])
+    payload = {"scene": write_scene(tmp_path, scene),
+               "cams": [camera_json(gs, c) for c, _ in views], "sizes": [s for _, s in views]}
+    print("node learned cull: 3M, 96 train + 8 valid + 8 held-out views at 1920x1080", flush=True)
+    got = run_node(LOAD_SCENE + """
+      const cams = input.cams.map(camera);
+      const batch = new facade.GaussianBatch(data, {initialIntersectionCapacity: 1 << 24});
+      const cull = new facade.LearnedCull(batch, 3);
+      cull.reserve(20000000);
+      for (let k = 0; k < 96; k++) cull.collect(cams[k], input.sizes[k][0], input.sizes[k][1], 0.05, k);
+      const n = cull.datasetSize();
+      cull.fitNormalization();
+      const loss0 = cull.train(1, {underWeight: 300, seed: 1});
+      cull.train(10000, {lr: 2e-3, underWeight: 300, seed: 2});
+      cull.train(5000, {lr: 5e-4, underWeight: 300, seed: 3});
+      const loss = cull.train(5000, {lr: 1e-4, underWeight: 300, seed: 4});
+      const cal = cull.calibrate(cams.slice(96, 104), input.sizes.slice(96, 104), 46);
+      const held = cams.slice(104), sizes = input.sizes.slice(104);
+      cull.disable();
+      batch.render(held, sizes);
+      const plain = held.map((c, v) => batch.readImage(v));
+      const keysPlain = batch.readStates().reduce((a, s) => a + s.intersections, 0);
+      cull.enable(1e-4);
+      batch.render(held, sizes);
+      const keysCull = batch.readStates().reduce((a, s) => a + s.intersections, 0);
+      const ps = held.map((c, v) => Math.min(999, facade.psnr(batch.readImage(v), plain[v])));
+      batch.dispose();
+      console.log(JSON.stringify({n, loss0, loss, shift: cal.shift, rows: cal.rows, keysPlain,
+                                  keysCull, ps}));""", payload, timeout=600)
+    print({k: v for k, v in got.items() if k != "rows"}, got["rows"][-3:])
+    assert got["n"] > 1_000_000 and got["loss"] < 0.5 * got["loss0"]
+    assert got["shift"] is not None
+    assert got["keysCull"] < 0.7 * got["keysPlain"]
+    assert np.mean(got["ps"]) >= 44.0
