@@ -830,8 +830,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 // one kernel per digit pass ("onesweep" with decoupled look-back):
 //  - k_sort_histograms counts the digits of every pass in one read of the
 //    keys (global digit totals, hence each digit's start in the output);
-//  - block b of k_sort_pass takes partition b of kSortPartition keys; wave
-//    w holds the partition's w-th run of 64 * kSortItems consecutive keys in
+//  - block b of k_sort_pass takes partition b of kSortThreads * kItems keys
+//    (kItems = 8, or 16 for sorts sized for at least kLargeSort keys); wave
+//    w holds the partition's w-th run of 64 * kItems consecutive keys in
 //    registers (key r*64 + lane of the run). Per round, the wave finds each
 //    key's same-digit peers with kBits + 1 ballots (the extra bit marks keys
 //    past the end); the key's rank in the run is the wave's running count of
@@ -855,8 +856,22 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 // exceeds the bound raises DeviceState::sort_error instead of waiting on.
 
 constexpr int kMaxPasses = 8;  // 32-bit keys with 4-bit digits
-constexpr int kLookbackWindow = 32;  // partitions read per look-back step
-constexpr uint32_t kHistBlocks = 256;  // blocks of the digit-total kernel
+// Partitions read per look-back step. Every step issues its loads
+// unconditionally, so a wide window mostly reads status rows it does not
+// need: measured on this GPU, 4 beats 1, 2, 8, 32 and 64 (1M/1080p frame:
+// depth + tile sort 0.133 ms at 4, 0.153 at 32, 0.182 at 64; 79 M tile keys:
+// 1.18 ms at 4, 1.57 at 32).
+constexpr int kLookbackWindow = 4;
+// Blocks of the digit-total kernel: a quarter of the 4096-key partitions,
+// between 256 and 1024 (79 M keys: 0.27 ms at 256 blocks, 0.17 at 1024; at
+// 1 M keys more blocks only add global atomics).
+constexpr uint32_t kHistBlocksMin = 256, kHistBlocksMax = 1024;
+// Sorts of at least this many keys (the capacity the grid is sized for) use
+// partitions of 16 keys per thread instead of 8: half the partitions, half
+// the look-back and status traffic (79 M tile keys: 1.07 -> 0.94 ms); below
+// it the larger partitions leave compute units idle (1M frame depth sort
+// 0.077 -> 0.087 ms).
+constexpr uint32_t kLargeSort = 16u << 20;
 constexpr int kHistItems = 8;          // keys per lane in flight (digit totals)
 constexpr int kSortWaves = kSortThreads / 64;
 // k_sort_pass gives each of the up to kMaxRadix digits its own thread (digit
@@ -956,21 +971,23 @@ __global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restr
     if (h[i]) atomicAdd(&ghist[(i / kRadix) * 256u + i % kRadix], h[i]);
 }
 
-template <int kBits>
+template <int kBits, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restrict__ in,
                                                       uint2* __restrict__ out,
                                                       const uint32_t* __restrict__ count_ptr,
                                                       uint32_t pass, SortWorkspace ws,
                                                       DeviceState* __restrict__ state,
                                                       uint32_t* __restrict__ in_place) {
+  static_assert(kItems % kSortItems == 0, "partitions are whole workspace partitions");
   constexpr uint32_t kRadix = 1u << kBits;
-  constexpr uint32_t kRun = 64u * kSortItems;  // keys of one wave, consecutive
+  constexpr uint32_t kRun = 64u * kItems;  // keys of one wave, consecutive
+  constexpr uint32_t kPartition = kSortThreads * kItems;
   __shared__ uint32_t wave_cnt[kSortWaves][kRadix];  // running counts, then offsets
   __shared__ uint32_t digit_base[kRadix];
   __shared__ uint32_t lds4[4];
   const uint32_t t = threadIdx.x;
   const uint32_t count = *count_ptr;
-  const uint32_t nparts = (count + kSortPartition - 1) / kSortPartition;
+  const uint32_t nparts = (count + kPartition - 1) / kPartition;
   const uint32_t part = blockIdx.x;
   if (in_place) {
     // The last pass: a digit shared by every key leaves the order as it is.
@@ -983,13 +1000,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   const uint32_t set = pass & 1u;
   const uint32_t shift = pass * kBits;
   const uint32_t wave = t >> 6, lane = lane_id();
-  const uint32_t run = part * (uint32_t)kSortPartition + wave * kRun;
+  const uint32_t run = part * kPartition + wave * kRun;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   // Every global load first: the wave's run of keys and this digit's total.
-  uint2 rec[kSortItems];
-  uint32_t dig[kSortItems], rank[kSortItems];
+  uint2 rec[kItems];
+  uint32_t dig[kItems], rank[kItems];
 #pragma unroll
-  for (int r = 0; r < kSortItems; r++) {
+  for (int r = 0; r < kItems; r++) {
     uint32_t i = run + r * 64u + lane;
     rec[r] = i < count ? in[i] : make_uint2(0u, 0u);
   }
@@ -1000,7 +1017,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   // A pass whose digit is the same for every key keeps the order: copy.
   if (__syncthreads_or(t < kRadix && my_total == count)) {
 #pragma unroll
-    for (int r = 0; r < kSortItems; r++) {
+    for (int r = 0; r < kItems; r++) {
       uint32_t i = run + r * 64u + lane;
       if (i < count) out[i] = rec[r];
     }
@@ -1010,7 +1027,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   // plus the wave's running count of the digit (read by every lane before
   // the group's highest lane adds the group size; the wave runs in step).
 #pragma unroll
-  for (int r = 0; r < kSortItems; r++) {
+  for (int r = 0; r < kItems; r++) {
     dig[r] = run + r * 64u + lane < count ? ((rec[r].x >> shift) & (kRadix - 1)) : kRadix;
     uint64_t peers = digit_peers<kBits>(dig[r]);
     uint32_t before = dig[r] < kRadix ? wave_cnt[wave][dig[r]] : 0u;
@@ -1076,7 +1093,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kSortItems; r++) {
+  for (int r = 0; r < kItems; r++) {
     uint32_t d = dig[r];
     if (d < kRadix) out[digit_base[d] + wave_cnt[wave][d] + rank[r]] = rec[r];
   }
@@ -2126,23 +2143,39 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 
 // Sort pairs in a <-> b for `passes` kBits-bit digits; the result lands in
 // a for an even pass count, in b otherwise.
+template <int kBits, int kItems>
+void sort_passes(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
+                 uint32_t passes, const SortWorkspace& ws, DeviceState* state,
+                 uint32_t* in_place, hipStream_t s) {
+  uint32_t blocks = div_up(max_count, (uint64_t)kSortThreads * kItems);
+  if (blocks == 0) blocks = 1;
+  for (uint32_t p = 0; p < passes; p++) {
+    const uint2* in = (p % 2 == 0) ? a : b;
+    uint2* out = (p % 2 == 0) ? b : a;
+    k_sort_pass<kBits, kItems><<<blocks, kSortThreads, 0, s>>>(
+        in, out, count_ptr, p, ws, state, p + 1 == passes ? in_place : nullptr);
+  }
+}
+
 template <int kBits>
 void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
                      uint32_t passes, const SortWorkspace& ws, DeviceState* state,
                      uint32_t* in_place, hipStream_t s) {
-  uint32_t blocks = div_up(max_count, kSortPartition);
-  if (blocks == 0) blocks = 1;
-  uint32_t clear_blocks = div_up((uint64_t)blocks * 256u + kMaxPasses * 256u, kBlock);
+  uint32_t parts = div_up(max_count, kSortPartition);
+  if (parts == 0) parts = 1;
+  uint32_t clear_blocks = div_up((uint64_t)parts * 256u + kMaxPasses * 256u, kBlock);
   if (clear_blocks > 1024) clear_blocks = 1024;
   k_sort_clear<<<clear_blocks, kBlock, 0, s>>>(ws, count_ptr);
-  k_sort_histograms<kBits><<<blocks < kHistBlocks ? blocks : kHistBlocks, kBlock, 0, s>>>(
-      a, count_ptr, passes, ws.ghist);
-  for (uint32_t p = 0; p < passes; p++) {
-    const uint2* in = (p % 2 == 0) ? a : b;
-    uint2* out = (p % 2 == 0) ? b : a;
-    k_sort_pass<kBits><<<blocks, kSortThreads, 0, s>>>(in, out, count_ptr, p, ws, state,
-                                                       p + 1 == passes ? in_place : nullptr);
-  }
+  uint32_t hist_blocks = parts / 4u;
+  hist_blocks = hist_blocks < kHistBlocksMin ? kHistBlocksMin
+                                             : (hist_blocks > kHistBlocksMax ? kHistBlocksMax
+                                                                             : hist_blocks);
+  if (hist_blocks > parts) hist_blocks = parts;
+  k_sort_histograms<kBits><<<hist_blocks, kBlock, 0, s>>>(a, count_ptr, passes, ws.ghist);
+  if (max_count >= kLargeSort)
+    sort_passes<kBits, 2 * kSortItems>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
+  else
+    sort_passes<kBits, kSortItems>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
 }
 
 // Sort pairs in a <-> b (see radix_sort_bits). With in_place, a last pass

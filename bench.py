@@ -44,34 +44,78 @@ def stage_bytes(n_active, visible, keys, tiles, pixels, depth_passes, tile_passe
         "visible": 4 * n_active + 4 * n_active + 8 * visible,  # flag in, offset out, record out
         "depth_sort": 16 * visible * depth_passes,      # read + write 8 B pair per pass
         "scan": 8 * visible + 4 * visible,              # gathered count in, offset out (+count)
-        "emit": 48 * visible + 8 * keys,
+        # emission reads each visible Gaussian's depth-ordered 16-byte tile-hit
+        # record (box and tile mask, made by projection) and its 4-byte write
+        # offset, and writes 8 B per key
+        "emit": 20 * visible + 8 * keys,
         "tile_sort": 16 * keys * tile_passes,
         "tile_offsets": 4 * keys + 4 * tiles,
         "raster": 8 * keys + 48 * keys + 8 * pixels,
     }
 
 
+def cpu_threads():
+    """Host threads this process may run at once: the CPU affinity set,
+    bounded by the cgroup CPU quota where one is set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def timed_oracle(gs_oracle, scene, frame, objects, threads, runs=5):
+    """Median wall time of `runs` oracle frames after one warm-up run."""
+    gs_oracle.run_frame(scene, frame, objects, capacity=1 << 28, threads=threads)
+    times, r = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        r = gs_oracle.run_frame(scene, frame, objects, capacity=1 << 28, threads=threads)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), r
+
+
 def cpu_baseline(gs, scene, camera, workload, threads):
+    """The oracle (the CPU restatement of the path) on the host cores: the
+    bench frame on every thread the process may use, and the 100k/720p frame
+    on those threads and on one (a bounded single-thread sample)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import gs_oracle  # the oracle, timed as the CPU restatement of the path
 
     frame = gs.frame_for(camera, workload.width, workload.height)
     objects = gs.object_frames_for(camera)
     gs_oracle.lib()
-    t0 = time.perf_counter()
-    r = gs_oracle.run_frame(scene, frame, objects, capacity=1 << 28, threads=threads)
-    secs = time.perf_counter() - t0
+    secs, r = timed_oracle(gs_oracle, scene, frame, objects, threads)
+    small = gs.WORKLOADS["small"]
+    s_scene = gs.workload_scene("small")
+    s_cam = small.camera()
+    s_frame = gs.frame_for(s_cam, small.width, small.height)
+    s_obj = gs.object_frames_for(s_cam)
+    s_all, _ = timed_oracle(gs_oracle, s_scene, s_frame, s_obj, threads)
+    s_one, _ = timed_oracle(gs_oracle, s_scene, s_frame, s_obj, 1)
     return {
         "value": round(1.0 / secs, 4),
         "unit": "frames/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one full frame of {workload.name} ({workload.count} Gaussians, "
-                  f"{workload.width}x{workload.height}) on {threads} host threads",
+        "sample": f"full frames of {workload.name} ({workload.count} Gaussians, "
+                  f"{workload.width}x{workload.height}) on {threads} host threads, median of 5 "
+                  "after one warm-up",
         "seconds": round(secs, 3),
         "stage_seconds": {s: round(v, 4) for s, v in zip(STAGES, r.counts.stage_seconds)},
         "keys": int(r.counts.count),
+        "nproc": os.cpu_count(),
+        "threads_available": threads,
         "cpu_model": _cpu_model(),
+        "small_frame": {"workload": small.name, "frames_per_s_all_threads": round(1.0 / s_all, 3),
+                        "frames_per_s_one_thread": round(1.0 / s_one, 3),
+                        "seconds_one_thread": round(s_one, 3)},
     }
 
 
@@ -163,6 +207,8 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     progress("learned cull: calibrating and evaluating")
     shift, rows = cull.calibrate([c for c, _ in valid], [s for _, s in valid],
                                  target_psnr=46.0)
+    if shift is None:
+        raise RuntimeError("the learned cull reached no calibration shift at 46 dB")
     cams, sizes = [c for c, _ in held], [s for _, s in held]
 
     def timed():
@@ -186,7 +232,31 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     k_cull = sum(x.intersections for x in st_cull)
     v_plain = sum(x.visible for x in st_plain)
     v_cull = sum(x.visible for x in st_cull)
+    # Culls without the network at the same mean PSNR on the same views: an
+    # opacity bound and a footprint (pixel radius) bound, their thresholds
+    # swept; each keeps the largest key reduction whose mean PSNR reaches the
+    # learned cull's.
+    learned = cull.params()
+    target = float(np.mean(ps))
+    lc = importlib.import_module("3dgs_tile_webgpu_amd.learned_cull")
+    baselines = {}
+    for name, feature, scale, taus in (
+            ("opacity_bound", lc.FEATURE_OPACITY, 10.0, [0.01, 0.02, 0.04, 0.06, 0.08, 0.1, 0.12,
+                                                          0.15, 0.2]),
+            ("footprint_bound", lc.FEATURE_LOG_RADIUS, 4.0, [0.25, 0.5, 0.75, 1.0, 1.25, 1.5])):
+        best = 0.0
+        for tau in taus:
+            cull.set_linear_score(feature, scale, float(np.log(1e-4)) - scale * tau)
+            cull.enable(1e-4)
+            b.render(cams, sizes)
+            keys = sum(x.intersections for x in b.read_states())
+            mean = float(np.mean([gs.psnr(b.read_image(v), plain[v]) for v in range(len(cams))]))
+            if mean >= target:
+                best = max(best, 1 - keys / max(1, k_plain))
+        baselines[name] = round(best, 4)
+    cull.set_params(learned)
     b.dispose()
+    chosen = next(r for r in rows if r[0] == shift)
     return {
         "workload": w.name, "gaussians": w.count, "resolution": f"{w.width}x{w.height}",
         "train_views": train_views, "eval_views": eval_views, "calibration_views": calib_views, "threshold": 1e-4,
@@ -199,6 +269,9 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
         "training_samples": n, "collect_seconds": round(collect_s, 2),
         "train_seconds": round(train_s, 2), "final_loss": round(loss, 4),
         "bias_shift": shift,
+        "calibration_psnr_mean_db": round(chosen[2], 2),
+        "calibration_psnr_min_db": round(chosen[3], 2),
+        "non_learned_keys_removed_at_same_psnr": baselines,
         "calibration": [{"shift": r[0], "keys_kept": round(r[1], 4), "psnr_mean": round(r[2], 2),
                          "psnr_min": round(r[3], 2)} for r in rows],
     }
@@ -307,7 +380,7 @@ def main():
         p2.dispose()
         flops = checked_heavy * 21
         achieved = flops / (heavy_kernel_ms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+        roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                     "traffic": (profiled_traffic("gs::k_raster_heavy<16, 0, false>")
                                 if args.workload == "1m_1080p" else None),
@@ -354,7 +427,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         progress("cpu baseline")
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = args.cpu_threads or cpu_threads()
         result["cpu_baseline"] = cpu_baseline(gs, scene, camera, w, threads)
     if not args.no_extras:
         del scene
