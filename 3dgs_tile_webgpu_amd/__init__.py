@@ -10,6 +10,8 @@ Public API (mirrors the reference's pass API for the frame path):
   LearnedCull                    -- the MLP cull fused into projection,
                                     trained on the GPU from plain frames
   PackedScene, workload_scene    -- packed buffers and synthetic workloads
+  read_ply, pack_scene           -- canonical 3DGS PLY files into the store's
+                                    slot buffers (holes beyond the cloud)
   PerspectiveCamera              -- Three.js camera conventions
 """
 from ._native import GsError, GsRangeError, GsTypeError, lib  # noqa: F401
@@ -25,4 +27,4 @@ from .gaussian_pass import (  # noqa: F401
 )
 from .learned_cull import LearnedCull, psnr  # noqa: F401
 from .scene import (BATCH_RESOLUTIONS, WORKLOADS, PackedScene, Workload, generate,  # noqa: F401
-                    spec_for, workload_scene)
+                    pack_scene, read_ply, spec_for, workload_scene)

@@ -151,6 +151,10 @@ class GsSceneSpec(C.Structure):
     ]
 
 
+class GsPlyInfo(C.Structure):
+    _fields_ = [("count", u32), ("sh_degree", u32), ("format", u32)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h.
 SIGNATURES = {
     "gs_config_default": (None, [C.POINTER(GsConfig)]),
@@ -203,6 +207,9 @@ SIGNATURES = {
     "gs_pack_sh_rgb8e8": (u32, [P]),
     "gs_encode_sh_rgb8e8": (None, [C.c_size_t, P, P]),
     "gs_f32_to_f16": (C.c_uint16, [f32]),
+    "gs_ply_parse": (C.c_int, [P, C.c_size_t, C.POINTER(GsPlyInfo), P, P, P, P, P]),
+    "gs_ply_last_error": (C.c_char_p, []),
+    "gs_pack_scene": (C.c_int, [C.c_size_t, C.c_size_t, u32, u32, P, P, P, P, P, P, P, P, P]),
     "gs_scene_spec_default": (None, [C.POINTER(GsSceneSpec)]),
     "gs_generate_scene": (C.c_int, [C.POINTER(GsSceneSpec), P, P, P, P, P, P, P]),
 }

@@ -2,10 +2,15 @@
 //
 // Host code (no device work). The encoder follows the reference's Rust
 // producer operation by operation in binary32; see include/gs_producer.h.
+#include <ctype.h>
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -97,6 +102,303 @@ extern "C" uint32_t gs_pack_sh_rgb8e8(const float rgb[3]) {
 
 extern "C" void gs_encode_sh_rgb8e8(size_t n, const float* rgb, uint32_t* out) {
   for (size_t i = 0; i < n; i++) out[i] = gs_pack_sh_rgb8e8(rgb + i * 3);
+}
+
+// ------------------------------------------------------------------ PLY ----
+//
+// Canonical 3DGS PLY files (CanonicalGaussianPlyLoader.ts:84-350): an ascii,
+// binary_little_endian or binary_big_endian header whose first non-empty
+// element is `vertex`, with scalar properties x, y, z, scale_0..2,
+// rot_0..3 (w, x, y, z), opacity (logit), f_dc_0..2 and f_rest_0..(3K - 1)
+// laid out channel-major (K higher-band coefficients per channel).
+
+namespace {
+
+struct PlyProp {
+  std::string name;
+  int type;  // byte size, negative for signed integers, 40 / 80 for float / double
+  size_t offset;
+};
+
+thread_local char g_ply_error[160];
+
+int ply_fail(const char* msg) {
+  strncpy(g_ply_error, msg, sizeof(g_ply_error) - 1);
+  g_ply_error[sizeof(g_ply_error) - 1] = 0;
+  return -1;
+}
+
+// Scalar type name -> (size, kind): kind 0 unsigned, 1 signed, 2 float.
+bool ply_type(const std::string& t, int& size, int& kind) {
+  static const struct { const char* name; int size, kind; } types[] = {
+      {"char", 1, 1},   {"int8", 1, 1},    {"uchar", 1, 0},   {"uint8", 1, 0},
+      {"short", 2, 1},  {"int16", 2, 1},   {"ushort", 2, 0},  {"uint16", 2, 0},
+      {"int", 4, 1},    {"int32", 4, 1},   {"uint", 4, 0},    {"uint32", 4, 0},
+      {"float", 4, 2},  {"float32", 4, 2}, {"double", 8, 2},  {"float64", 8, 2}};
+  for (const auto& e : types)
+    if (t == e.name) {
+      size = e.size;
+      kind = e.kind;
+      return true;
+    }
+  return false;
+}
+
+struct PlyHeader {
+  int format = -1;  // 0 ascii, 1 little endian, 2 big endian
+  uint64_t count = 0;
+  size_t stride = 0, data = 0;
+  std::vector<std::string> names;
+  std::vector<int> sizes, kinds;
+  std::vector<size_t> offsets;
+};
+
+int ply_header(const unsigned char* b, size_t n, PlyHeader& h) {
+  static const char kEnd[] = "end_header";
+  size_t mark = n;
+  for (size_t i = 0; i + 10 <= n; i++)
+    if (memcmp(b + i, kEnd, 10) == 0) {
+      mark = i;
+      break;
+    }
+  if (mark == n) return ply_fail("Invalid PLY: end_header is missing");
+  size_t data = mark + 10;
+  if (data < n && b[data] == 13) data++;
+  if (data >= n || b[data] != 10) return ply_fail("Invalid PLY: end_header must terminate a line");
+  data++;
+  std::string text((const char*)b, data);
+  std::vector<std::string> lines;
+  size_t at = 0;
+  while (at < text.size()) {
+    size_t e = text.find('\n', at);
+    if (e == std::string::npos) e = text.size();
+    std::string line = text.substr(at, e - at);
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    lines.push_back(line);
+    at = e + 1;
+  }
+  auto tokens = [](const std::string& l) {
+    std::vector<std::string> t;
+    size_t i = 0;
+    while (i < l.size()) {
+      while (i < l.size() && isspace((unsigned char)l[i])) i++;
+      size_t j = i;
+      while (j < l.size() && !isspace((unsigned char)l[j])) j++;
+      if (j > i) t.push_back(l.substr(i, j - i));
+      i = j;
+    }
+    return t;
+  };
+  if (lines.empty() || tokens(lines[0]) != std::vector<std::string>{"ply"})
+    return ply_fail("Invalid PLY signature");
+  std::string element;
+  bool vertex_seen = false, first_nonempty_checked = false;
+  int64_t vertex_count = -1;
+  for (const std::string& line : lines) {
+    std::vector<std::string> t = tokens(line);
+    if (t.empty()) continue;
+    if (t[0] == "format") {
+      if (t.size() < 2) return ply_fail("Unsupported PLY format: unknown");
+      if (t[1] == "ascii") h.format = 0;
+      else if (t[1] == "binary_little_endian") h.format = 1;
+      else if (t[1] == "binary_big_endian") h.format = 2;
+      else return ply_fail("Unsupported PLY format");
+    } else if (t[0] == "element") {
+      element = t.size() > 1 ? t[1] : "";
+      char* end = nullptr;
+      long long c = t.size() > 2 ? strtoll(t[2].c_str(), &end, 10) : -1;
+      if (t.size() < 3 || c < 0 || !end || *end) return ply_fail("Invalid element count");
+      if (element == "vertex") {
+        vertex_count = c;
+        vertex_seen = true;
+      }
+      if (c > 0 && !first_nonempty_checked) {
+        first_nonempty_checked = true;
+        if (element != "vertex")
+          return ply_fail("The canonical 3DGS vertex element must be first");
+      }
+    } else if (t[0] == "property" && element == "vertex") {
+      if (t.size() > 1 && t[1] == "list")
+        return ply_fail("List properties are not supported in the vertex element");
+      int size = 0, kind = 0;
+      if (t.size() < 3 || !ply_type(t[1], size, kind))
+        return ply_fail("Unsupported vertex property");
+      h.names.push_back(t[2]);
+      h.sizes.push_back(size);
+      h.kinds.push_back(kind);
+      h.offsets.push_back(h.stride);
+      h.stride += (size_t)size;
+    }
+  }
+  if (h.format < 0) return ply_fail("Invalid PLY: format is missing");
+  if (!vertex_seen || vertex_count <= 0) return ply_fail("PLY must contain at least one vertex");
+  if (vertex_count > 0xffffffffll) return ply_fail("PLY vertex count exceeds 2^32 - 1");
+  h.count = (uint64_t)vertex_count;
+  h.data = data;
+  return 0;
+}
+
+double ply_scalar(const unsigned char* p, int size, int kind, bool little) {
+  unsigned char tmp[8];
+  for (int i = 0; i < size; i++) tmp[i] = little ? p[i] : p[size - 1 - i];
+  if (kind == 2) {
+    if (size == 4) {
+      float f;
+      memcpy(&f, tmp, 4);
+      return f;
+    }
+    double d;
+    memcpy(&d, tmp, 8);
+    return d;
+  }
+  uint64_t u = 0;
+  memcpy(&u, tmp, (size_t)size);
+  if (kind == 1) {
+    int64_t v = (int64_t)(u << (64 - 8 * size)) >> (64 - 8 * size);  // sign-extend
+    return (double)v;
+  }
+  return (double)u;
+}
+
+}  // namespace
+
+extern "C" const char* gs_ply_last_error(void) { return g_ply_error; }
+
+extern "C" int gs_ply_parse(const void* data, size_t size, gs_ply_info* info, float* means,
+                            float* scales, float* opacity, float* quats, float* sh) {
+  g_ply_error[0] = 0;
+  if (!data || !info) return ply_fail("missing PLY data");
+  const unsigned char* b = (const unsigned char*)data;
+  PlyHeader h;
+  if (ply_header(b, size, h)) return -1;
+  auto index = [&](const std::string& name) -> int {
+    for (size_t i = 0; i < h.names.size(); i++)
+      if (h.names[i] == name) return (int)i;
+    return -1;
+  };
+  static const char* kRequired[] = {"x",     "y",     "z",     "scale_0", "scale_1",
+                                    "scale_2", "rot_0", "rot_1", "rot_2",   "rot_3",
+                                    "opacity", "f_dc_0", "f_dc_1", "f_dc_2"};
+  int req[14];
+  for (int i = 0; i < 14; i++)
+    if ((req[i] = index(kRequired[i])) < 0) {
+      snprintf(g_ply_error, sizeof(g_ply_error),
+               "Not a canonical 3DGS PLY: missing property %s", kRequired[i]);
+      return -1;
+    }
+  // f_rest_k must be exactly 0..R-1 with R divisible by three.
+  std::vector<int> rest;
+  {
+    std::vector<std::pair<long, int>> found;
+    for (size_t i = 0; i < h.names.size(); i++) {
+      const std::string& n = h.names[i];
+      if (n.compare(0, 7, "f_rest_") != 0 || n.size() == 7) continue;
+      bool digits = true;
+      for (size_t k = 7; k < n.size(); k++) digits = digits && isdigit((unsigned char)n[k]);
+      if (!digits) continue;
+      found.push_back({strtol(n.c_str() + 7, nullptr, 10), (int)i});
+    }
+    std::sort(found.begin(), found.end());
+    for (size_t k = 0; k < found.size(); k++) {
+      if (found[k].first != (long)k)
+        return ply_fail("f_rest_* properties must be contiguous from f_rest_0");
+      rest.push_back(found[k].second);
+    }
+  }
+  if (rest.size() % 3 != 0) return ply_fail("f_rest_* property count must be divisible by three");
+  size_t per_channel = rest.size() / 3, coefs = per_channel + 1;
+  uint32_t degree = coefs == 1 ? 0u : coefs == 4 ? 1u : coefs == 9 ? 2u : coefs == 16 ? 3u : 9u;
+  if (degree > 3)
+    return ply_fail("PLY must contain one, four, nine, or sixteen SH coefficients per channel");
+  info->count = (uint32_t)h.count;
+  info->sh_degree = degree;
+  info->format = (uint32_t)h.format;
+  if (!means && !scales && !opacity && !quats && !sh) return 0;  // header only
+  if (!means || !scales || !opacity || !quats || !sh) return ply_fail("missing output arrays");
+  const size_t nprop = h.names.size();
+  std::vector<double> ascii;
+  if (h.format == 0) {
+    // Whitespace-separated numbers, count * properties of them.
+    ascii.resize(h.count * nprop);
+    size_t at = h.data;
+    for (size_t k = 0; k < ascii.size(); k++) {
+      while (at < size && isspace(b[at])) at++;
+      size_t start = at;
+      while (at < size && !isspace(b[at])) at++;
+      std::string tok((const char*)b + start, at - start);
+      char* end = nullptr;
+      double v = strtod(tok.c_str(), &end);
+      if (tok.empty() || !end || *end || !std::isfinite(v)) {
+        snprintf(g_ply_error, sizeof(g_ply_error), "Invalid ASCII PLY value at scalar %zu", k);
+        return -1;
+      }
+      ascii[k] = v;
+    }
+  } else if (h.data + h.count * h.stride > size) {
+    return ply_fail("Binary PLY ends before the vertex data is complete");
+  }
+  const bool little = h.format == 1;
+  auto read = [&](size_t v, int prop) -> double {
+    if (h.format == 0) return ascii[v * nprop + (size_t)prop];
+    return ply_scalar(b + h.data + v * h.stride + h.offsets[(size_t)prop], h.sizes[(size_t)prop],
+                      h.kinds[(size_t)prop], little);
+  };
+  // CanonicalGaussianPlyLoader.ts:130-175, evaluated in double like the
+  // JavaScript and stored as float32.
+  for (size_t g = 0; g < h.count; g++) {
+    for (int k = 0; k < 3; k++) means[g * 3 + k] = (float)read(g, req[k]);
+    for (int k = 0; k < 3; k++) scales[g * 3 + k] = (float)std::max(exp(read(g, req[3 + k])), 1e-6);
+    opacity[g] = (float)(1.0 / (1.0 + exp(-read(g, req[10]))));
+    double w = read(g, req[6]), x = read(g, req[7]), y = read(g, req[8]), z = read(g, req[9]);
+    double len = sqrt(x * x + y * y + z * z + w * w);  // Math.hypot
+    if (len > 1e-12) {
+      quats[g * 4 + 0] = (float)(x / len);
+      quats[g * 4 + 1] = (float)(y / len);
+      quats[g * 4 + 2] = (float)(z / len);
+      quats[g * 4 + 3] = (float)(w / len);
+    } else {
+      quats[g * 4 + 0] = quats[g * 4 + 1] = quats[g * 4 + 2] = 0.0f;
+      quats[g * 4 + 3] = 1.0f;
+    }
+    float* c = sh + g * coefs * 3;
+    for (int k = 0; k < 3; k++) c[k] = (float)read(g, req[11 + k]);
+    for (size_t j = 1; j < coefs; j++)
+      for (size_t ch = 0; ch < 3; ch++)
+        c[j * 3 + ch] = (float)read(g, rest[ch * per_channel + (j - 1)]);
+  }
+  return 0;
+}
+
+// The store's slot buffers for one cloud (engine.rs:1801-1949 with the
+// hole rows of :1917-1923): slots [0, n) hold the Gaussians in order with
+// the object id in means.w; slots [n, capacity) are holes (means.w = -1,
+// every other word 0), which projection skips.
+extern "C" int gs_pack_scene(size_t n, size_t capacity, uint32_t object_id, uint32_t sh_degree,
+                             const float* means3, const float* scales, const float* opacity,
+                             const float* quats, const float* sh, float* out_means,
+                             uint32_t* out_shape, uint32_t* out_rotation, uint32_t* out_sh) {
+  if (n > capacity || sh_degree > 3) return -1;
+  const size_t coefs = (size_t)(sh_degree + 1) * (sh_degree + 1);
+  for (size_t i = 0; i < capacity; i++) {
+    float* m = out_means + i * 4;
+    if (i < n) {
+      m[0] = means3[i * 3];
+      m[1] = means3[i * 3 + 1];
+      m[2] = means3[i * 3 + 2];
+      m[3] = (float)object_id;
+    } else {
+      m[0] = m[1] = m[2] = 0.0f;
+      m[3] = -1.0f;
+    }
+  }
+  gs_encode_compact(n, scales, opacity, quats, out_shape, out_rotation);
+  gs_encode_sh_rgb8e8(n * coefs, sh, out_sh);
+  if (capacity > n) {
+    memset(out_shape + n * 2, 0, (capacity - n) * 2 * sizeof(uint32_t));
+    memset(out_rotation + n, 0, (capacity - n) * sizeof(uint32_t));
+    memset(out_sh + n * coefs, 0, (capacity - n) * coefs * sizeof(uint32_t));
+  }
+  return 0;
 }
 
 namespace {

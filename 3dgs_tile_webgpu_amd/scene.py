@@ -121,5 +121,43 @@ def generate(spec: N.GsSceneSpec, keep_raw: bool = False) -> PackedScene:
     return PackedScene(means, shape, rot, sh, deg, None, raw)
 
 
+def read_ply(data: bytes) -> dict:
+    """A canonical 3DGS PLY (CanonicalGaussianPlyLoader.ts:84-350) as float
+    arrays: means (n, 3), scales (n, 3), opacity (n,), quats (n, 4) xyzw,
+    sh (n, (deg+1)^2, 3), sh_degree."""
+    lib = N.lib()
+    buf = np.frombuffer(data, np.uint8)
+    info = N.GsPlyInfo()
+    if lib.gs_ply_parse(buf.ctypes.data, buf.size, C.byref(info), None, None, None, None, None):
+        raise ValueError(lib.gs_ply_last_error().decode())
+    n, deg = info.count, info.sh_degree
+    out = {"means": np.empty((n, 3), np.float32), "scales": np.empty((n, 3), np.float32),
+           "opacity": np.empty(n, np.float32), "quats": np.empty((n, 4), np.float32),
+           "sh": np.empty((n, (deg + 1) ** 2, 3), np.float32)}
+    if lib.gs_ply_parse(buf.ctypes.data, buf.size, C.byref(info), *(out[k].ctypes.data for k in
+                        ("means", "scales", "opacity", "quats", "sh"))):
+        raise ValueError(lib.gs_ply_last_error().decode())
+    out["sh_degree"] = deg
+    return out
+
+
+def pack_scene(g: dict, capacity: int | None = None, object_id: int = 0) -> PackedScene:
+    """The store's slot buffers of one cloud (engine.rs:1801-1949): the
+    Gaussians in slots [0, n), holes (means.w = -1) up to `capacity`."""
+    n = g["means"].shape[0]
+    capacity = n if capacity is None else capacity
+    deg = int(g["sh_degree"])
+    ncoef = (deg + 1) ** 2
+    means = np.empty((capacity, 4), np.float32)
+    shape = np.empty((capacity, 2), np.uint32)
+    rot = np.empty(capacity, np.uint32)
+    sh = np.empty((capacity, ncoef), np.uint32)
+    src = [np.ascontiguousarray(g[k], np.float32) for k in ("means", "scales", "opacity", "quats", "sh")]
+    if N.lib().gs_pack_scene(n, capacity, object_id, deg, *(a.ctypes.data for a in src),
+                             means.ctypes.data, shape.ctypes.data, rot.ctypes.data, sh.ctypes.data):
+        raise ValueError("pack_scene: n exceeds capacity or SH degree above 3")
+    return PackedScene(means, shape, rot, sh, deg)
+
+
 def workload_scene(name: str, count: int | None = None, keep_raw: bool = False) -> PackedScene:
     return generate(spec_for(WORKLOADS[name], count), keep_raw)

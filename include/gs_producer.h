@@ -38,6 +38,38 @@ void gs_encode_sh_rgb8e8(size_t n, const float* rgb, uint32_t* out);
 /* IEEE binary16 round-to-nearest-even (half::f16::from_f32). */
 uint16_t gs_f32_to_f16(float v);
 
+/* Canonical 3DGS PLY reader (src/streaming-backend-impl/
+ * CanonicalGaussianPlyLoader.ts:84-350): ascii, binary_little_endian or
+ * binary_big_endian, vertex element first, scalar properties x y z,
+ * scale_0..2 (log), rot_0..3 (w x y z), opacity (logit), f_dc_0..2 and a
+ * contiguous f_rest_0..(3K-1) block (channel-major), K in {0, 3, 8, 15}.
+ * Outputs, as the loader computes them: means (count * 3), scales
+ * max(exp(s), 1e-6) (count * 3), opacity sigmoid(o) (count), quaternions
+ * normalised, (x, y, z, w), identity when degenerate (count * 4), SH
+ * coefficients (count * (deg+1)^2 * 3: DC first, rgb per coefficient).
+ * Call with every output NULL to read only *info. Returns 0, or -1 with the
+ * reference's message in gs_ply_last_error() (per thread). */
+typedef struct gs_ply_info {
+  uint32_t count;
+  uint32_t sh_degree;
+  uint32_t format; /* 0 ascii, 1 binary little endian, 2 binary big endian */
+} gs_ply_info;
+int gs_ply_parse(const void* data, size_t size, gs_ply_info* info, float* means, float* scales,
+                 float* opacity, float* quats, float* sh);
+const char* gs_ply_last_error(void);
+
+/* The store's slot buffers for one cloud (engine.rs:1801-1949, hole rows
+ * :1917-1923): `capacity` slots, the first n holding the Gaussians in order
+ * (means.w = object_id; compact shape and oct101012 rotation as
+ * gs_encode_compact; SH as gs_encode_sh_rgb8e8), the rest holes with
+ * means.w = -1 and every other word 0. out_means: capacity * 4 f32;
+ * out_shape: capacity * 2; out_rotation: capacity; out_sh: capacity *
+ * (deg+1)^2. Returns 0, or -1 when n > capacity or sh_degree > 3. */
+int gs_pack_scene(size_t n, size_t capacity, uint32_t object_id, uint32_t sh_degree,
+                  const float* means3, const float* scales, const float* opacity,
+                  const float* quats, const float* sh, float* out_means, uint32_t* out_shape,
+                  uint32_t* out_rotation, uint32_t* out_sh);
+
 typedef struct gs_scene_spec {
   uint64_t seed;
   uint32_t count;

@@ -198,3 +198,18 @@ def test_1m_1080p_bit_exact(gs, oracle, gpu_available, opts):
     assert ref.counts.chunk_tasks > 0  # dense cluster tiles take the chunk path
     assert_frame_matches(p, ref)
     p.dispose()
+
+
+def test_ply_scene_with_holes_bit_exact(gs, oracle, gpu_available):
+    """A real-scene file through the producer: the canonical PLY fixture
+    packed into 96 slots (32 hole rows, means.w = -1) renders bit for bit
+    like the oracle's frame of it."""
+    from pathlib import Path
+
+    g = gs.read_ply((Path(__file__).resolve().parent / "golden" / "scene_64.ply").read_bytes())
+    scene = gs.pack_scene(g, capacity=96)
+    cam = gs.PerspectiveCamera(fov=60.0, aspect=1.0, near=0.1, far=100.0, position=(0.0, 0.0, 4.0))
+    p, ref = render_both(gs, oracle, scene, cam, 64, 64)
+    assert ref.counts.visible > 32
+    assert_frame_matches(p, ref)
+    p.dispose()
