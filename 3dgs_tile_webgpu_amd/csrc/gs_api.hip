@@ -39,8 +39,6 @@ struct DeviceBuffer {
   size_t bytes = 0;
 };
 
-constexpr uint32_t kMaxLanes = 16;
-
 struct Lane {
   hipStream_t stream = nullptr;  // lane 0: the context stream (not owned)
   hipStream_t aux = nullptr;     // second raster branch, forked per frame
@@ -139,13 +137,24 @@ int hip_fail(gs_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, GS_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Grow-only device allocation (AttributePool.ts:41-67 semantics). Every
-// lane joins the context stream, so synchronising it retires all users.
+// Wait for everything this context enqueued: the context stream, each
+// lane's stream and each raster branch stream.
+hipError_t sync_streams(gs_ctx* ctx) {
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  for (const Lane& l : ctx->lanes) {
+    if (e == hipSuccess && l.stream && l.stream != ctx->stream) e = hipStreamSynchronize(l.stream);
+    if (e == hipSuccess && l.aux) e = hipStreamSynchronize(l.aux);
+  }
+  return e;
+}
+
+// Grow-only device allocation (AttributePool.ts:41-67 semantics). A buffer
+// is freed only once every stream of the context has drained.
 int ensure(gs_ctx* ctx, DeviceBuffer& b, size_t bytes, const char* what) {
   if (bytes == 0) bytes = 16;
   if (b.bytes >= bytes) return GS_OK;
   if (b.ptr) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = sync_streams(ctx);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
     (void)hipFree(b.ptr);
     b.ptr = nullptr;
@@ -232,7 +241,8 @@ int lane_sort(gs_ctx* ctx, Lane& l) {
   if ((rc = ensure(ctx, l.hist, sort_words * 4, "radix-sort-workspace")) ||
       (rc = ensure(ctx, l.partials, scan_bytes, "scan-workspace")))
     return rc;
-  hipError_t e = hipMemsetAsync(l.partials.ptr, 0, l.partials.bytes, l.stream);
+  // On the context stream: every lane's work forks from it after this.
+  hipError_t e = hipMemsetAsync(l.partials.ptr, 0, l.partials.bytes, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync");
   l.sort_capacity = m;
   l.sort_parts = parts;
@@ -956,7 +966,16 @@ int launch_set(gs_ctx* ctx, ViewSet& set, bool profile) {
       set.graph_valid = false;
       return hip_fail(ctx, e, "frame capture");
     }
+    // An update in place only for graphs of one lane (its frames and their
+    // raster branches, a topology the re-capture tests cover); any other
+    // graph, or a failed update, is instantiated afresh.
     bool updated = false;
+    bool one_lane = std::all_of(set.lane_of.begin(), set.lane_of.end(),
+                                [](uint32_t l) { return l == 0; });
+    if (set.exec && !one_lane) {
+      (void)hipGraphExecDestroy(set.exec);
+      set.exec = nullptr;
+    }
     if (set.exec) {
       hipGraphExecUpdateResult result;
       hipGraphNode_t error_node = nullptr;
@@ -1062,6 +1081,16 @@ int render_set(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames,
     if (ctx->cfg.output_depth && !depth_dst[v] &&
         (rc = ensure(ctx, set.depth[v], pixels * 4, "depth-target")))
       return rc;
+  }
+  // Every view's lane exists and its working set covers the view.
+  for (uint32_t v = 0; v < n; v++) {
+    uint32_t l = set.lane_of[v];
+    if (l >= ctx->lanes.size()) return fail(ctx, GS_E_STATE, "view assigned to no lane");
+    const Lane& lane = ctx->lanes[l];
+    uint32_t tiles = div_up(frames[v].width, ts) * div_up(frames[v].height, ts);
+    if (lane.capacity < set.capacity[v] || lane.tile_capacity < tiles ||
+        lane.slot_capacity < ctx->active_count)
+      return fail(ctx, GS_E_STATE, "lane working set smaller than its view");
   }
   set.frames.resize(n);
   for (uint32_t v = 0; v < n; v++) {

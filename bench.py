@@ -119,21 +119,30 @@ def cpu_baseline(gs, scene, camera, workload, threads):
     }
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "frame_1m_1080p_hbm_traffic.md"
+# Committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries per workload, and the
+# heavy raster instance that workload's frames run (whole tiles at 1M/1080p,
+# 4-row slabs at 6M/4K).
+TRAFFIC_PROFILES = {
+    "1m_1080p": ("profiles/frame_1m_1080p_hbm_traffic.md", "gs::k_raster_heavy<16, 0, false>"),
+    "6m_4k": ("profiles/frame_6m_4k_hbm_traffic.md", "gs::k_raster_heavy<16, 2, false>"),
+}
 
 
-def profiled_traffic(kernel_prefix):
-    """HBM bytes per dispatch of a kernel (read + written, counter-scaled) from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 1M/1080p
-    frame, or None when the summary has no such row."""
+def profiled_traffic(workload):
+    """HBM bytes per dispatch of the workload's heavy raster kernel (read +
+    written, counter-scaled) from its committed summary, with the summary's
+    path; (None, None) when there is no summary or no such row."""
+    if workload not in TRAFFIC_PROFILES:
+        return None, None
+    rel, kernel = TRAFFIC_PROFILES[workload]
     try:
-        for line in TRAFFIC_PROFILE.read_text().splitlines():
+        for line in (ROOT / rel).read_text().splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 6 and cells[0].strip("`").startswith(kernel_prefix):
-                return int(round((float(cells[4]) + float(cells[5])) * 1e6))
+            if len(cells) == 6 and cells[0].strip("`") == kernel:
+                return int(round((float(cells[4]) + float(cells[5])) * 1e6)), rel
     except (OSError, ValueError):
         pass
-    return None
+    return None, None
 
 
 def progress(msg):
@@ -380,12 +389,12 @@ def main():
         p2.dispose()
         flops = checked_heavy * 21
         achieved = flops / (heavy_kernel_ms * 1e-3) / 1e12
+        traffic, traffic_file = profiled_traffic(args.workload)
         roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                    "traffic": (profiled_traffic("gs::k_raster_heavy<16, 0, false>")
-                                if args.workload == "1m_1080p" else None),
-                    "traffic_source": "profiles/frame_1m_1080p_hbm_traffic.md (rocprofv3 "
-                                      "FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                    "traffic": traffic,
+                    "traffic_source": (f"{traffic_file} (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                       "copy-calibrated)" if traffic_file else None),
                     "kernel": "k_raster_heavy",
                     "kernel_ms": round(heavy_kernel_ms, 4), "algorithmic_flops": int(flops),
                     "checked_pairs_kernel": int(checked_heavy), "checked_pairs_frame": int(checked),

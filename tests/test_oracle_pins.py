@@ -1,4 +1,4 @@
-"""Independent pins of three oracle functions that the bit-exact GPU tests
+"""Independent pins of four oracle functions that the bit-exact GPU tests
 cannot pin by themselves (the kernels were written from the same WGSL):
 
 - SH colour (projectionHelpers.ts:118-179): against a float64 real
@@ -13,6 +13,9 @@ cannot pin by themselves (the kernels were written from the same WGSL):
 - Depth finalisation (finalizeGaussianDepth.ts:18-68 with the default depth
   node, GaussianContextNodes.ts:125-131): the known answers of
   test/ScalarOutput.test.ts:61-96 and the closed form of the encoding.
+- The raster work counters (TileRasterizer.ts:617, :750-768): restated per
+  pixel and chunk task over the oracle's own sorted records, vectorised in
+  numpy, and shown to tell apart the plausible miscounts.
 """
 from __future__ import annotations
 
@@ -314,3 +317,85 @@ def test_default_depth_encoding_closed_form(oracle):
     assert oracle.finalize_depth(3.0 * 0.9, 0.1, near=near, far=far) == 1.0  # coverage 0.9 < 0.95
     assert oracle.finalize_depth(3.0 * 0.9, 0.1, near=near, far=far, alpha_threshold=0.5) \
         == pytest.approx((near - 3.0) * far / ((far - near) * -3.0), rel=1e-6)
+
+
+# ------------------------------------------------------- raster counters ----
+
+def counted_pairs(ref, width, height, chunk, thr_t=1e-4, edge=16, slip=None):
+    """The reference's raster counters restated over the oracle's sorted
+    records and projected values (TileRasterizer.ts:547-709): per in-image
+    pixel and per chunk task (the whole tile when it is not chunked), every
+    record up to and including the one that drops T below the threshold is
+    checked; those with power in range and alpha >= 1/255 are blended. Every
+    chunk task of a chunked tile runs from T = 1 and is counted in full.
+    Vectorised over a tile's pixels in binary32 (numpy's exp and log)."""
+    f = np.float32
+    off = ref.get("tile_offsets").astype(np.int64)
+    ids = ref.get("sorted_records")[:, 1]
+    pm, pc = ref.get("projected_mean"), ref.get("projected_conic")
+    tiles_x = -(-width // edge)
+    checked = blended = heavy = tasks = 0
+    for t in range(len(off) - 1):
+        b, n = off[t], off[t + 1] - off[t]
+        if n == 0:
+            continue
+        x0, y0 = (t % tiles_x) * edge, (t // tiles_x) * edge
+        xs, ys = np.meshgrid(np.arange(edge) + x0, np.arange(edge) + y0)
+        inside = (xs < width) & (ys < height)
+        if slip == "all_pixels":
+            inside[:] = True
+        px, py = (xs[inside] + f(0.5)).astype(f), (ys[inside] + f(0.5)).astype(f)
+        chunked = chunk and n > chunk
+        spans = [(c, min(n, c + chunk)) for c in range(0, n, chunk)] if chunked else [(0, n)]
+        tasks += len(spans) if chunked else 0
+        tile_checked = 0
+        t_comp = np.ones(px.shape, f)
+        for lo, hi in spans:
+            g = ids[b + lo: b + hi]
+            m, k = pm[g], pc[g]
+            dx = px[None, :] - m[:, 0:1]
+            dy = py[None, :] - m[:, 1:2]
+            power = (k[:, 0:1] * (dx * dx) + k[:, 1:2] * f(2) * dx * dy
+                     + k[:, 2:3] * (dy * dy)) * f(-0.5)
+            cut = np.log(m[:, 3:4] * f(255))
+            alpha = np.clip(m[:, 3:4] * np.exp(power), f(0), f(0.99)).astype(f)
+            ok = ~((power > 0) | (power < -cut)) & (alpha >= f(1 / 255))
+            trans = np.cumprod(np.where(ok, f(1) - alpha, f(1)).astype(f), axis=0, dtype=f)
+            below = trans < f(thr_t)
+            stop = np.where(below.any(0), below.argmax(0) + 1, hi - lo)
+            if slip == "exclude_stop":
+                stop = np.where(below.any(0), stop - 1, stop)
+            if slip == "stop_after_composite" and chunked:
+                stop = np.where(t_comp < f(thr_t), 0, stop)
+            t_comp = (t_comp * trans[-1]).astype(f)
+            rows = np.arange(hi - lo)[:, None]
+            tile_checked += int(stop.sum())
+            blended += int((ok & (rows < stop[None, :])).sum())
+        checked += tile_checked
+        if chunked or n > 1024:
+            heavy += tile_checked
+    return {"checked": checked, "blended": blended, "checked_heavy": heavy, "chunk_tasks": tasks}
+
+
+@pytest.mark.parametrize("chunk", [256, 0, 8192])
+def test_raster_counters_restated(gs, oracle, chunk):
+    """The oracle's checked / blended / heavy counters equal the restatement
+    on a dense 72 x 40 frame (ragged edge tiles; with 256-record chunks, most
+    tiles are chunked and many pixels saturate inside an early chunk)."""
+    scene, cam = tiny_scene(gs, 7, 4000)
+    w, h = 72, 40
+    frame = gs.frame_for(cam, w, h)
+    objects = gs.object_frames_for(cam)[None]
+    ref = oracle.run_frame(scene, frame, objects, capacity=1 << 22, threads=1, chunk_size=chunk)
+    assert ref.counts.overflow == 0
+    c = ref.counts
+    got = {"checked": c.checked, "blended": c.blended, "checked_heavy": c.checked_heavy,
+           "chunk_tasks": c.chunk_tasks}
+    want = counted_pairs(ref, w, h, chunk)
+    print(chunk, got)
+    assert got == want
+    if chunk == 256:
+        assert c.chunk_tasks > 20 and 0 < c.checked_heavy
+        # The restatement tells the plausible miscounts apart.
+        for slip in ("all_pixels", "exclude_stop", "stop_after_composite"):
+            assert counted_pairs(ref, w, h, chunk, slip=slip)["checked"] != c.checked, slip
