@@ -837,12 +837,14 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 //    key's same-digit peers with kBits + 1 ballots (the extra bit marks keys
 //    past the end); the key's rank in the run is the wave's running count of
 //    its digit plus the popcount of its lower peers. One thread per digit
-//    then turns the four waves' counts into offsets in wave order, which is
+//    then turns the waves' counts into offsets in wave order, which is
 //    input order, publishes the partition's digit count, walks
 //    back over the preceding partitions' published counts (kLookbackWindow
 //    partitions per step) until it meets an inclusive prefix, and publishes
-//    its own inclusive prefix. The keys are scattered straight from
-//    registers. Keys keep their input order within a digit.
+//    its own inclusive prefix. The keys go to their place in the
+//    partition's digit order in LDS and from there to the output, each
+//    digit's run written by consecutive threads. Keys keep their input
+//    order within a digit.
 // A pass whose digit totals put every key in one digit is a stable identity
 // permutation; its blocks only copy their keys (depth keys of one frame
 // often share the top byte). When the caller passes `in_place` for the last
@@ -984,7 +986,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   constexpr uint32_t kPartition = kSortThreads * kItems;
   __shared__ uint32_t wave_cnt[kSortWaves][kRadix];  // running counts, then offsets
   __shared__ uint32_t digit_base[kRadix];
-  __shared__ uint32_t lds4[4];
+  __shared__ uint32_t lds4[4], lds4b[4];
+  // The partition's keys in digit order, and where each digit's keys start
+  // among them.
+  __shared__ uint32_t part_start[kRadix];
+  __shared__ uint2 stage[kPartition];
   const uint32_t t = threadIdx.x;
   const uint32_t count = *count_ptr;
   const uint32_t nparts = (count + kPartition - 1) / kPartition;
@@ -1050,6 +1056,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   }
   // Each digit's start in the output: exclusive scan of the digit totals.
   uint32_t digit_start = digit_exclusive_scan(my_total, lds4);
+  // Each digit's start within the partition: exclusive scan of its counts.
+  uint32_t start = digit_exclusive_scan(local, lds4b);
+  if (t < kRadix) part_start[t] = start;
   if (t < kRadix) {
     uint32_t excl = 0u;
     if (part > 0) {
@@ -1092,10 +1101,25 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
     digit_base[t] = digit_start + excl;
   }
   __syncthreads();
+  // Keys to their place in the partition's digit order in LDS, then out in
+  // that order: consecutive threads write consecutive records of a digit's
+  // run, so a wave's store covers a few lines instead of one line per key
+  // (6M/4K tile sort, two passes over 79 M keys: 0.94 -> 0.84 ms).
 #pragma unroll
   for (int r = 0; r < kItems; r++) {
     uint32_t d = dig[r];
-    if (d < kRadix) out[digit_base[d] + wave_cnt[wave][d] + rank[r]] = rec[r];
+    if (d < kRadix) stage[part_start[d] + wave_cnt[wave][d] + rank[r]] = rec[r];
+  }
+  __syncthreads();
+  const uint32_t valid = min((uint32_t)kPartition, count - part * (uint32_t)kPartition);
+#pragma unroll
+  for (int r = 0; r < kItems; r++) {
+    uint32_t i = t + (uint32_t)r * kSortThreads;
+    if (i < valid) {
+      uint2 k = stage[i];
+      uint32_t d = (k.x >> shift) & (kRadix - 1);
+      out[digit_base[d] - part_start[d] + i] = k;
+    }
   }
 }
 
@@ -1556,27 +1580,34 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
           bits &= bits - 1u;
         }
         if (!v[0]) break;
+        // Slots past the pixel's last candidate evaluate record j = w * 32
+        // of the round, a staged slot; their results are never taken.
         Candidate cd[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          if (v[q]) cd[q] = eval_candidate(L, j[q], pxc, pyc, vw, vh, axis);
+        for (int q = 0; q < 4; q++) cd[q] = eval_candidate(L, j[q], pxc, pyc, vw, vh, axis);
+        // The blends in order, as selects rather than branches (the same
+        // operations on the same values; a skipped candidate keeps every
+        // value as it was).
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          if (!v[q] || done) continue;
-          float alpha = cd[q].alpha;
-          if (alpha < (float)(1.0 / 255.0)) continue;
-          float contribution = s.t * alpha;
-          if (kRecord) atomicMax(&record[L.gid[j[q]]], gs_f2b(contribution));
-          s.r = s.r + cd[q].color.x * contribution;
-          s.g = s.g + cd[q].color.y * contribution;
-          s.b = s.b + cd[q].color.z * contribution;
-          s.d = s.d + cd[q].depth * contribution;
-          s.blended++;
-          s.t = s.t * (1.0f - alpha);
-          if (s.t < threshold) {
-            done = true;
-            stop = j[q] + 1u;
-          }
+          const float alpha = cd[q].alpha;
+          const bool take = v[q] && !done && !(alpha < (float)(1.0 / 255.0));
+          const float contribution = s.t * alpha;
+          if (kRecord && take) atomicMax(&record[L.gid[j[q]]], gs_f2b(contribution));
+          const float r = s.r + cd[q].color.x * contribution;
+          const float g = s.g + cd[q].color.y * contribution;
+          const float b = s.b + cd[q].color.z * contribution;
+          const float d = s.d + cd[q].depth * contribution;
+          const float nt = s.t * (1.0f - alpha);
+          s.r = take ? r : s.r;
+          s.g = take ? g : s.g;
+          s.b = take ? b : s.b;
+          s.d = take ? d : s.d;
+          s.blended += take ? 1u : 0u;
+          s.t = take ? nt : s.t;
+          const bool fin = take && nt < threshold;
+          stop = fin ? j[q] + 1u : stop;
+          done = done || fin;
         }
         if (done) break;
       }
@@ -1751,21 +1782,27 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
         }
         float qa[kLanes];
         group_bcast_all<kLanes>(a, qa);
+        // The blends in order, as selects rather than branches (the same
+        // operations on the same values; a skipped candidate keeps every
+        // value as it was).
 #pragma unroll
         for (int qq = 0; qq < kLanes; qq++) {
-          if (!v[qq] || done) continue;
-          float alpha = qa[qq];
-          if (alpha < (float)(1.0 / 255.0)) continue;
-          float contribution = s.t * alpha;
-          if (kRecord && q == (uint32_t)qq) atomicMax(&record[L.gid[j[qq]]], gs_f2b(contribution));
+          const float alpha = qa[qq];
+          const bool take = v[qq] && !done && !(alpha < (float)(1.0 / 255.0));
+          const float contribution = s.t * alpha;
+          if (kRecord && take && q == (uint32_t)qq)
+            atomicMax(&record[L.gid[j[qq]]], gs_f2b(contribution));
 #pragma unroll
-          for (int h = 0; h < kChannels; h++) acc[h] = acc[h] + ch[qq][h] * contribution;
-          s.blended++;
-          s.t = s.t * (1.0f - alpha);
-          if (s.t < threshold) {
-            done = true;
-            stop = j[qq] + 1u;
+          for (int h = 0; h < kChannels; h++) {
+            const float sum = acc[h] + ch[qq][h] * contribution;
+            acc[h] = take ? sum : acc[h];
           }
+          const float nt = s.t * (1.0f - alpha);
+          s.blended += take ? 1u : 0u;
+          s.t = take ? nt : s.t;
+          const bool fin = take && nt < threshold;
+          stop = fin ? j[qq] + 1u : stop;
+          done = done || fin;
         }
         if (done) break;
       }
