@@ -1576,38 +1576,60 @@ GS_DEV PixelSum blend_tile_range(const FrameParams& fp, uint32_t tile, uint32_t 
             bits = L.mask[w][t];
           }
           v[q] = bits != 0u;
-          j[q] = w * 32u + (v[q] ? (uint32_t)__builtin_ctz(bits) : 0u);
+          j[q] = v[q] ? w * 32u + (uint32_t)__builtin_ctz(bits) : (q == 0 ? 0u : j[0]);
           bits &= bits - 1u;
         }
         if (!v[0]) break;
-        // Slots past the pixel's last candidate evaluate record j = w * 32
-        // of the round, a staged slot; their results are never taken.
+        // Slots past the pixel's last candidate evaluate the step's first
+        // candidate again; their results are never taken.
         Candidate cd[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) cd[q] = eval_candidate(L, j[q], pxc, pyc, vw, vh, axis);
-        // The blends in order, as selects rather than branches (the same
-        // operations on the same values; a skipped candidate keeps every
-        // value as it was).
+        // The blends in order. First assuming none of them ends the pixel:
+        // a skipped candidate enters with alpha 0, which leaves every value
+        // as it was (its colour and depth are finite). T only falls, so if
+        // it ends at or above the threshold no blend of the step crossed
+        // it; otherwise the step is applied again from the values before
+        // it, as selects that stop at the crossing blend (the same
+        // operations on the same values).
+        const PixelSum in = s;
+        if (!kRecord) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const float alpha = cd[q].alpha;
-          const bool take = v[q] && !done && !(alpha < (float)(1.0 / 255.0));
-          const float contribution = s.t * alpha;
-          if (kRecord && take) atomicMax(&record[L.gid[j[q]]], gs_f2b(contribution));
-          const float r = s.r + cd[q].color.x * contribution;
-          const float g = s.g + cd[q].color.y * contribution;
-          const float b = s.b + cd[q].color.z * contribution;
-          const float d = s.d + cd[q].depth * contribution;
-          const float nt = s.t * (1.0f - alpha);
-          s.r = take ? r : s.r;
-          s.g = take ? g : s.g;
-          s.b = take ? b : s.b;
-          s.d = take ? d : s.d;
-          s.blended += take ? 1u : 0u;
-          s.t = take ? nt : s.t;
-          const bool fin = take && nt < threshold;
-          stop = fin ? j[q] + 1u : stop;
-          done = done || fin;
+          for (int q = 0; q < 4; q++) {
+            const bool take = v[q] && !(cd[q].alpha < (float)(1.0 / 255.0));
+            const float alpha = take ? cd[q].alpha : 0.0f;
+            const float contribution = s.t * alpha;
+            s.r = s.r + cd[q].color.x * contribution;
+            s.g = s.g + cd[q].color.y * contribution;
+            s.b = s.b + cd[q].color.z * contribution;
+            s.d = s.d + cd[q].depth * contribution;
+            s.t = s.t * (1.0f - alpha);
+            s.blended += take ? 1u : 0u;
+          }
+        }
+        if (kRecord || s.t < threshold) {
+          s = in;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const float alpha = cd[q].alpha;
+            const bool take = v[q] && !done && !(alpha < (float)(1.0 / 255.0));
+            const float contribution = s.t * alpha;
+            if (kRecord && take) atomicMax(&record[L.gid[j[q]]], gs_f2b(contribution));
+            const float r = s.r + cd[q].color.x * contribution;
+            const float g = s.g + cd[q].color.y * contribution;
+            const float b = s.b + cd[q].color.z * contribution;
+            const float d = s.d + cd[q].depth * contribution;
+            const float nt = s.t * (1.0f - alpha);
+            s.r = take ? r : s.r;
+            s.g = take ? g : s.g;
+            s.b = take ? b : s.b;
+            s.d = take ? d : s.d;
+            s.blended += take ? 1u : 0u;
+            s.t = take ? nt : s.t;
+            const bool fin = take && nt < threshold;
+            stop = fin ? j[q] + 1u : stop;
+            done = done || fin;
+          }
         }
         if (done) break;
       }
@@ -1754,7 +1776,7 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
             bits = L.mask[w][pix];
           }
           v[qq] = bits != 0u;
-          j[qq] = w * 32u + (v[qq] ? (uint32_t)__builtin_ctz(bits) : 0u);
+          j[qq] = v[qq] ? w * 32u + (uint32_t)__builtin_ctz(bits) : (qq == 0 ? 0u : j[0]);
           bits &= bits - 1u;
         }
         if (!v[0]) break;
@@ -1782,27 +1804,54 @@ GS_DEV PixelSum blend_heavy_group(const FrameParams& fp, uint32_t tile, uint32_t
         }
         float qa[kLanes];
         group_bcast_all<kLanes>(a, qa);
-        // The blends in order, as selects rather than branches (the same
-        // operations on the same values; a skipped candidate keeps every
-        // value as it was).
+        // The blends in order. First assuming none of them ends the pixel:
+        // a skipped candidate enters with alpha 0, which leaves every value
+        // as it was (its colour is finite: absent candidates read the
+        // step's first one). T only falls, so if it ends at or above the
+        // threshold no blend of the step crossed it; otherwise the step is
+        // applied again from the values before it, as selects that stop at
+        // the crossing blend (the same operations on the same values).
+        const float t_in = s.t;
+        const uint32_t blended_in = s.blended;
+        float acc_in[kChannels];
 #pragma unroll
-        for (int qq = 0; qq < kLanes; qq++) {
-          const float alpha = qa[qq];
-          const bool take = v[qq] && !done && !(alpha < (float)(1.0 / 255.0));
-          const float contribution = s.t * alpha;
-          if (kRecord && take && q == (uint32_t)qq)
-            atomicMax(&record[L.gid[j[qq]]], gs_f2b(contribution));
+        for (int h = 0; h < kChannels; h++) acc_in[h] = acc[h];
+        if (!kRecord) {
 #pragma unroll
-          for (int h = 0; h < kChannels; h++) {
-            const float sum = acc[h] + ch[qq][h] * contribution;
-            acc[h] = take ? sum : acc[h];
+          for (int qq = 0; qq < kLanes; qq++) {
+            const bool take = v[qq] && !(qa[qq] < (float)(1.0 / 255.0));
+            const float alpha = take ? qa[qq] : 0.0f;
+            const float contribution = s.t * alpha;
+#pragma unroll
+            for (int h = 0; h < kChannels; h++) acc[h] = acc[h] + ch[qq][h] * contribution;
+            s.t = s.t * (1.0f - alpha);
+            s.blended += take ? 1u : 0u;
           }
-          const float nt = s.t * (1.0f - alpha);
-          s.blended += take ? 1u : 0u;
-          s.t = take ? nt : s.t;
-          const bool fin = take && nt < threshold;
-          stop = fin ? j[qq] + 1u : stop;
-          done = done || fin;
+        }
+        if (kRecord || s.t < threshold) {
+          s.t = t_in;
+          s.blended = blended_in;
+#pragma unroll
+          for (int h = 0; h < kChannels; h++) acc[h] = acc_in[h];
+#pragma unroll
+          for (int qq = 0; qq < kLanes; qq++) {
+            const float alpha = qa[qq];
+            const bool take = v[qq] && !done && !(alpha < (float)(1.0 / 255.0));
+            const float contribution = s.t * alpha;
+            if (kRecord && take && q == (uint32_t)qq)
+              atomicMax(&record[L.gid[j[qq]]], gs_f2b(contribution));
+#pragma unroll
+            for (int h = 0; h < kChannels; h++) {
+              const float sum = acc[h] + ch[qq][h] * contribution;
+              acc[h] = take ? sum : acc[h];
+            }
+            const float nt = s.t * (1.0f - alpha);
+            s.blended += take ? 1u : 0u;
+            s.t = take ? nt : s.t;
+            const bool fin = take && nt < threshold;
+            stop = fin ? j[qq] + 1u : stop;
+            done = done || fin;
+          }
         }
         if (done) break;
       }
