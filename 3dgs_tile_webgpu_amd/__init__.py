@@ -12,9 +12,13 @@ Public API (mirrors the reference's pass API for the frame path):
   PackedScene, workload_scene    -- packed buffers and synthetic workloads
   read_ply, pack_scene           -- canonical 3DGS PLY files into the store's
                                     slot buffers (holes beyond the cloud)
+  ActiveSlots, check_data        -- GaussianData's active slot list (staged
+                                    removals/additions, committed ranges) and
+                                    its buffer checks
   PerspectiveCamera              -- Three.js camera conventions
 """
 from ._native import GsError, GsRangeError, GsTypeError, lib  # noqa: F401
+from .active_slots import ActiveSlots, check_data  # noqa: F401
 from .batch import BatchedPass  # noqa: F401
 from .camera import PerspectiveCamera, elements, orbit_cameras  # noqa: F401
 from .gaussian_pass import (  # noqa: F401

@@ -20,6 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as N
+from .active_slots import ActiveSlots, check_data
 from .camera import PerspectiveCamera, elements
 from .scene import PackedScene
 
@@ -207,7 +208,9 @@ def config_for(o: dict, policy: IntersectionCapacityPolicy) -> N.GsConfig:
 
 
 def upload_scene(ctx, data: PackedScene) -> list:
-    """gs_set_data from a PackedScene; returns the arrays the call read."""
+    """gs_set_data from a PackedScene (after GaussianData's checks); returns
+    the arrays the call read."""
+    check_data(data)
     keep = [np.ascontiguousarray(data.means, np.float32),
             np.ascontiguousarray(data.scales_opacity, np.uint32),
             np.ascontiguousarray(data.rotations, np.uint32),
@@ -292,11 +295,13 @@ class GaussianPass:
         self._dirty = True
         self._snapshot = None
         self.allocation_error: str | None = None  # set when growth failed (allocation-failed)
+        self._tracked: list = []
         self.update_data(data)
 
     # -- data --------------------------------------------------------------
     def update_data(self, data: PackedScene) -> None:
-        """TiledGaussianPipeline.updateData (TiledGaussianPipeline.ts:203-218)."""
+        """TiledGaussianPipeline.updateData (TiledGaussianPipeline.ts:203-218);
+        the buffers are checked as GaussianData's constructor checks them."""
         self._keep = upload_scene(self._ctx, data)
         self.data = data
         self._dirty = True
@@ -306,6 +311,21 @@ class GaussianPass:
         N.check(N.lib().gs_update_active(self._ctx, start, s.size, s.ctypes.data if s.size else None,
                                          active_count), self._ctx)
         self._dirty = True
+
+    def track_active(self, active: ActiveSlots) -> None:
+        """Upload every committed change of ``active`` as it is committed
+        (GaussianData.subscribeActiveSlots and ProjectionStage.updateActiveSlots,
+        ProjectionStage.ts:298-329): the changed ordinal range and the new
+        active count."""
+        if active.capacity != (self.data.count if self.data is not None else active.capacity):
+            raise N.GsRangeError(N.GS_E_RANGE, "Active GPU slot exceeds capacity")
+
+        def on_commit(rng: dict) -> None:
+            start, n = rng["start"], rng["count"]
+            self.update_active(start, active.slots[start:start + n], active.count)
+
+        active.subscribe(on_commit)
+        self._tracked.append((active, on_commit))
 
     def invalidate(self) -> None:
         """Force the next render to encode a frame (GaussianPass.ts:604-606)."""
@@ -448,6 +468,9 @@ class GaussianPass:
         return N.lib().gs_stream(self._ctx) or 0
 
     def dispose(self) -> None:
+        for active, listener in getattr(self, "_tracked", []):
+            active.unsubscribe(listener)
+        self._tracked = []
         if getattr(self, "_ctx", None):
             N.lib().gs_destroy(self._ctx)
             self._ctx = None

@@ -53,6 +53,8 @@ export interface GaussianDataBuffers {
   count: number;
   activeCount?: number;
   shDegree?: 0 | 1 | 2 | 3;
+  /** Only rgb8e8 words are rendered by this path (float32 SH is a TypeError). */
+  shFormat?: 'rgb8e8' | 'float32';
   means: Float32Array;          // count * 4
   scalesOpacity: Uint32Array;   // count * 2
   rotations: Uint32Array;       // count
@@ -132,6 +134,8 @@ export class GaussianPass {
   updateData(data: GaussianDataBuffers): void;
   /** Active slots [start, start + slots.length) and the new active count. */
   updateActive(start: number, slots: Uint32Array, activeCount: number): void;
+  /** Upload each committed change of the list (its capacity is data.count). */
+  trackActiveSlots(list: ActiveSlots): void;
   /** Force the next render to encode a frame. */
   invalidate(): void;
   /** The error of a failed capacity growth; no frame is encoded after it. */
@@ -195,6 +199,25 @@ export class LearnedCull {
   calibrate(cameras: CameraLike[], sizes: [number, number][], targetPsnr?: number,
             shifts?: number[]): { shift: number | null; rows: CalibrationRow[] };
 }
+
+export interface ActiveSlotRange { start: number; count: number }
+
+/** GaussianData's active slot list (GaussianData.ts:170-232). */
+export class ActiveSlots {
+  constructor(capacity: number, slots?: Uint32Array);
+  readonly capacity: number;
+  readonly count: number;
+  readonly version: number;
+  /** The committed list, a view of its first `count` entries. */
+  readonly slots: Uint32Array;
+  subscribe(listener: (range: ActiveSlotRange) => void): void;
+  unsubscribe(listener: (range: ActiveSlotRange) => void): void;
+  /** Removals first, then additions; with commit, publish the changed range. */
+  stageActivation(added: Uint32Array, removed: Uint32Array, commit: boolean): ActiveSlotRange | null;
+}
+
+/** GaussianData's buffer checks (RangeError / TypeError as the reference). */
+export function checkData(data: GaussianDataBuffers): void;
 
 /** PSNR of two rgba16float images (raw half bits) over rgb. */
 export function psnr(a: Uint16Array, b: Uint16Array): number;

@@ -197,6 +197,132 @@ function checkObjectTables(tables) {
 
 const sameArray = (a, b) => a !== null && b !== null && a.length === b.length && a.every((v, i) => v === b[i]);
 
+// The store's active slot list (GaussianData.ts:86-97, 170-232): a compact
+// list of active slots with each slot's ordinal in it. Staged removals move
+// the list's last slot into the removed slot's place, staged additions
+// append; a commit swaps the front and back lists, bumps the version and
+// hands every subscriber the changed ordinal range, which
+// GaussianPass.trackActiveSlots uploads (ProjectionStage.updateActiveSlots,
+// ProjectionStage.ts:298-329).
+class ActiveSlots {
+  constructor(capacity, slots) {
+    if (!Number.isInteger(capacity) || capacity <= 0) {
+      throw new RangeError('GaussianData count must be a positive integer');
+    }
+    this.capacity = capacity;
+    this._values = new Uint32Array(capacity);
+    this._ordinals = new Int32Array(capacity).fill(-1);
+    this._stagedValues = new Uint32Array(capacity);
+    this._stagedOrdinals = new Int32Array(capacity);
+    this.count = 0;
+    this.version = 0;
+    this._stagedCount = 0;
+    this._staging = false;
+    this._first = Infinity;
+    this._last = -1;
+    this._listeners = new Set();
+    if (slots) this.stageActivation(slots, new Uint32Array(0), true);
+  }
+
+  get slots() { return this._values.subarray(0, this.count); }
+
+  subscribe(listener) { this._listeners.add(listener); }
+  unsubscribe(listener) { this._listeners.delete(listener); }
+
+  stageActivation(added, removed, commit) {
+    if (!this._staging && added.length === 0 && removed.length === 0) return null;
+    for (const slot of added) {
+      if (slot >= this.capacity) throw new RangeError('Active GPU slot exceeds capacity');
+    }
+    for (const slot of removed) {
+      if (slot >= this.capacity) throw new RangeError('Removed GPU slot exceeds capacity');
+    }
+    if (!this._staging) {
+      this._stagedValues.set(this._values.subarray(0, this.count));
+      this._stagedOrdinals.set(this._ordinals);
+      this._stagedCount = this.count;
+      this._first = Infinity;
+      this._last = -1;
+      this._staging = true;
+    }
+    // Removals come before additions across delta batches.
+    for (const slot of removed) {
+      const ordinal = this._stagedOrdinals[slot];
+      if (ordinal < 0) continue;
+      const last = this._stagedValues[--this._stagedCount];
+      this._stagedValues[ordinal] = last;
+      this._stagedOrdinals[last] = ordinal;
+      this._stagedOrdinals[slot] = -1;
+      this._first = Math.min(this._first, ordinal);
+      this._last = Math.max(this._last, ordinal);
+    }
+    for (const slot of added) {
+      if (this._stagedOrdinals[slot] >= 0) continue;
+      const ordinal = this._stagedCount++;
+      this._stagedValues[ordinal] = slot;
+      this._stagedOrdinals[slot] = ordinal;
+      this._first = Math.min(this._first, ordinal);
+      this._last = Math.max(this._last, ordinal);
+    }
+    if (!commit) return null;
+    const values = this._values;
+    this._values = this._stagedValues;
+    this._stagedValues = values;
+    const ordinals = this._ordinals;
+    this._ordinals = this._stagedOrdinals;
+    this._stagedOrdinals = ordinals;
+    this.count = this._stagedCount;
+    this._staging = false;
+    this.version++;
+    const start = Math.min(this._first, this.count);
+    const end = Math.min(this._last + 1, this.count);
+    const range = { start, count: Math.max(0, end - start) };
+    for (const listener of this._listeners) listener(range);
+    return range;
+  }
+}
+
+// GaussianData's buffer checks (GaussianData.ts:76-166, 265-318) for the
+// compact + rgb8e8 contract this path renders, with its error classes and
+// messages.
+function checkData(data) {
+  const count = data.count;
+  if (!Number.isInteger(count) || count <= 0) {
+    throw new RangeError('GaussianData count must be a positive integer');
+  }
+  const shDegree = dflt(data.shDegree, 3);
+  if (!Number.isInteger(shDegree) || shDegree < 0 || shDegree > 3) {
+    throw new RangeError('GaussianData shDegree must be 0, 1, 2, or 3');
+  }
+  const shFormat = dflt(data.shFormat, 'rgb8e8');
+  if (shFormat !== 'float32' && shFormat !== 'rgb8e8') {
+    throw new RangeError('GaussianData shFormat must be float32 or rgb8e8');
+  }
+  if (!(data.means instanceof Float32Array)) {
+    throw new TypeError('GaussianData means must use Float32Array storage');
+  }
+  if (data.means.length < count * 4) {
+    throw new RangeError(`GaussianData means has ${Math.floor(data.means.length / 4)} items; at least ${count} are required`);
+  }
+  if (!(data.scalesOpacity instanceof Uint32Array) || data.scalesOpacity.length < count * 2 ||
+      !(data.rotations instanceof Uint32Array) || data.rotations.length < count) {
+    throw new TypeError('Invalid compact Gaussian shape attribute');
+  }
+  if (!(data.shCoefficients instanceof Uint32Array) || shFormat !== 'rgb8e8') {
+    throw new TypeError(`GaussianData ${shFormat} shCoefficients use the wrong typed array`);
+  }
+  const need = count * (shDegree + 1) * (shDegree + 1);
+  if (data.shCoefficients.length < need) {
+    throw new RangeError(`GaussianData shCoefficients has ${data.shCoefficients.length} items; at least ${need} are required`);
+  }
+  if (data.activeSlots) {
+    const n = Math.min(dflt(data.activeCount, data.activeSlots.length), data.activeSlots.length);
+    for (let i = 0; i < n; i++) {
+      if (data.activeSlots[i] >= count) throw new RangeError('Active GPU slot exceeds capacity');
+    }
+  }
+}
+
 class GaussianPass {
   constructor(camera, data, options = {}) {
     this.options = validateOptions(options);
@@ -232,9 +358,22 @@ class GaussianPass {
 
   // TiledGaussianPipeline.updateData (TiledGaussianPipeline.ts:203-218).
   updateData(data) {
+    checkData(data);
     native.setData(this._ctx, data);
     this.data = data;
     this._dirty = true;
+  }
+
+  // Upload every committed change of an ActiveSlots list as it is committed
+  // (GaussianData.subscribeActiveSlots, ProjectionStage.ts:298-329).
+  trackActiveSlots(list) {
+    if (list.capacity !== this.data.count) throw new RangeError('Active GPU slot exceeds capacity');
+    const listener = (range) => {
+      this.updateActive(range.start, list.slots.subarray(range.start, range.start + range.count), list.count);
+    };
+    list.subscribe(listener);
+    if (!this._tracked) this._tracked = [];
+    this._tracked.push([list, listener]);
   }
 
   // Active-slot update of [start, start + slots.length)
@@ -380,6 +519,8 @@ class GaussianPass {
   }
 
   dispose() {
+    for (const [list, listener] of this._tracked || []) list.unsubscribe(listener);
+    this._tracked = [];
     if (this._ctx) native.destroy(this._ctx);
     this._ctx = null;
   }
@@ -397,6 +538,7 @@ class GaussianBatch {
     this.capacities = [];
     this.sizes = [];
     this.attempt = 0;
+    checkData(data);
     native.setData(this._ctx, data);
   }
 
@@ -615,6 +757,6 @@ function gaussianPass(camera, data, options = {}) {
 }
 
 module.exports = {
-  gaussianPass, GaussianPass, GaussianBatch, IntersectionCapacityPolicy, LearnedCull, validateOptions,
-  objectFrames, checkObjectTables, psnr, DEFAULTS, native,
+  gaussianPass, GaussianPass, GaussianBatch, IntersectionCapacityPolicy, LearnedCull, ActiveSlots,
+  validateOptions, checkData, objectFrames, checkObjectTables, psnr, DEFAULTS, native,
 };
