@@ -10,6 +10,9 @@ Node facade's twins are in test_node_facade.py).
   publishes; a fixed capacity reports budget-exceeded and never grows; a
   failed growth reports allocation-failed and stops encoding; independent
   passes grow from their own demand.
+- Viewport resizes (test/ResizeResources.test.ts:17-42, 74-92, 129+): buffers
+  are grow-only, a shrunk frame reads back only its own tile ranges plus the
+  sentinel, and frames after shrinking and regrowing stay bit-exact.
 """
 from __future__ import annotations
 
@@ -29,6 +32,8 @@ def test_redraw_always_is_the_default(gs, gpu_available, scene):
     p.render(320, 180)
     p.render(320, 180)
     assert p.redraw_strategy == "always" and p.render_count == 2 and p.cache_hit_count == 0
+    # rasterWork is only reported with rasterStats (test/RasterWorkStats.test.ts:27-33)
+    assert p.read_stats()["rasterWork"] is None
     p.dispose()
 
 
@@ -125,3 +130,39 @@ def test_independent_passes_grow_from_their_own_demand(gs, gpu_available, scene)
     assert a.capacity != b.capacity
     a.dispose()
     b.dispose()
+
+
+def test_resizes_reuse_buffers_and_read_back_active_tiles(gs, oracle, gpu_available, scene):
+    """test/ResizeResources.test.ts:17-42, 74-92, 129+: a pass renders through
+    widths 272..329, a 1x1 frame and back to 272 without an error; after each
+    shrink the tile ranges it reports cover only the current grid plus the
+    sentinel, and the regrown frame is bit-exact against the oracle (buffers
+    are grow-only, so nothing stale from the larger frames reaches it)."""
+    from test_gpu_pipeline import oracle_for
+
+    cam = gs.WORKLOADS["small"].camera()
+    p = gs.gaussian_pass(cam, scene, {"rasterChunkSize": 512, "outputDepth": True})
+    objects = gs.object_frames_for(cam)[None]
+
+    def check(w, h):
+        p.render(w, h)
+        st = p.read_state()
+        assert st.overflow == 0 and st.error_flags == 0
+        tiles = -(-w // 16) * -(-h // 16)
+        assert st.tiles_x * st.tiles_y == tiles
+        offsets = p.read_debug("tile_offsets").view(np.uint32)
+        assert offsets.size == tiles + 1 and offsets[-1] == st.intersections
+        ref = oracle_for(gs, oracle, p, scene, cam, w, h, objects)
+        np.testing.assert_array_equal(p.read_image().view(np.uint16), ref.get("image_f16"))
+        np.testing.assert_array_equal(p.read_depth().view(np.uint32),
+                                      ref.get("depth_target").view(np.uint32))
+
+    check(272, 256)
+    for width in range(273, 330, 7):
+        p.render(width, 256)
+    check(329, 256)
+    check(1, 1)
+    check(272, 256)
+    check(1024, 512)  # growth past the first tile grid
+    check(16, 16)
+    p.dispose()
