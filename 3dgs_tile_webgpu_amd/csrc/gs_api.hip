@@ -232,17 +232,21 @@ int lane_sort(gs_ctx* ctx, Lane& l) {
   if (parts == 0) parts = 1;
   uint32_t scan_parts = div_up(m, gs::kScanPartition);
   int rc;
-  // Sort workspace: digit totals of up to 8 passes, then two sets of
-  // partition status rows (passes alternate between them).
-  size_t sort_words = 8 * 256 + (size_t)2 * parts * 256;
+  // Sort workspace: the digit totals of the depth and tile sorts, then two
+  // sets of partition status rows (passes alternate between them). The
+  // totals start at zero; afterwards each sort's counting kernel zeroes the
+  // other sort's totals.
+  size_t sort_words = gs::kSortTotalsWords + (size_t)2 * parts * 256;
   // Scan workspace: a 64-bit status word per partition. It starts at zero;
-  // the radix sort's clearing kernel zeroes it again after each scan.
+  // the radix sort's counting kernel zeroes it again after each scan.
   size_t scan_bytes = (size_t)8 * (scan_parts + 1);
   if ((rc = ensure(ctx, l.hist, sort_words * 4, "radix-sort-workspace")) ||
       (rc = ensure(ctx, l.partials, scan_bytes, "scan-workspace")))
     return rc;
   // On the context stream: every lane's work forks from it after this.
   hipError_t e = hipMemsetAsync(l.partials.ptr, 0, l.partials.bytes, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemsetAsync(l.hist.ptr, 0, gs::kSortTotalsWords * 4, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync");
   l.sort_capacity = m;
   l.sort_parts = parts;
@@ -1507,7 +1511,7 @@ NoAllocLayout noalloc_layout(const gs_config& c, uint32_t n, uint32_t capacity, 
   L.intersection_offsets = take((size_t)n * 4);
   L.records_a = take((size_t)capacity * 8);
   L.records_b = take((size_t)capacity * 8);
-  L.hist = take(((size_t)8 * 256 + (size_t)2 * L.sort_parts * 256) * 4);
+  L.hist = take((gs::kSortTotalsWords + (size_t)2 * L.sort_parts * 256) * 4);
   L.partials = take((size_t)8 * (L.scan_parts + 1));
   L.tile_offsets = take(((size_t)tiles + 1) * 4);
   L.chunk_first = take(((size_t)tiles + 1) * 4);
@@ -1520,15 +1524,20 @@ NoAllocLayout noalloc_layout(const gs_config& c, uint32_t n, uint32_t capacity, 
 
 // The frame's camera uniforms (passed by value, so a captured launch keeps
 // them) into the scratch, and the parts of the scratch that must start at
-// zero: the frame state and the scan look-back status words.
+// zero: the frame state, the depth sort's digit totals (the depth sort's
+// counting kernel zeroes the tile sort's) and the scan look-back status
+// words.
 __global__ __launch_bounds__(256) void k_noalloc_begin(FrameCamera cam, FrameCamera* cam_out,
                                                         uint32_t* state_words,
                                                         uint32_t state_count,
+                                                        uint32_t* depth_totals,
                                                         unsigned long long* scan_status,
                                                         uint32_t scan_count) {
   uint32_t t = threadIdx.x;
   if (t == 0) *cam_out = cam;
   for (uint32_t i = t; i < state_count; i += blockDim.x) state_words[i] = 0u;
+  for (uint32_t i = t; i < (uint32_t)(gs::kSortTotalsWords / 2); i += blockDim.x)
+    depth_totals[i] = 0u;
   for (uint32_t i = t; i < scan_count; i += blockDim.x) scan_status[i] = 0ull;
 }
 
@@ -1621,7 +1630,7 @@ extern "C" int gs_render_noalloc(const gs_config* config, const gs_data_desc* da
   cam.reset_valid = frame->reset_valid;
   hipStream_t s = (hipStream_t)stream;
   k_noalloc_begin<<<1, 256, 0, s>>>(cam, camera, (uint32_t*)b.state,
-                                    (uint32_t)(sizeof(DeviceState) / 4),
+                                    (uint32_t)(sizeof(DeviceState) / 4), b.hist,
                                     (unsigned long long*)b.partials, L.scan_parts + 1);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = gs::launch_frame(b, s, gs::RasterStreams{nullptr, nullptr, nullptr},

@@ -94,6 +94,20 @@ typedef float gs_f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ gs_f32x4 mfma16(gs_bf16x4 a, gs_bf16x4 b, gs_f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+typedef __bf16 gs_bf16x8 __attribute__((ext_vector_type(8)));
+typedef short gs_i16x8 __attribute__((ext_vector_type(8)));
+// v_mfma_f32_16x16x32_bf16 (gfx950) on two 16x16x16 operand pairs: K slots
+// 0..3 of lane group g are the first pair's, 4..7 the second's. A and B
+// put the same K index in the same slot of a lane group, so any assignment
+// of K to slots that A and B share gives the same contraction:
+// D = C + a0 b0 + a1 b1 over the 32 K indices of both pairs.
+__device__ __forceinline__ gs_f32x4 mfma32(gs_bf16x4 a0, gs_bf16x4 a1, gs_bf16x4 b0,
+                                           gs_bf16x4 b1, gs_f32x4 c) {
+  const gs_i16x8 a = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const gs_i16x8 b = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(gs_bf16x8, a),
+                                                 __builtin_bit_cast(gs_bf16x8, b), c, 0, 0, 0);
+}
 
 // Wave-scope ordering of LDS writes before reads by other lanes of the
 // same wave.
@@ -272,7 +286,8 @@ __device__ __forceinline__ gs_bf16x4 bf16x4_relu(gs_f32x4 z) {
 // sample's prediction; xs is the wave's 64 x 16 bf16 LDS rows for the
 // standardised inputs. Same bf16 operands as cull_forward_wave, fp32
 // accumulation (the bias is added inside the accumulation rather than
-// after it).
+// after it); layer 2 takes two K = 32 products (v_mfma_f32_16x16x32_bf16)
+// per output tile instead of four K = 16 ones.
 __device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
                                         const float f[kCullIn]) {
   const int l = __lane_id();
@@ -312,8 +327,8 @@ __device__ inline float cull_infer_wave(const CullLdsNet& N, uint16_t* xs,
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
       gs_f32x4 acc = {b2.x, b2.y, b2.z, b2.w};
-#pragma unroll
-      for (int kt = 0; kt < 4; kt++) acc = mfma16(w[kt], a1t[kt][nt], acc);
+      acc = mfma32(w[0], w[1], a1t[0][nt], a1t[1][nt], acc);
+      acc = mfma32(w[2], w[3], a1t[2][nt], a1t[3][nt], acc);
       yp[nt] = __builtin_fmaf(relu_bits(acc[0]), w3.x, yp[nt]);
       yp[nt] = __builtin_fmaf(relu_bits(acc[1]), w3.y, yp[nt]);
       yp[nt] = __builtin_fmaf(relu_bits(acc[2]), w3.z, yp[nt]);

@@ -21,6 +21,13 @@ constexpr int kSortPartition = kSortThreads * kSortItems;  // keys per sort bloc
 constexpr int kMaxRadixBits = 8;     // widest digit of the LSD sorts
 constexpr int kMaxRadix = 1 << kMaxRadixBits;
 constexpr int kMaxTile = 32;         // widest tile edge (pixels)
+constexpr int kMaxSortPasses = 8;    // 32-bit keys with 4-bit digits
+// Digit totals of the two sorts of a frame (depth, then tile): each sort has
+// kMaxSortPasses rows of 256 totals, each row held as kHistShards partial
+// copies (a block adds into copy blockIdx % kHistShards), so that no total
+// takes more than a few same-address atomics per block count.
+constexpr int kHistShards = 8;
+constexpr size_t kSortTotalsWords = (size_t)2 * kMaxSortPasses * kHistShards * 256;
 
 // The part of a frame's uniforms that changes with the camera
 // (FrameUniforms.ts:41-58): it lives in device memory and is uploaded
@@ -143,8 +150,9 @@ struct FrameBuffers {
   uint2* records_a;
   uint2* records_b;
   uint32_t* tile_offsets;
-  uint32_t* hist;       // sort workspace: 8 x 256 digit totals, then
-                        // 2 x sort_parts x 256 partition status words
+  uint32_t* hist;       // sort workspace: kSortTotalsWords digit totals
+                        // (zero before the first frame), then 2 x
+                        // sort_parts x 256 partition status words
   uint32_t sort_parts;  // partitions the status rows are sized for
   uint32_t* partials;   // scan workspace: one 64-bit status word per scan
                         // partition (zero between scans)

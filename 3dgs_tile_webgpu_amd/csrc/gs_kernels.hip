@@ -658,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
 // always resident and every wait ends. No per-block atomic is taken (on
 // this GPU same-address atomics from many blocks serialize at about 12 ns
 // each). The status words start at zero: the next radix sort's clearing
-// kernel zeroes them after each scan (k_sort_clear). A look-back wait is
+// kernel zeroes them after each scan (k_sort_histograms). A look-back wait is
 // bounded; exceeding it raises bit 1 of DeviceState::sort_error.
 
 constexpr uint32_t kLookbackSpinLimit = 1u << 20;
@@ -829,7 +829,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 // Stable LSD sort of (key, value) pairs with kBits-bit digits (4, 6 or 8),
 // one kernel per digit pass ("onesweep" with decoupled look-back):
 //  - k_sort_histograms counts the digits of every pass in one read of the
-//    keys (global digit totals, hence each digit's start in the output);
+//    keys (global digit totals, hence each digit's start in the output),
+//    and clears what the passes and the next scan need at zero;
 //  - block b of k_sort_pass takes partition b of kSortThreads * kItems keys
 //    (kItems = 8, or 16 for sorts sized for at least kLargeSort keys); wave
 //    w holds the partition's w-th run of 64 * kItems consecutive keys in
@@ -857,16 +858,17 @@ __global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec
 // resident (as for the scans); the wait is bounded anyway, and a block that
 // exceeds the bound raises DeviceState::sort_error instead of waiting on.
 
-constexpr int kMaxPasses = 8;  // 32-bit keys with 4-bit digits
+constexpr int kMaxPasses = kMaxSortPasses;
 // Partitions read per look-back step. Every step issues its loads
 // unconditionally, so a wide window mostly reads status rows it does not
 // need: measured on this GPU, 4 beats 1, 2, 8, 32 and 64 (1M/1080p frame:
 // depth + tile sort 0.133 ms at 4, 0.153 at 32, 0.182 at 64; 79 M tile keys:
 // 1.18 ms at 4, 1.57 at 32).
 constexpr int kLookbackWindow = 4;
-// Blocks of the digit-total kernel: a quarter of the 4096-key partitions,
-// between 256 and 1024 (79 M keys: 0.27 ms at 256 blocks, 0.17 at 1024; at
-// 1 M keys more blocks only add global atomics).
+// Blocks of the digit-total kernel: a quarter of the 4096-key partitions
+// the sort is sized for, between 256 and 1024 (79 M keys: 0.27 ms at 256
+// blocks, 0.17 at 1024). Of these, only as many count as the keys present
+// fill with kHistKeys each; the others only clear.
 constexpr uint32_t kHistBlocksMin = 256, kHistBlocksMax = 1024;
 // Sorts of at least this many keys (the capacity the grid is sized for) use
 // partitions of 16 keys per thread instead of 8: half the partitions, half
@@ -874,7 +876,8 @@ constexpr uint32_t kHistBlocksMin = 256, kHistBlocksMax = 1024;
 // it the larger partitions leave compute units idle (1M frame depth sort
 // 0.077 -> 0.087 ms).
 constexpr uint32_t kLargeSort = 16u << 20;
-constexpr int kHistItems = 8;          // keys per lane in flight (digit totals)
+constexpr int kHistItems = 16;         // keys per lane in flight (digit totals)
+constexpr uint32_t kHistKeys = kBlock * kHistItems;  // keys per counting block and round
 constexpr int kSortWaves = kSortThreads / 64;
 // k_sort_pass gives each of the up to kMaxRadix digits its own thread (digit
 // totals, offsets, look-back, status rows) and its block scan takes four
@@ -899,11 +902,16 @@ constexpr uint32_t kStatusAggregate = 1u << 30;
 constexpr uint32_t kStatusPrefix = 2u << 30;
 constexpr uint32_t kStatusCount = (1u << 30) - 1u;
 
-// Views into a lane's sort workspace: per-pass digit totals [pass][256],
-// two sets of partition status rows [set][parts][256], and the scans'
-// status words, which the clearing kernel also zeroes.
+// Views into a lane's sort workspace. The digit totals of the frame's two
+// sorts sit side by side (kMaxPasses rows of kHistShards x 256 words each):
+// `ghist` is this sort's rows, `ghist_other` the other sort's, which this
+// sort's counting kernel zeroes for their next use (the other sort has
+// finished by then, and its rows start at zero before the first frame).
+// Then come two sets of partition status rows [set][parts][256]; and the
+// scans' status words, which the counting kernel also zeroes.
 struct SortWorkspace {
   uint32_t* ghist;
+  uint32_t* ghist_other;
   uint32_t* status;
   uint32_t parts;
   unsigned long long* scan_status;
@@ -914,17 +922,13 @@ GS_DEV uint32_t* status_row(const SortWorkspace& ws, uint32_t set, uint32_t part
   return ws.status + ((size_t)set * ws.parts + part) * 256u;
 }
 
-// Clears the digit totals, the status rows of the first pass and the scan
-// status words (a kernel rather than a memset, so the frame can be
-// captured).
-__global__ __launch_bounds__(kBlock) void k_sort_clear(SortWorkspace ws,
-                                                       const uint32_t* __restrict__ count_ptr) {
-  uint32_t nparts = (*count_ptr + kSortPartition - 1) / kSortPartition;
-  uint32_t stride = gridDim.x * kBlock;
-  uint32_t first = blockIdx.x * kBlock + threadIdx.x;
-  for (uint32_t i = first; i < (uint32_t)(kMaxPasses * 256); i += stride) ws.ghist[i] = 0u;
-  for (uint32_t i = first; i < nparts * 256u; i += stride) ws.status[i] = 0u;
-  for (uint32_t i = first; i < ws.scan_words; i += stride) ws.scan_status[i] = 0ull;
+// Total of digit t in pass `pass` (the sum of its shard copies).
+GS_DEV uint32_t digit_total(const SortWorkspace& ws, uint32_t pass, uint32_t t) {
+  const uint32_t* row = ws.ghist + (size_t)pass * kHistShards * 256u + t;
+  uint32_t v = 0u;
+#pragma unroll
+  for (int sh = 0; sh < kHistShards; sh++) v += row[sh * 256];
+  return v;
 }
 
 // Same-digit peers of this lane within the wave (kBits + 1 ballots).
@@ -939,18 +943,40 @@ GS_DEV uint64_t digit_peers(uint32_t d) {
   return peers;
 }
 
+// Digit totals of every pass in one read of the keys, into this sort's
+// rows (zero on entry). Every block first clears, striding over the grid:
+// the other sort's rows, the status rows of the first pass (set 0) for the
+// partitions present, and the scans' status words -- their last users (the
+// other sort, the previous sort's passes, the scan just before) have
+// finished. Then the first ceil(count / kHistKeys) blocks (at most the
+// grid) count rounds of kHistKeys keys, all of a lane's loads in flight at
+// once, into LDS, and add their nonzero counts into shard blockIdx %
+// kHistShards of the totals; the other blocks are done. Counting blocks
+// only for the keys present keeps the global atomics per total near
+// count / (kHistKeys kHistShards): a grid sized for a sort's capacity
+// would add every block's counts (at about 12 ns per same-address atomic
+// on this GPU, 1,024 blocks cost 12 us per total).
 template <int kBits>
 __global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restrict__ in,
                                                             const uint32_t* __restrict__ count_ptr,
-                                                            uint32_t passes,
-                                                            uint32_t* __restrict__ ghist) {
+                                                            uint32_t passes, SortWorkspace ws) {
   constexpr uint32_t kRadix = 1u << kBits;
   __shared__ uint32_t h[kMaxPasses * kRadix];
+  const uint32_t count = *count_ptr, lane = lane_id();
+  {
+    const uint32_t stride = gridDim.x * kBlock, first = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t nparts = (count + kSortPartition - 1) / kSortPartition;
+    for (uint32_t i = first; i < (uint32_t)(kMaxPasses * kHistShards * 256); i += stride)
+      ws.ghist_other[i] = 0u;
+    for (uint32_t i = first; i < nparts * 256u; i += stride) ws.status[i] = 0u;
+    for (uint32_t i = first; i < ws.scan_words; i += stride) ws.scan_status[i] = 0ull;
+  }
+  const uint32_t counting = min(gridDim.x, max(1u, (count + kHistKeys - 1) / kHistKeys));
+  if (blockIdx.x >= counting) return;  // block-uniform
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0u;
   __syncthreads();
   // Each wave takes runs of 64 * kHistItems keys, all loads in flight at once.
-  const uint32_t count = *count_ptr, lane = lane_id();
-  const uint32_t stride = gridDim.x * kBlock * kHistItems;
+  const uint32_t stride = counting * kHistKeys;
   for (uint32_t base = (blockIdx.x * kBlock + (threadIdx.x & ~63u)) * kHistItems; base < count;
        base += stride) {
     uint32_t key[kHistItems];
@@ -969,8 +995,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restr
     }
   }
   __syncthreads();
+  uint32_t* shard = ws.ghist + (blockIdx.x % kHistShards) * 256u;
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock)
-    if (h[i]) atomicAdd(&ghist[(i / kRadix) * 256u + i % kRadix], h[i]);
+    if (h[i]) atomicAdd(&shard[(i / kRadix) * kHistShards * 256u + i % kRadix], h[i]);
 }
 
 template <int kBits, int kItems>
@@ -997,7 +1024,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   const uint32_t part = blockIdx.x;
   if (in_place) {
     // The last pass: a digit shared by every key leaves the order as it is.
-    const uint32_t total = t < kRadix ? ws.ghist[pass * 256u + t] : 0u;
+    const uint32_t total = t < kRadix ? digit_total(ws, pass, t) : 0u;
     bool uniform = __syncthreads_or(t < kRadix && total == count);
     if (part == 0 && t == 0) *in_place = uniform ? 1u : 0u;
     if (uniform) return;
@@ -1016,7 +1043,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
     uint32_t i = run + r * 64u + lane;
     rec[r] = i < count ? in[i] : make_uint2(0u, 0u);
   }
-  const uint32_t my_total = t < kRadix ? ws.ghist[pass * 256u + t] : 0u;
+  const uint32_t my_total = t < kRadix ? digit_total(ws, pass, t) : 0u;
   for (uint32_t i = t; i < kSortWaves * kRadix; i += kSortThreads) (&wave_cnt[0][0])[i] = 0u;
   // The row this partition uses in the next pass.
   if (t < 256u) status_row(ws, set ^ 1u, part)[t] = 0u;
@@ -2249,15 +2276,11 @@ void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max
                      uint32_t* in_place, hipStream_t s) {
   uint32_t parts = div_up(max_count, kSortPartition);
   if (parts == 0) parts = 1;
-  uint32_t clear_blocks = div_up((uint64_t)parts * 256u + kMaxPasses * 256u, kBlock);
-  if (clear_blocks > 1024) clear_blocks = 1024;
-  k_sort_clear<<<clear_blocks, kBlock, 0, s>>>(ws, count_ptr);
   uint32_t hist_blocks = parts / 4u;
   hist_blocks = hist_blocks < kHistBlocksMin ? kHistBlocksMin
                                              : (hist_blocks > kHistBlocksMax ? kHistBlocksMax
                                                                              : hist_blocks);
-  if (hist_blocks > parts) hist_blocks = parts;
-  k_sort_histograms<kBits><<<hist_blocks, kBlock, 0, s>>>(a, count_ptr, passes, ws.ghist);
+  k_sort_histograms<kBits><<<hist_blocks, kBlock, 0, s>>>(a, count_ptr, passes, ws);
   if (max_count >= kLargeSort)
     sort_passes<kBits, 2 * kSortItems>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
   else
@@ -2380,7 +2403,10 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStream
 
   // Depth sort over V.
   mark(kStageDepthSort);
-  SortWorkspace ws = {b.hist, b.hist + kMaxPasses * 256, b.sort_parts, sws.status, scan_blocks};
+  uint32_t* depth_totals = b.hist;
+  uint32_t* tile_totals = b.hist + kMaxPasses * kHistShards * 256;
+  SortWorkspace ws = {depth_totals, tile_totals, b.hist + kSortTotalsWords, b.sort_parts,
+                      sws.status, scan_blocks};
   radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.radix_bits, ws, st,
              &st->depth_in_place, s);
   const uint2* depth_out = sorted_depth_records(b, 0u), *depth_in = sorted_depth_records(b, 1u);
@@ -2398,6 +2424,8 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStream
                                      b.intersection_offsets, b.records_a);
 
   mark(kStageTileSort);
+  ws.ghist = tile_totals;
+  ws.ghist_other = depth_totals;
   radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, ws, st,
              nullptr, s);
   const uint2* sorted = sorted_tile_records(b);

@@ -199,22 +199,25 @@ def heuristic_keys_at_psnr(gs, cull, b, cams, sizes, plain, keys_plain, feature,
 
 
 def test_training_culls_on_held_out_1080p_views(gs, gpu_available):
-    """The 3M-Gaussian scene at 1920x1080: train on 96 orbit views, calibrate
+    """The 3M-Gaussian scene at 1920x1080: train on 384 orbit views, calibrate
     the output bias on 16 validation views (mean PSNR >= 46 dB against the
     plain frames), evaluate on 16 held-out views. The trained cull removes
     most keys at a mean PSNR >= 45 dB, runs the batched views at least 1.25x
     faster than the plain path, and removes more keys than a non-learned cull
     (an opacity bound, or a footprint bound) at the same PSNR."""
     scene = gs.workload_scene("3m_cull")
-    views = gs.orbit_cameras(128, seed=99, radius=(12.0, 25.0), resolutions=[(1920, 1080)])
-    train, valid, held = views[:96], views[96:112], views[112:]
+    # The network learns the scene's occlusion from the views' positions and
+    # directions: 96 training views left held-out directions too sparsely
+    # covered (one run calibrated to +1.0 and removed 52 % of the keys).
+    views = gs.orbit_cameras(416, seed=99, radius=(12.0, 25.0), resolutions=[(1920, 1080)])
+    train, valid, held = views[:384], views[384:400], views[400:]
     b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 24})
     cull = gs.LearnedCull(b, seed=3)
-    cull.reserve(30_000_000)
+    cull.reserve(64_000_000)
     for k, (cam, (w, h)) in enumerate(train):
         cull.collect(cam, w, h, rate=0.05, seed=k)
     n = cull.dataset_size()
-    assert n > 10_000_000
+    assert n > 40_000_000
     cull.fit_normalization()
     loss0 = cull.train(1, batch=65536, under_weight=300.0, seed=1)
     cull.train(10000, batch=65536, lr=2e-3, under_weight=300.0, seed=2)
