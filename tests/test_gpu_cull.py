@@ -200,8 +200,8 @@ def heuristic_keys_at_psnr(gs, cull, b, cams, sizes, plain, keys_plain, feature,
 
 def test_training_culls_on_held_out_1080p_views(gs, gpu_available):
     """The 3M-Gaussian scene at 1920x1080: train on 384 orbit views, calibrate
-    the output bias on 16 validation views (mean PSNR >= 46 dB against the
-    plain frames), evaluate on 16 held-out views. The trained cull removes
+    the output bias on 32 validation views (mean PSNR >= 46 dB against the
+    plain frames), evaluate on 32 held-out views. The trained cull removes
     most keys at a mean PSNR >= 45 dB, runs the batched views at least 1.25x
     faster than the plain path, and removes more keys than a non-learned cull
     (an opacity bound, or a footprint bound) at the same PSNR."""
@@ -209,8 +209,10 @@ def test_training_culls_on_held_out_1080p_views(gs, gpu_available):
     # The network learns the scene's occlusion from the views' positions and
     # directions: 96 training views left held-out directions too sparsely
     # covered (one run calibrated to +1.0 and removed 52 % of the keys).
-    views = gs.orbit_cameras(416, seed=99, radius=(12.0, 25.0), resolutions=[(1920, 1080)])
-    train, valid, held = views[:384], views[384:400], views[400:]
+    # With 16 validation and 16 held-out views the two means differed by up
+    # to 1.8 dB; 32 of each halve the spread of that difference.
+    views = gs.orbit_cameras(448, seed=99, radius=(12.0, 25.0), resolutions=[(1920, 1080)])
+    train, valid, held = views[:384], views[384:416], views[416:]
     b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 24})
     cull = gs.LearnedCull(b, seed=3)
     cull.reserve(64_000_000)
