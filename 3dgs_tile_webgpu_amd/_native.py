@@ -54,6 +54,7 @@ class GsConfig(C.Structure):
         ("batch_lanes", u32),
         ("heavy_split_units", u32),
         ("heavy_fine_split_units", u32),
+        ("view_group", u32),
     ]
 
 

@@ -67,6 +67,11 @@ struct ViewSet {
   size_t stride = 0;
   std::vector<uint32_t> capacity;  // intersection capacity per view
   std::vector<uint32_t> lane_of;
+  // View groups (gs_config.view_group): the views in launch order, groups
+  // being consecutive runs of view_group of them; each view's working set
+  // (0: its lane's, j > 0: the context's group slot j - 1).
+  uint32_t group = 1;
+  std::vector<uint32_t> order, slot_of;
   std::vector<DeviceBuffer> color, depth;  // library-owned images per view
   std::vector<FrameBuffers> frames;        // last launch description
   std::vector<uint32_t> width, height;
@@ -74,7 +79,7 @@ struct ViewSet {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<FrameBuffers> key;
-  std::vector<uint32_t> key_lanes;
+  std::vector<uint32_t> key_lanes, key_order, key_slots;
   bool graph_valid = false;
   bool rendered = false;
 };
@@ -112,6 +117,9 @@ struct gs_ctx {
   // Intersection capacity of the single-camera path.
   uint32_t capacity = 0;
   std::vector<Lane> lanes;
+  // Working sets of the second and later views of a view group (no streams:
+  // a group's views are launched together on the context stream).
+  std::vector<Lane> slots;
   ViewSet single, batch;
   // Camera-table staging: a ring of pinned slots.
   static constexpr int kRing = 8;
@@ -340,6 +348,7 @@ extern "C" void gs_config_default(gs_config* c) {
   c->radix_bits = 8;
   c->raster_batch = 256;
   c->batch_lanes = 1;
+  c->view_group = 1;
   c->heavy_split_units = GS_SPLIT_AUTO;
   c->heavy_fine_split_units = GS_SPLIT_AUTO;
 }
@@ -394,6 +403,10 @@ static int validate_config(const gs_config* c, const char*& msg) {
   // gs_render_batch whose cause is not established.
   if (c->batch_lanes != 1) {
     msg = "batch lanes above 1 are disabled";
+    return GS_E_RANGE;
+  }
+  if (c->view_group < 1 || c->view_group > (uint32_t)gs::kViewGroupMax) {
+    msg = "viewGroup must be an integer in [1, 8]";
     return GS_E_RANGE;
   }
   return GS_OK;
@@ -460,6 +473,7 @@ extern "C" void gs_destroy(gs_ctx* ctx) {
     for (DeviceBuffer* b : cb) release(*b);
   }
   for (Lane& l : ctx->lanes) lane_destroy(ctx, l);
+  for (Lane& l : ctx->slots) lane_destroy(ctx, l);
   for (int i = 0; i < gs_ctx::kRing; i++) {
     if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
     if (ctx->ring_done[i]) (void)hipEventDestroy(ctx->ring_done[i]);
@@ -914,6 +928,20 @@ int upload_cameras(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames
 // and joins it at the end, frames of one lane run in order.
 hipError_t enqueue_set(gs_ctx* ctx, ViewSet& set, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
+  if (set.group > 1) {
+    // View groups: each stage one launch over the group's views, on the
+    // context stream (the raster's second branch on lane 0's aux stream).
+    const Lane& l0 = ctx->lanes[0];
+    FrameBuffers grp[gs::kViewGroupMax];
+    for (uint32_t first = 0; first < set.n; first += set.group) {
+      uint32_t m = std::min(set.group, set.n - first);
+      for (uint32_t k = 0; k < m; k++) grp[k] = set.frames[set.order[first + k]];
+      hipError_t e = gs::launch_group(grp, m, s, gs::RasterStreams{l0.aux, l0.fork, l0.join},
+                                      marks);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   uint32_t used = 0;
   for (uint32_t v = 0; v < set.n; v++) used = std::max(used, set.lane_of[v] + 1);
   hipError_t e = hipSuccess;
@@ -938,7 +966,9 @@ hipError_t enqueue_set(gs_ctx* ctx, ViewSet& set, hipEvent_t* marks) {
 }
 
 bool same_key(const ViewSet& set) {
-  if (!set.graph_valid || set.key.size() != set.n || set.key_lanes != set.lane_of) return false;
+  if (!set.graph_valid || set.key.size() != set.n || set.key_lanes != set.lane_of ||
+      set.key_order != set.order || set.key_slots != set.slot_of)
+    return false;
   return memcmp(set.key.data(), set.frames.data(), sizeof(FrameBuffers) * set.n) == 0;
 }
 
@@ -1003,6 +1033,8 @@ int launch_set(gs_ctx* ctx, ViewSet& set, bool profile) {
     set.graph = g;
     set.key = set.frames;
     set.key_lanes = set.lane_of;
+    set.key_order = set.order;
+    set.key_slots = set.slot_of;
     set.graph_valid = true;
   }
   hipError_t e = hipGraphLaunch(set.exec, s);
@@ -1029,6 +1061,28 @@ void assign_lanes(gs_ctx* ctx, ViewSet& set, const gs_frame* frames, bool single
     set.lane_of[v] = best;
     load[best] += (uint64_t)frames[v].width * frames[v].height;
   }
+}
+
+// View groups of a batch: the views by area, largest first (stable), in
+// groups of view_group; view k of a group uses working set k.
+void assign_groups(gs_ctx* ctx, ViewSet& set, const gs_frame* frames, bool single) {
+  set.group = single ? 1u : std::min<uint32_t>(ctx->cfg.view_group, set.n);
+  if (set.group == 0) set.group = 1;
+  set.order.resize(set.n);
+  for (uint32_t v = 0; v < set.n; v++) set.order[v] = v;
+  set.slot_of.assign(set.n, 0);
+  if (set.group == 1) return;
+  std::stable_sort(set.order.begin(), set.order.end(), [&](uint32_t a, uint32_t b) {
+    return (uint64_t)frames[a].width * frames[a].height >
+           (uint64_t)frames[b].width * frames[b].height;
+  });
+  for (uint32_t i = 0; i < set.n; i++) set.slot_of[set.order[i]] = i % set.group;
+}
+
+// Working set of view v: its lane's, or a group slot's.
+Lane& working_set(gs_ctx* ctx, const ViewSet& set, uint32_t v) {
+  uint32_t j = set.slot_of[v];
+  return j == 0 ? ctx->lanes[set.lane_of[v]] : ctx->slots[j - 1];
 }
 
 // prepareFrame + render for the views of a set (TiledGaussianPipeline.ts
@@ -1059,18 +1113,27 @@ int render_set(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames,
   set.width.resize(n);
   set.height.resize(n);
   assign_lanes(ctx, set, frames, single);
-  // Lane working sets: the largest capacity and tile grid of their views.
+  assign_groups(ctx, set, frames, single);
+  if (ctx->slots.size() + 1 < set.group) ctx->slots.resize(set.group - 1);
+  // Working sets (lanes, then group slots): the largest capacity and tile
+  // grid of their views.
   uint32_t ts = ctx->cfg.tile_size;
-  std::vector<uint32_t> lane_cap(ctx->lanes.size(), 0), lane_tiles(ctx->lanes.size(), 0);
+  const size_t n_sets = ctx->lanes.size() + ctx->slots.size();
+  auto set_index = [&](uint32_t v) {
+    return set.slot_of[v] == 0 ? (size_t)set.lane_of[v]
+                               : ctx->lanes.size() + set.slot_of[v] - 1;
+  };
+  std::vector<uint32_t> lane_cap(n_sets, 0), lane_tiles(n_sets, 0);
   for (uint32_t v = 0; v < n; v++) {
-    uint32_t l = set.lane_of[v];
+    size_t l = set_index(v);
     uint32_t tiles = div_up(frames[v].width, ts) * div_up(frames[v].height, ts);
     lane_cap[l] = std::max(lane_cap[l], set.capacity[v]);
     lane_tiles[l] = std::max(lane_tiles[l], tiles);
   }
-  for (size_t l = 0; l < ctx->lanes.size(); l++)
-    if (lane_cap[l] && (rc = lane_frame_buffers(ctx, ctx->lanes[l], lane_cap[l], lane_tiles[l])))
-      return rc;
+  for (size_t l = 0; l < n_sets; l++) {
+    Lane& w = l < ctx->lanes.size() ? ctx->lanes[l] : ctx->slots[l - ctx->lanes.size()];
+    if (lane_cap[l] && (rc = lane_frame_buffers(ctx, w, lane_cap[l], lane_tiles[l]))) return rc;
+  }
   // Outputs.
   std::vector<void*> color_dst(n, nullptr), depth_dst(n, nullptr);
   for (uint32_t v = 0; v < n; v++) {
@@ -1090,7 +1153,9 @@ int render_set(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames,
   for (uint32_t v = 0; v < n; v++) {
     uint32_t l = set.lane_of[v];
     if (l >= ctx->lanes.size()) return fail(ctx, GS_E_STATE, "view assigned to no lane");
-    const Lane& lane = ctx->lanes[l];
+    if (set.slot_of[v] > ctx->slots.size())
+      return fail(ctx, GS_E_STATE, "view assigned to no group slot");
+    const Lane& lane = working_set(ctx, set, v);
     uint32_t tiles = div_up(frames[v].width, ts) * div_up(frames[v].height, ts);
     if (lane.capacity < set.capacity[v] || lane.tile_capacity < tiles ||
         lane.slot_capacity < ctx->active_count)
@@ -1098,10 +1163,18 @@ int render_set(gs_ctx* ctx, ViewSet& set, uint32_t n, const gs_frame* frames,
   }
   set.frames.resize(n);
   for (uint32_t v = 0; v < n; v++) {
-    fill_frame(ctx, set, v, &frames[v], ctx->lanes[set.lane_of[v]], color_dst[v], depth_dst[v],
+    fill_frame(ctx, set, v, &frames[v], working_set(ctx, set, v), color_dst[v], depth_dst[v],
                set.frames[v]);
     set.width[v] = frames[v].width;
     set.height[v] = frames[v].height;
+  }
+  // The views of a group sort their tile keys with the same passes: a pass
+  // above a view's own tile bits finds one digit for every key and copies.
+  for (uint32_t first = 0; set.group > 1 && first < n; first += set.group) {
+    uint32_t m = std::min(set.group, n - first), passes = 0;
+    for (uint32_t k = 0; k < m; k++)
+      passes = std::max(passes, set.frames[set.order[first + k]].tile_passes);
+    for (uint32_t k = 0; k < m; k++) set.frames[set.order[first + k]].tile_passes = passes;
   }
   if ((rc = upload_cameras(ctx, set, n, frames))) return rc;
   if ((rc = launch_set(ctx, set, single && ctx->profile_capacity != 0))) return rc;

@@ -21,6 +21,7 @@ constexpr int kSortPartition = kSortThreads * kSortItems;  // keys per sort bloc
 constexpr int kMaxRadixBits = 8;     // widest digit of the LSD sorts
 constexpr int kMaxRadix = 1 << kMaxRadixBits;
 constexpr int kMaxTile = 32;         // widest tile edge (pixels)
+constexpr int kViewGroupMax = 8;     // views of one group launch
 constexpr int kMaxSortPasses = 8;    // 32-bit keys with 4-bit digits
 // Digit totals of the two sorts of a frame (depth, then tile): each sort has
 // kMaxSortPasses rows of 256 totals, each row held as kHistShards partial
@@ -212,6 +213,13 @@ struct RasterStreams {
 // `marks` is null or kMarks events (stage boundaries, heavy-unit kernels).
 hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, const RasterStreams& rs,
                         hipEvent_t* marks);
+// Enqueue the frames of a group of up to kViewGroupMax views on `stream`,
+// each stage one launch over all of them. The views share the launch shape,
+// the mode (plain, cull, collect), the scene and their sort pass counts
+// (hipErrorInvalidValue otherwise); each has its own working buffers,
+// state and outputs. launch_frame is the group of one.
+hipError_t launch_group(const FrameBuffers* views, uint32_t n, hipStream_t stream,
+                        const RasterStreams& rs, hipEvent_t* marks);
 // Location of the sorted depth / tile records after launch_frame
 // (depth_in_place: DeviceState::depth_in_place of the frame).
 const uint2* sorted_depth_records(const FrameBuffers& b, uint32_t depth_in_place);

@@ -18,6 +18,9 @@
 // Every size that the reference reads through indirect dispatch (V, K) is
 // read from DeviceState on the device, so the frame can be replayed as a
 // captured graph; grids are sized by capacity and idle blocks exit at once.
+#include <algorithm>
+#include <utility>
+
 #include "gs_cull.h"
 #include "gs_internal.h"
 #include "gs_project.h"
@@ -449,6 +452,28 @@ GS_DEV float sh_top_exponent(const FrameParams& fp, const SceneView& sc, uint32_
   return (float)(int)top - 127.0f;
 }
 
+// ------------------------------------------------------ group launches ----
+//
+// Every kernel of the frame takes its arguments for a group of up to
+// kGroupMax views at once (by value, in the kernel argument segment, so a
+// captured launch keeps them): block b works for view b / bpv as its block
+// b % bpv, or, in the persistent raster kernels, walks the views in order.
+// A single frame is a group of one. Batches render several views per
+// group, so each stage is one launch over all of them: the short,
+// latency-bound launches (sort passes, scans, the heavy-unit list) carry
+// several views' work, and one view's raster tail overlaps the next one's
+// units.
+constexpr int kGroupMax = kViewGroupMax;
+// Kernels are instantiated for groups of one (kN = 1: the view index is a
+// constant, so the arguments are read at fixed offsets as for a plain
+// launch) and of up to kGroupMax views.
+template <class A, int kN = kGroupMax>
+struct ViewArgs {
+  A v[kN];
+  uint32_t n;    // views in the group
+  uint32_t bpv;  // blocks per view
+};
+
 // ProjectionStage.ts:385-649 for active ordinal gid (live lanes; every
 // lane of the wave must call it). Boxes above kMaskTiles tiles are counted
 // by the whole wave. With kMode != plain it also fills the 16 cull features
@@ -495,13 +520,28 @@ GS_DEV void store_projection(const ProjectedView& out, uint32_t gid, const ProjO
 // Projection of every active Gaussian; the collect variant also stores the
 // features and clears the per-Gaussian contribution record of the
 // instrumented raster.
-template <int kMode>
-__global__ __launch_bounds__(kBlock) void k_project(const FrameParams fp,
-                                                    DeviceState* __restrict__ state,
-                                                    SceneView sc, ProjectedView out,
-                                                    float* __restrict__ feats,
-                                                    uint32_t* __restrict__ record) {
-  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+struct ProjectArgs {
+  FrameParams fp;
+  DeviceState* state;
+  SceneView sc;
+  ProjectedView out;
+  float* feats;
+  uint32_t* record;
+  const CullNet* net;
+};
+
+template <int kMode, int kN>
+__global__ __launch_bounds__(kBlock) void k_project(const ViewArgs<ProjectArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const ProjectArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  DeviceState* __restrict__ state = a.state;
+  const SceneView& sc = a.sc;
+  const ProjectedView& out = a.out;
+  float* __restrict__ feats = a.feats;
+  uint32_t* __restrict__ record = a.record;
+  const uint32_t block = blockIdx.x - view * grp.bpv;
+  uint32_t gid = block * kBlock + threadIdx.x;
   if (gid == 0) {
     state->checked = 0ull;
     state->blended = 0ull;
@@ -533,10 +573,16 @@ struct CullSurvivor {
   uint32_t gid, slot;
   float cx, cy, depth, opacity, c0, c1, c2, thr, rx, ry, d0, d1, d2;
 };
-__global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
-                                                         DeviceState* __restrict__ state,
-                                                         SceneView sc, ProjectedView out,
-                                                         const CullNet* __restrict__ net) {
+template <int kN>
+__global__ __launch_bounds__(kBlock) void k_project_cull(const ViewArgs<ProjectArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const ProjectArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  DeviceState* __restrict__ state = a.state;
+  const SceneView& sc = a.sc;
+  const ProjectedView& out = a.out;
+  const CullNet* __restrict__ net = a.net;
+  const uint32_t block = blockIdx.x - view * grp.bpv;
   struct Mlp {
     CullLdsNet net;
     __align__(16) uint16_t x[kBlock / 64][64 * kCullIn];
@@ -547,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_project_cull(const FrameParams fp,
     CullSurvivor surv[kBlock];
   } lds;
   __shared__ uint32_t wave_kept[kBlock / 64];
-  uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t gid = block * kBlock + threadIdx.x;
   if (gid == 0) {
     state->checked = 0ull;
     state->blended = 0ull;
@@ -739,17 +785,35 @@ GS_DEV void scan_finish(ScanMode mode, uint32_t total, uint32_t capacity,
 GS_DEV uint32_t wave_sum_sat(uint32_t v) { return __shfl(wave_inclusive_scan(v), 63, 64); }
 
 template <class Load, class Store>
-__global__ __launch_bounds__(kBlock) void k_scan(Load load, Store store, LenSpec len_spec,
-                                                 ScanWorkspace ws, ScanMode mode,
-                                                 uint32_t capacity,
-                                                 const FrameCamera* __restrict__ camera,
-                                                 DeviceState* __restrict__ state) {
+struct ScanArgs {
+  Load load;
+  Store store;
+  LenSpec len_spec;
+  ScanWorkspace ws;
+  ScanMode mode;
+  uint32_t capacity;
+  const FrameCamera* camera;
+  DeviceState* state;
+};
+
+template <class Load, class Store, int kN>
+__global__ __launch_bounds__(kBlock) void k_scan(const ViewArgs<ScanArgs<Load, Store>, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const ScanArgs<Load, Store>& a = grp.v[view];
+  const Load& load = a.load;
+  const Store& store = a.store;
+  const LenSpec& len_spec = a.len_spec;
+  const ScanWorkspace& ws = a.ws;
+  const ScanMode mode = a.mode;
+  const uint32_t capacity = a.capacity;
+  const FrameCamera* __restrict__ camera = a.camera;
+  DeviceState* __restrict__ state = a.state;
   __shared__ uint32_t s_prefix;
   __shared__ uint32_t wave_tot[kBlock / 64];
   const uint32_t t = threadIdx.x, wave = t >> 6, lane = lane_id();
   const uint32_t len = eval_len(len_spec);
   const uint32_t nparts = (uint32_t)(((uint64_t)len + kScanPartition - 1) / kScanPartition);
-  const uint32_t part = blockIdx.x;
+  const uint32_t part = blockIdx.x - view * grp.bpv;
   if (part < nparts) {
     const uint32_t run = part * (uint32_t)kScanPartition + wave * 64u * kScanItems;
     uint32_t val[kScanItems], ex[kScanItems];
@@ -956,28 +1020,40 @@ GS_DEV uint64_t digit_peers(uint32_t d) {
 // count / (kHistKeys kHistShards): a grid sized for a sort's capacity
 // would add every block's counts (at about 12 ns per same-address atomic
 // on this GPU, 1,024 blocks cost 12 us per total).
-template <int kBits>
-__global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restrict__ in,
-                                                            const uint32_t* __restrict__ count_ptr,
-                                                            uint32_t passes, SortWorkspace ws) {
+struct HistArgs {
+  const uint2* in;
+  const uint32_t* count_ptr;
+  uint32_t passes;
+  SortWorkspace ws;
+};
+
+template <int kBits, int kN>
+__global__ __launch_bounds__(kBlock) void k_sort_histograms(const ViewArgs<HistArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const HistArgs& a = grp.v[view];
+  const uint2* __restrict__ in = a.in;
+  const uint32_t* __restrict__ count_ptr = a.count_ptr;
+  const uint32_t passes = a.passes;
+  const SortWorkspace& ws = a.ws;
+  const uint32_t block = blockIdx.x - view * grp.bpv, grid = grp.bpv;
   constexpr uint32_t kRadix = 1u << kBits;
   __shared__ uint32_t h[kMaxPasses * kRadix];
   const uint32_t count = *count_ptr, lane = lane_id();
   {
-    const uint32_t stride = gridDim.x * kBlock, first = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t stride = grid * kBlock, first = block * kBlock + threadIdx.x;
     const uint32_t nparts = (count + kSortPartition - 1) / kSortPartition;
     for (uint32_t i = first; i < (uint32_t)(kMaxPasses * kHistShards * 256); i += stride)
       ws.ghist_other[i] = 0u;
     for (uint32_t i = first; i < nparts * 256u; i += stride) ws.status[i] = 0u;
     for (uint32_t i = first; i < ws.scan_words; i += stride) ws.scan_status[i] = 0ull;
   }
-  const uint32_t counting = min(gridDim.x, max(1u, (count + kHistKeys - 1) / kHistKeys));
-  if (blockIdx.x >= counting) return;  // block-uniform
+  const uint32_t counting = min(grid, max(1u, (count + kHistKeys - 1) / kHistKeys));
+  if (block >= counting) return;  // block-uniform
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0u;
   __syncthreads();
   // Each wave takes runs of 64 * kHistItems keys, all loads in flight at once.
   const uint32_t stride = counting * kHistKeys;
-  for (uint32_t base = (blockIdx.x * kBlock + (threadIdx.x & ~63u)) * kHistItems; base < count;
+  for (uint32_t base = (block * kBlock + (threadIdx.x & ~63u)) * kHistItems; base < count;
        base += stride) {
     uint32_t key[kHistItems];
 #pragma unroll
@@ -995,18 +1071,32 @@ __global__ __launch_bounds__(kBlock) void k_sort_histograms(const uint2* __restr
     }
   }
   __syncthreads();
-  uint32_t* shard = ws.ghist + (blockIdx.x % kHistShards) * 256u;
+  uint32_t* shard = ws.ghist + (block % kHistShards) * 256u;
   for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kBlock)
     if (h[i]) atomicAdd(&shard[(i / kRadix) * kHistShards * 256u + i % kRadix], h[i]);
 }
 
-template <int kBits, int kItems>
-__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restrict__ in,
-                                                      uint2* __restrict__ out,
-                                                      const uint32_t* __restrict__ count_ptr,
-                                                      uint32_t pass, SortWorkspace ws,
-                                                      DeviceState* __restrict__ state,
-                                                      uint32_t* __restrict__ in_place) {
+struct SortPassArgs {
+  const uint2* in;
+  uint2* out;
+  const uint32_t* count_ptr;
+  uint32_t pass;
+  SortWorkspace ws;
+  DeviceState* state;
+  uint32_t* in_place;
+};
+
+template <int kBits, int kItems, int kN>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const ViewArgs<SortPassArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const SortPassArgs& a = grp.v[view];
+  const uint2* __restrict__ in = a.in;
+  uint2* __restrict__ out = a.out;
+  const uint32_t* __restrict__ count_ptr = a.count_ptr;
+  const uint32_t pass = a.pass;
+  const SortWorkspace& ws = a.ws;
+  DeviceState* __restrict__ state = a.state;
+  uint32_t* __restrict__ in_place = a.in_place;
   static_assert(kItems % kSortItems == 0, "partitions are whole workspace partitions");
   constexpr uint32_t kRadix = 1u << kBits;
   constexpr uint32_t kRun = 64u * kItems;  // keys of one wave, consecutive
@@ -1021,7 +1111,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
   const uint32_t t = threadIdx.x;
   const uint32_t count = *count_ptr;
   const uint32_t nparts = (count + kPartition - 1) / kPartition;
-  const uint32_t part = blockIdx.x;
+  const uint32_t part = blockIdx.x - view * grp.bpv;
   if (in_place) {
     // The last pass: a digit shared by every key leaves the order as it is.
     const uint32_t total = t < kRadix ? digit_total(ws, pass, t) : 0u;
@@ -1158,15 +1248,30 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint2* __restr
 // whole wave, one box at a time in chunks of 64 tiles: lane l tests the
 // chunk's l-th tile and its key goes after the hits of the earlier tiles
 // (a ballot prefix), which is the row-major order of the serial walk.
-__global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
-                                                 const DeviceState* __restrict__ state,
-                                                 const uint2* __restrict__ sorted_a,
-                                                 const uint2* __restrict__ sorted_b,
-                                                 const Splat* __restrict__ splat,
-                                                 const uint4* __restrict__ ordered_hits,
-                                                 const uint32_t* __restrict__ offsets,
-                                                 uint2* __restrict__ records) {
-  uint32_t rank = blockIdx.x * kBlock + threadIdx.x;
+struct EmitArgs {
+  FrameParams fp;
+  const DeviceState* state;
+  const uint2* sorted_a;
+  const uint2* sorted_b;
+  const Splat* splat;
+  const uint4* ordered_hits;
+  const uint32_t* offsets;
+  uint2* records;
+};
+
+template <int kN>
+__global__ __launch_bounds__(kBlock) void k_emit(const ViewArgs<EmitArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const EmitArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  const DeviceState* __restrict__ state = a.state;
+  const uint2* __restrict__ sorted_a = a.sorted_a;
+  const uint2* __restrict__ sorted_b = a.sorted_b;
+  const Splat* __restrict__ splat = a.splat;
+  const uint4* __restrict__ ordered_hits = a.ordered_hits;
+  const uint32_t* __restrict__ offsets = a.offsets;
+  uint2* __restrict__ records = a.records;
+  uint32_t rank = (blockIdx.x - view * grp.bpv) * kBlock + threadIdx.x;
   if (state->overflow) return;  // uniform
   const bool live = rank < state->visible;
   if (__ballot(live) == 0ull) return;  // wave-uniform
@@ -1274,13 +1379,25 @@ __global__ __launch_bounds__(kBlock) void k_emit(const FrameParams fp,
 // the first sentinel). Runs of empty tiles can be long (a frame with few
 // occupied tiles), so a lane with a run above 8 tiles hands it to its whole
 // wave, which fills it 64 tiles at a time.
-__global__ __launch_bounds__(kBlock) void k_tile_offsets(const FrameParams fp,
-                                                         const DeviceState* __restrict__ state,
-                                                         const uint2* __restrict__ sorted,
-                                                         uint32_t* __restrict__ off) {
+struct TileOffsetArgs {
+  FrameParams fp;
+  const DeviceState* state;
+  const uint2* sorted;
+  uint32_t* off;
+};
+
+template <int kN>
+__global__ __launch_bounds__(kBlock) void k_tile_offsets(const ViewArgs<TileOffsetArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const TileOffsetArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  const DeviceState* __restrict__ state = a.state;
+  const uint2* __restrict__ sorted = a.sorted;
+  uint32_t* __restrict__ off = a.off;
+  const uint32_t block = blockIdx.x - view * grp.bpv, grid = grp.bpv;
   const uint32_t count = state->count, tiles = fp.tile_count, lane = lane_id();
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base <= count;
-       base += gridDim.x * kBlock) {
+  for (uint32_t base = block * kBlock + (threadIdx.x & ~63u); base <= count;
+       base += grid * kBlock) {
     uint32_t i = base + lane;
     uint32_t lo = 1u, hi = 0u;  // empty
     if (i <= count) {
@@ -1983,15 +2100,37 @@ GS_DEV void light_tile(const FrameParams& fp, DeviceState* __restrict__ state, u
 // slab kernel takes the light tiles and the empty ones too, and this kernel
 // exits at once (while the whole-tile kernel it runs beside does the same),
 // before the slab kernel's persistent grid takes every CU.
-template <int kEdge, int kBatch, bool kRecord>
+struct RasterArgs {
+  FrameParams fp;
+  DeviceState* state;
+  const uint2* sorted;
+  const uint32_t* off;
+  const Splat* splat;
+  const uint2* units;
+  const uint32_t* chunk_first;
+  float4* partials;
+  float* partial_depth;
+  uint16_t* color_out;
+  float* depth_out;
+  uint32_t* record;
+};
+
+template <int kEdge, int kBatch, bool kRecord, int kN>
 __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
-    const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
-    const uint32_t* __restrict__ off, const Splat* __restrict__ splat,
-    uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
-    uint32_t* __restrict__ record) {
+    const ViewArgs<RasterArgs, kN> grp) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
-  if (state->overflow) return;  // keep the previous image (intersections.ts:114-134)
-  if (kEdge == 16 && state->heavy_split != 0u) return;
+  for (uint32_t view = 0; view < (kN == 1 ? 1u : grp.n); view++) {
+  const RasterArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  DeviceState* __restrict__ state = a.state;
+  const uint2* __restrict__ sorted = a.sorted;
+  const uint32_t* __restrict__ off = a.off;
+  const Splat* __restrict__ splat = a.splat;
+  uint16_t* __restrict__ color_out = a.color_out;
+  float* __restrict__ depth_out = a.depth_out;
+  uint32_t* __restrict__ record = a.record;
+  if (state->overflow) continue;  // keep the previous image (intersections.ts:114-134)
+  if (kEdge == 16 && state->heavy_split != 0u) continue;
   for (uint32_t tile = blockIdx.x; tile < fp.tile_count; tile += gridDim.x) {
     uint32_t begin = off[tile], end = off[tile + 1];
     // Heavy units and chunked tiles (chunk sizes below kLightMax make some
@@ -2001,6 +2140,7 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
     light_tile<kEdge, kBatch, kRecord>(fp, state, tile, begin, end, sorted, splat, lds,
                                        color_out, depth_out, record);
   }
+  }
 }
 
 // rasterChunks.ts:3-65 in one block, extended into the heavy work list:
@@ -2009,12 +2149,24 @@ __global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
 // each tile's first task), then the remaining tiles above kLightMax,
 // heaviest size class first. Units are (tile, chunk) with chunk = ~0u for a
 // whole tile.
-__global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
-                                                      DeviceState* __restrict__ state,
-                                                      const uint32_t* __restrict__ off,
-                                                      uint32_t* __restrict__ chunk_first,
-                                                      uint2* __restrict__ units,
-                                                      uint32_t unit_capacity) {
+struct HeavyUnitArgs {
+  FrameParams fp;
+  DeviceState* state;
+  const uint32_t* off;
+  uint32_t* chunk_first;
+  uint2* units;
+  uint32_t unit_capacity;
+};
+
+template <int kN>
+__global__ __launch_bounds__(1024) void k_heavy_units(const ViewArgs<HeavyUnitArgs, kN> grp) {
+  const HeavyUnitArgs& a = grp.v[kN == 1 ? 0u : blockIdx.x];
+  const FrameParams& fp = a.fp;
+  DeviceState* __restrict__ state = a.state;
+  const uint32_t* __restrict__ off = a.off;
+  uint32_t* __restrict__ chunk_first = a.chunk_first;
+  uint2* __restrict__ units = a.units;
+  const uint32_t unit_capacity = a.unit_capacity;
   // Tiles are visited in strides of the block; the waves reserve their chunk
   // tasks with one LDS add each, so no iteration waits on another (a tile's
   // chunks are contiguous and start at chunk_first[tile]; the order between
@@ -2105,14 +2257,9 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const FrameParams fp,
 // tiles per item. A block starts with item b / 8 and then takes the XCD's
 // next unclaimed item (one atomic per item), so blocks that finish early
 // keep working.
-template <int kEdge, int kShape, bool kRecord>
+template <int kEdge, int kShape, bool kRecord, int kN>
 __global__ __launch_bounds__(heavy_threads(kEdge, kShape)) void k_raster_heavy(
-    const FrameParams fp, DeviceState* __restrict__ state, const uint2* __restrict__ sorted,
-    const uint32_t* __restrict__ off, const Splat* __restrict__ splat,
-    const uint2* __restrict__ units,
-    float4* __restrict__ partials, float* __restrict__ partial_depth,
-    uint16_t* __restrict__ color_out, float* __restrict__ depth_out,
-    uint32_t* __restrict__ record) {
+    const ViewArgs<RasterArgs, kN> grp) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr bool kGroup = kEdge != 32;  // lane groups per pixel (edges 8, 16)
   constexpr bool kSplit = kShape != kShapeWhole;  // row slabs, which also take the light tiles
@@ -2120,7 +2267,20 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kShape)) void k_raster_heavy(
   constexpr int kBatch = heavy_batch(kEdge, kShape);
   __shared__ RasterLds<heavy_slab_pixels(kEdge, kShape), kBatch> lds;
   __shared__ uint32_t next_item;
-  if (kEdge == 16 && state->heavy_split != (uint32_t)kShape) return;
+  for (uint32_t view = 0; view < (kN == 1 ? 1u : grp.n); view++) {
+  const RasterArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  DeviceState* __restrict__ state = a.state;
+  const uint2* __restrict__ sorted = a.sorted;
+  const uint32_t* __restrict__ off = a.off;
+  const Splat* __restrict__ splat = a.splat;
+  const uint2* __restrict__ units = a.units;
+  float4* __restrict__ partials = a.partials;
+  float* __restrict__ partial_depth = a.partial_depth;
+  uint16_t* __restrict__ color_out = a.color_out;
+  float* __restrict__ depth_out = a.depth_out;
+  uint32_t* __restrict__ record = a.record;
+  if (kEdge == 16 && state->heavy_split != (uint32_t)kShape) continue;
   const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
   const uint32_t n_heavy = state->heavy_units;
   const uint32_t n_light = kSplit ? state->light_units : 0u;
@@ -2198,6 +2358,7 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kShape)) void k_raster_heavy(
     __syncthreads();
     item = next_item;
   }
+  }
 }
 
 // Persistent grid of k_raster_heavy, a multiple of 8 blocks (at least 8):
@@ -2215,15 +2376,21 @@ inline uint32_t heavy_grid(uint32_t cu_count) {
 }
 
 // TileRasterizer.ts:807-902: ordered over-composite of a tile's chunks.
-template <int kEdge>
-__global__ __launch_bounds__(kEdge * kEdge) void k_composite(
-    const FrameParams fp, const DeviceState* __restrict__ state,
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ chunk_first,
-    const float4* __restrict__ partials, const float* __restrict__ partial_depth,
-    uint16_t* __restrict__ color_out, float* __restrict__ depth_out) {
+template <int kEdge, int kN>
+__global__ __launch_bounds__(kEdge * kEdge) void k_composite(const ViewArgs<RasterArgs, kN> grp) {
+  const uint32_t view = kN == 1 ? 0u : blockIdx.x / grp.bpv;
+  const RasterArgs& a = grp.v[view];
+  const FrameParams& fp = a.fp;
+  const DeviceState* __restrict__ state = a.state;
+  const uint32_t* __restrict__ off = a.off;
+  const uint32_t* __restrict__ chunk_first = a.chunk_first;
+  const float4* __restrict__ partials = a.partials;
+  const float* __restrict__ partial_depth = a.partial_depth;
+  uint16_t* __restrict__ color_out = a.color_out;
+  float* __restrict__ depth_out = a.depth_out;
   constexpr int kPixels = kEdge * kEdge;
   if (state->overflow) return;
-  uint32_t tile = blockIdx.x;
+  uint32_t tile = blockIdx.x - view * grp.bpv;
   if (tile >= fp.tile_count || fp.chunk_size == 0) return;
   uint32_t count = off[tile + 1] - off[tile];
   uint32_t cs = fp.chunk_size;
@@ -2254,63 +2421,129 @@ namespace {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// A sort of one view of a group: its ping-pong buffers, key count, workspace
+// and state.
+struct SortView {
+  uint2* a;
+  uint2* b;
+  const uint32_t* count_ptr;
+  SortWorkspace ws;
+  DeviceState* state;
+  uint32_t* in_place;
+};
+
+// The first view's arguments of a group, for the kN = 1 instances.
+template <class A>
+ViewArgs<A, 1> first_view(const ViewArgs<A>& g) {
+  ViewArgs<A, 1> o{};
+  o.v[0] = g.v[0];
+  o.n = 1;
+  o.bpv = g.bpv;
+  return o;
+}
+
+template <class A>
+ViewArgs<A> view_args(uint32_t n, uint32_t bpv) {
+  ViewArgs<A> g{};
+  g.n = n;
+  g.bpv = bpv ? bpv : 1u;
+  return g;
+}
+
 // Sort pairs in a <-> b for `passes` kBits-bit digits; the result lands in
-// a for an even pass count, in b otherwise.
+// a for an even pass count, in b otherwise. Every view of the group runs
+// the same passes (a pass whose digit is zero for every key only copies).
 template <int kBits, int kItems>
-void sort_passes(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
-                 uint32_t passes, const SortWorkspace& ws, DeviceState* state,
-                 uint32_t* in_place, hipStream_t s) {
+void sort_passes(const SortView* sv, uint32_t n, uint32_t max_count, uint32_t passes,
+                 hipStream_t s) {
   uint32_t blocks = div_up(max_count, (uint64_t)kSortThreads * kItems);
   if (blocks == 0) blocks = 1;
   for (uint32_t p = 0; p < passes; p++) {
-    const uint2* in = (p % 2 == 0) ? a : b;
-    uint2* out = (p % 2 == 0) ? b : a;
-    k_sort_pass<kBits, kItems><<<blocks, kSortThreads, 0, s>>>(
-        in, out, count_ptr, p, ws, state, p + 1 == passes ? in_place : nullptr);
+    ViewArgs<SortPassArgs> grp = view_args<SortPassArgs>(n, blocks);
+    for (uint32_t v = 0; v < n; v++)
+      grp.v[v] = {(p % 2 == 0) ? sv[v].a : sv[v].b, (p % 2 == 0) ? sv[v].b : sv[v].a,
+                sv[v].count_ptr, p, sv[v].ws, sv[v].state,
+                p + 1 == passes ? sv[v].in_place : nullptr};
+    if (n == 1)
+      k_sort_pass<kBits, kItems, 1><<<blocks, kSortThreads, 0, s>>>(first_view(grp));
+    else
+      k_sort_pass<kBits, kItems, kGroupMax><<<n * blocks, kSortThreads, 0, s>>>(grp);
   }
 }
 
 template <int kBits>
-void radix_sort_bits(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
-                     uint32_t passes, const SortWorkspace& ws, DeviceState* state,
-                     uint32_t* in_place, hipStream_t s) {
+void radix_sort_bits(const SortView* sv, uint32_t n, uint32_t max_count, uint32_t passes,
+                     hipStream_t s) {
   uint32_t parts = div_up(max_count, kSortPartition);
   if (parts == 0) parts = 1;
   uint32_t hist_blocks = parts / 4u;
   hist_blocks = hist_blocks < kHistBlocksMin ? kHistBlocksMin
                                              : (hist_blocks > kHistBlocksMax ? kHistBlocksMax
                                                                              : hist_blocks);
-  k_sort_histograms<kBits><<<hist_blocks, kBlock, 0, s>>>(a, count_ptr, passes, ws);
+  ViewArgs<HistArgs> grp = view_args<HistArgs>(n, hist_blocks);
+  for (uint32_t v = 0; v < n; v++) grp.v[v] = {sv[v].a, sv[v].count_ptr, passes, sv[v].ws};
+  if (n == 1)
+    k_sort_histograms<kBits, 1><<<hist_blocks, kBlock, 0, s>>>(first_view(grp));
+  else
+    k_sort_histograms<kBits, kGroupMax><<<n * hist_blocks, kBlock, 0, s>>>(grp);
   if (max_count >= kLargeSort)
-    sort_passes<kBits, 2 * kSortItems>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
+    sort_passes<kBits, 2 * kSortItems>(sv, n, max_count, passes, s);
   else
-    sort_passes<kBits, kSortItems>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
+    sort_passes<kBits, kSortItems>(sv, n, max_count, passes, s);
 }
 
-// Sort pairs in a <-> b (see radix_sort_bits). With in_place, a last pass
-// whose digit is one value for every key is skipped and *in_place records
-// it; the sorted pairs are then in the last pass's input buffer.
-void radix_sort(uint2* a, uint2* b, const uint32_t* count_ptr, uint32_t max_count,
-                uint32_t passes, uint32_t bits, const SortWorkspace& ws, DeviceState* state,
-                uint32_t* in_place, hipStream_t s) {
+// Sort pairs in a <-> b (see radix_sort_bits) for each view of a group,
+// sized for max_count keys per view. With in_place, a last pass whose digit
+// is one value for every key of its view is skipped and *in_place records
+// it; that view's sorted pairs are then in the last pass's input buffer.
+void radix_sort(const SortView* sv, uint32_t n, uint32_t max_count, uint32_t passes,
+                uint32_t bits, hipStream_t s) {
   if (bits == 4)
-    radix_sort_bits<4>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
+    radix_sort_bits<4>(sv, n, max_count, passes, s);
   else if (bits == 6)
-    radix_sort_bits<6>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
+    radix_sort_bits<6>(sv, n, max_count, passes, s);
   else
-    radix_sort_bits<8>(a, b, count_ptr, max_count, passes, ws, state, in_place, s);
+    radix_sort_bits<8>(sv, n, max_count, passes, s);
 }
 
+template <int kEdge, int kShape, bool kRecord>
+void launch_heavy(const ViewArgs<RasterArgs>& ra, uint32_t cu_count, hipStream_t s) {
+  const uint32_t grid = heavy_grid<kEdge, kShape>(cu_count);
+  if (ra.n == 1)
+    k_raster_heavy<kEdge, kShape, kRecord, 1>
+        <<<grid, heavy_threads(kEdge, kShape), 0, s>>>(first_view(ra));
+  else
+    k_raster_heavy<kEdge, kShape, kRecord, kGroupMax>
+        <<<grid, heavy_threads(kEdge, kShape), 0, s>>>(ra);
+}
 
-// The raster stage: the heavy work list on s, then heavy units on aux
-// concurrently with light tiles on s, then the chunk composite.
+// Tile records of view b after a tile sort of `passes` passes.
+const uint2* tile_records_after(const FrameBuffers& b, uint32_t passes) {
+  return (passes % 2 == 0) ? b.records_a : b.records_b;
+}
+
+// The raster stage of a group: the heavy work lists on s, then heavy units
+// on aux concurrently with light tiles on s, then the chunk composite.
 template <int kEdge, int kBatch, bool kRecord>
-hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
+hipError_t launch_raster(const FrameBuffers* views, uint32_t n, hipStream_t s,
                          const RasterStreams& rs, hipEvent_t* marks) {
-  const FrameParams& fp = b.params;
-  DeviceState* st = b.state;
-  k_heavy_units<<<1, 1024, 0, s>>>(fp, st, b.tile_offsets, b.chunk_first, b.tasks,
-                                   b.unit_capacity);
+  const FrameBuffers& b0 = views[0];
+  uint32_t max_tiles = 1;
+  ViewArgs<HeavyUnitArgs> hu = view_args<HeavyUnitArgs>(n, 1);
+  ViewArgs<RasterArgs> ra = view_args<RasterArgs>(n, 1);
+  for (uint32_t v = 0; v < n; v++) {
+    const FrameBuffers& b = views[v];
+    max_tiles = std::max(max_tiles, b.max_tiles);
+    hu.v[v] = {b.params, b.state, b.tile_offsets, b.chunk_first, b.tasks, b.unit_capacity};
+    ra.v[v] = {b.params,      b.state,          tile_records_after(b, b.tile_passes),
+               b.tile_offsets, b.proj.splat,    b.tasks,
+               b.chunk_first,  b.chunk_partials, b.partial_depth,
+               b.color_out,    b.depth_out,      b.record};
+  }
+  if (n == 1)
+    k_heavy_units<1><<<1, 1024, 0, s>>>(first_view(hu));
+  else
+    k_heavy_units<kGroupMax><<<n, 1024, 0, s>>>(hu);
   // Without a second stream (rs.aux null) the heavy kernels run on s, ahead
   // of the light-tile kernel, and no event is recorded.
   const hipStream_t aux = rs.aux ? rs.aux : s;
@@ -2320,131 +2553,230 @@ hipError_t launch_raster(const FrameBuffers& b, const uint2* sorted, hipStream_t
     if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
     if (e != hipSuccess) return e;
   }
-  // The whole-tile kernel, then (edge 16) the two row-slab kernels: all but
-  // the frame's shape exit at once, each before the next can fill the GPU.
+  // The whole-tile kernel, then (edge 16) the two row-slab kernels: each
+  // takes the views of its shape, and one whose views all chose another
+  // shape exits at once, before the next can fill the GPU.
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
-  k_raster_heavy<kEdge, kShapeWhole, kRecord>
-      <<<heavy_grid<kEdge, kShapeWhole>(b.cu_count), heavy_threads(kEdge, kShapeWhole), 0,
-         aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
-                   b.partial_depth, b.color_out, b.depth_out, b.record);
+  launch_heavy<kEdge, kShapeWhole, kRecord>(ra, b0.cu_count, aux);
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 1], aux);
   if constexpr (kEdge == 16) {
-    k_raster_heavy<kEdge, kShapeSlab8, kRecord>
-        <<<heavy_grid<kEdge, kShapeSlab8>(b.cu_count), heavy_threads(kEdge, kShapeSlab8), 0,
-           aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
-                     b.partial_depth, b.color_out, b.depth_out, b.record);
+    launch_heavy<kEdge, kShapeSlab8, kRecord>(ra, b0.cu_count, aux);
   }
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], aux);
   if constexpr (kEdge == 16) {
-    k_raster_heavy<kEdge, kShapeSlab4, kRecord>
-        <<<heavy_grid<kEdge, kShapeSlab4>(b.cu_count), heavy_threads(kEdge, kShapeSlab4), 0,
-           aux>>>(fp, st, sorted, b.tile_offsets, b.proj.splat, b.tasks, b.chunk_partials,
-                     b.partial_depth, b.color_out, b.depth_out, b.record);
+    launch_heavy<kEdge, kShapeSlab4, kRecord>(ra, b0.cu_count, aux);
   }
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], aux);
   if (rs.aux) {
     e = hipEventRecord(rs.join, rs.aux);
     if (e != hipSuccess) return e;
   }
-  uint32_t light_blocks = b.max_tiles < kLightBlocks ? b.max_tiles : kLightBlocks;
-  k_raster_light<kEdge, kBatch, kRecord><<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(
-      fp, st, sorted, b.tile_offsets, b.proj.splat, b.color_out, b.depth_out, b.record);
+  uint32_t light_blocks = max_tiles < kLightBlocks ? max_tiles : kLightBlocks;
+  if (n == 1)
+    k_raster_light<kEdge, kBatch, kRecord, 1>
+        <<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(first_view(ra));
+  else
+    k_raster_light<kEdge, kBatch, kRecord, kGroupMax>
+        <<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(ra);
   if (rs.aux) {
     e = hipStreamWaitEvent(s, rs.join, 0);
     if (e != hipSuccess) return e;
   }
-  if (fp.chunk_size != 0)
-    k_composite<kEdge><<<b.max_tiles, kEdge * kEdge, 0, s>>>(
-        fp, st, b.tile_offsets, b.chunk_first, b.chunk_partials, b.partial_depth, b.color_out,
-        b.depth_out);
+  if (b0.params.chunk_size != 0) {
+    ra.bpv = max_tiles;
+    if (n == 1)
+      k_composite<kEdge, 1><<<max_tiles, kEdge * kEdge, 0, s>>>(first_view(ra));
+    else
+      k_composite<kEdge, kGroupMax><<<n * max_tiles, kEdge * kEdge, 0, s>>>(ra);
+  }
   return hipGetLastError();
 }
 
 // The instrumented raster (contribution record) uses the 256-record batch.
 template <int kEdge>
-hipError_t launch_raster_edge(const FrameBuffers& b, const uint2* sorted, hipStream_t s,
+hipError_t launch_raster_edge(const FrameBuffers* views, uint32_t n, hipStream_t s,
                               const RasterStreams& rs, hipEvent_t* marks) {
-  if (b.record) return launch_raster<kEdge, 256, true>(b, sorted, s, rs, marks);
-  if (b.raster_batch == 128) return launch_raster<kEdge, 128, false>(b, sorted, s, rs, marks);
-  return launch_raster<kEdge, 256, false>(b, sorted, s, rs, marks);
+  const FrameBuffers& b = views[0];
+  if (b.record) return launch_raster<kEdge, 256, true>(views, n, s, rs, marks);
+  if (b.raster_batch == 128) return launch_raster<kEdge, 128, false>(views, n, s, rs, marks);
+  return launch_raster<kEdge, 256, false>(views, n, s, rs, marks);
 }
 
 }  // namespace
 
-
-hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStreams& rs,
-                        hipEvent_t* marks) {
+hipError_t launch_group(const FrameBuffers* views, uint32_t n, hipStream_t s,
+                        const RasterStreams& rs, hipEvent_t* marks) {
   auto mark = [&](int stage) {
     if (marks) (void)hipEventRecord(marks[stage], s);
   };
-  const FrameParams& fp = b.params;
-  DeviceState* st = b.state;
-  uint32_t n_blocks = div_up(b.max_n, kBlock);
+  if (n == 0 || n > (uint32_t)kGroupMax) return hipErrorInvalidValue;
+  // The views of a group share the launch shape, the mode (plain, cull,
+  // collect), the scene and the sort pass counts.
+  const FrameBuffers& b0 = views[0];
+  uint32_t max_n = 0, max_k = 0, max_tiles = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    const FrameBuffers& b = views[v];
+    if (b.params.tile_size != b0.params.tile_size || b.radix_bits != b0.radix_bits ||
+        b.raster_batch != b0.raster_batch || (b.record != nullptr) != (b0.record != nullptr) ||
+        (b.cull_net != nullptr) != (b0.cull_net != nullptr) ||
+        b.depth_passes != b0.depth_passes || b.tile_passes != b0.tile_passes ||
+        b.params.chunk_size != b0.params.chunk_size)
+      return hipErrorInvalidValue;
+    max_n = std::max(max_n, b.max_n);
+    max_k = std::max(max_k, b.max_k);
+    max_tiles = std::max(max_tiles, b.max_tiles);
+  }
+  uint32_t n_blocks = div_up(max_n, kBlock);
   if (n_blocks == 0) n_blocks = 1;
+  uint32_t scan_blocks = div_up(max_n, kScanPartition);
+  if (scan_blocks == 0) scan_blocks = 1;
   mark(kStageProject);
-  if (b.cull_net)
-    k_project_cull<<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, b.cull_net);
-  else if (b.record)
-    k_project<kModeCollect><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, b.cull_feats,
-                                                        b.record);
-  else
-    k_project<kModePlain><<<n_blocks, kBlock, 0, s>>>(fp, st, b.scene, b.proj, nullptr,
-                                                      nullptr);
+  {
+    ViewArgs<ProjectArgs> grp = view_args<ProjectArgs>(n, n_blocks);
+    for (uint32_t v = 0; v < n; v++) {
+      const FrameBuffers& b = views[v];
+      grp.v[v] = {b.params, b.state, b.scene, b.proj, b.cull_feats, b.record, b.cull_net};
+    }
+    if (n == 1) {
+      if (b0.cull_net)
+        k_project_cull<1><<<n_blocks, kBlock, 0, s>>>(first_view(grp));
+      else if (b0.record)
+        k_project<kModeCollect, 1><<<n_blocks, kBlock, 0, s>>>(first_view(grp));
+      else
+        k_project<kModePlain, 1><<<n_blocks, kBlock, 0, s>>>(first_view(grp));
+    } else {
+      if (b0.cull_net)
+        k_project_cull<kGroupMax><<<n * n_blocks, kBlock, 0, s>>>(grp);
+      else if (b0.record)
+        k_project<kModeCollect, kGroupMax><<<n * n_blocks, kBlock, 0, s>>>(grp);
+      else
+        k_project<kModePlain, kGroupMax><<<n * n_blocks, kBlock, 0, s>>>(grp);
+    }
+  }
   mark(kStageVisible);
 
   // Visible flags -> exclusive offsets -> compacted (depth key, ordinal).
-  LenSpec active_len = {nullptr, 0u, 1u, fp.active_count};
-  uint32_t scan_blocks = div_up(b.max_n, kScanPartition);
-  if (scan_blocks == 0) scan_blocks = 1;
-  ScanWorkspace sws = {reinterpret_cast<unsigned long long*>(b.partials)};
-  StoreCompact sc{b.proj.mean, fp.depth_mode, fp.camera, b.visible_offsets, b.depth_a};
-  k_scan<LoadVisible, StoreCompact><<<scan_blocks, kBlock, 0, s>>>(
-      LoadVisible{b.proj.mean}, sc, active_len, sws, kScanVisible, 0u, fp.camera, st);
+  {
+    ViewArgs<ScanArgs<LoadVisible, StoreCompact>> grp =
+        view_args<ScanArgs<LoadVisible, StoreCompact>>(n, scan_blocks);
+    for (uint32_t v = 0; v < n; v++) {
+      const FrameBuffers& b = views[v];
+      const FrameParams& fp = b.params;
+      grp.v[v] = {LoadVisible{b.proj.mean},
+                StoreCompact{b.proj.mean, fp.depth_mode, fp.camera, b.visible_offsets, b.depth_a},
+                LenSpec{nullptr, 0u, 1u, fp.active_count},
+                ScanWorkspace{reinterpret_cast<unsigned long long*>(b.partials)},
+                kScanVisible,
+                0u,
+                fp.camera,
+                b.state};
+    }
+    if (n == 1)
+      k_scan<LoadVisible, StoreCompact, 1><<<scan_blocks, kBlock, 0, s>>>(first_view(grp));
+    else
+      k_scan<LoadVisible, StoreCompact, kGroupMax><<<n * scan_blocks, kBlock, 0, s>>>(grp);
+  }
 
   // Depth sort over V.
   mark(kStageDepthSort);
-  uint32_t* depth_totals = b.hist;
-  uint32_t* tile_totals = b.hist + kMaxPasses * kHistShards * 256;
-  SortWorkspace ws = {depth_totals, tile_totals, b.hist + kSortTotalsWords, b.sort_parts,
-                      sws.status, scan_blocks};
-  radix_sort(b.depth_a, b.depth_b, &st->visible, b.max_n, b.depth_passes, b.radix_bits, ws, st,
-             &st->depth_in_place, s);
-  const uint2* depth_out = sorted_depth_records(b, 0u), *depth_in = sorted_depth_records(b, 1u);
+  SortView sv[kGroupMax];
+  for (uint32_t v = 0; v < n; v++) {
+    const FrameBuffers& b = views[v];
+    DeviceState* st = b.state;
+    uint32_t* depth_totals = b.hist;
+    uint32_t* tile_totals = b.hist + kMaxPasses * kHistShards * 256;
+    uint32_t view_scan_blocks = div_up(b.max_n, kScanPartition);
+    sv[v] = {b.depth_a,
+             b.depth_b,
+             &st->visible,
+             {depth_totals, tile_totals, b.hist + kSortTotalsWords, b.sort_parts,
+              reinterpret_cast<unsigned long long*>(b.partials),
+              view_scan_blocks ? view_scan_blocks : 1u},
+             st,
+             &st->depth_in_place};
+  }
+  radix_sort(sv, n, max_n, b0.depth_passes, b0.radix_bits, s);
 
   // Depth-ordered tile counts -> write offsets; K, overflow, output state.
   mark(kStageScan);
-  LenSpec vis_len = {&st->visible, 0u, 1u, 0u};
-  k_scan<LoadOrdered, StoreOrdered><<<scan_blocks, kBlock, 0, s>>>(
-      LoadOrdered{depth_out, depth_in, &st->depth_in_place, b.proj.hits, b.ordered_hits},
-      StoreOrdered{b.ordered_counts, b.intersection_offsets}, vis_len, sws, kScanIntersections,
-      fp.capacity, fp.camera, st);
+  {
+    ViewArgs<ScanArgs<LoadOrdered, StoreOrdered>> grp =
+        view_args<ScanArgs<LoadOrdered, StoreOrdered>>(n, scan_blocks);
+    for (uint32_t v = 0; v < n; v++) {
+      const FrameBuffers& b = views[v];
+      DeviceState* st = b.state;
+      grp.v[v] = {LoadOrdered{sorted_depth_records(b, 0u), sorted_depth_records(b, 1u),
+                            &st->depth_in_place, b.proj.hits, b.ordered_hits},
+                StoreOrdered{b.ordered_counts, b.intersection_offsets},
+                LenSpec{&st->visible, 0u, 1u, 0u},
+                ScanWorkspace{reinterpret_cast<unsigned long long*>(b.partials)},
+                kScanIntersections,
+                b.params.capacity,
+                b.params.camera,
+                st};
+    }
+    if (n == 1)
+      k_scan<LoadOrdered, StoreOrdered, 1><<<scan_blocks, kBlock, 0, s>>>(first_view(grp));
+    else
+      k_scan<LoadOrdered, StoreOrdered, kGroupMax><<<n * scan_blocks, kBlock, 0, s>>>(grp);
+  }
 
   mark(kStageEmit);
-  k_emit<<<n_blocks, kBlock, 0, s>>>(fp, st, depth_out, depth_in, b.proj.splat, b.ordered_hits,
-                                     b.intersection_offsets, b.records_a);
+  {
+    ViewArgs<EmitArgs> grp = view_args<EmitArgs>(n, n_blocks);
+    for (uint32_t v = 0; v < n; v++) {
+      const FrameBuffers& b = views[v];
+      grp.v[v] = {b.params,         b.state,          sorted_depth_records(b, 0u),
+                sorted_depth_records(b, 1u), b.proj.splat, b.ordered_hits,
+                b.intersection_offsets, b.records_a};
+    }
+    if (n == 1)
+      k_emit<1><<<n_blocks, kBlock, 0, s>>>(first_view(grp));
+    else
+      k_emit<kGroupMax><<<n * n_blocks, kBlock, 0, s>>>(grp);
+  }
 
   mark(kStageTileSort);
-  ws.ghist = tile_totals;
-  ws.ghist_other = depth_totals;
-  radix_sort(b.records_a, b.records_b, &st->count, b.max_k, b.tile_passes, b.radix_bits, ws, st,
-             nullptr, s);
-  const uint2* sorted = sorted_tile_records(b);
+  for (uint32_t v = 0; v < n; v++) {
+    const FrameBuffers& b = views[v];
+    std::swap(sv[v].ws.ghist, sv[v].ws.ghist_other);
+    sv[v].a = b.records_a;
+    sv[v].b = b.records_b;
+    sv[v].count_ptr = &b.state->count;
+    sv[v].in_place = nullptr;
+  }
+  radix_sort(sv, n, max_k, b0.tile_passes, b0.radix_bits, s);
 
   mark(kStageTileOffsets);
-  uint32_t off_blocks = div_up((uint64_t)b.max_k + 1, kBlock);
-  if (off_blocks > 2048) off_blocks = 2048;
-  k_tile_offsets<<<off_blocks, kBlock, 0, s>>>(fp, st, sorted, b.tile_offsets);
+  {
+    uint32_t off_blocks = div_up((uint64_t)max_k + 1, kBlock);
+    if (off_blocks > 2048) off_blocks = 2048;
+    ViewArgs<TileOffsetArgs> grp = view_args<TileOffsetArgs>(n, off_blocks);
+    for (uint32_t v = 0; v < n; v++) {
+      const FrameBuffers& b = views[v];
+      grp.v[v] = {b.params, b.state, sorted_tile_records(b), b.tile_offsets};
+    }
+    if (n == 1)
+      k_tile_offsets<1><<<off_blocks, kBlock, 0, s>>>(first_view(grp));
+    else
+      k_tile_offsets<kGroupMax><<<n * off_blocks, kBlock, 0, s>>>(grp);
+  }
   mark(kStageRaster);
   hipError_t e;
-  if (fp.tile_size == 8)
-    e = launch_raster_edge<8>(b, sorted, s, rs, marks);
-  else if (fp.tile_size == 32)
-    e = launch_raster_edge<32>(b, sorted, s, rs, marks);
+  if (b0.params.tile_size == 8)
+    e = launch_raster_edge<8>(views, n, s, rs, marks);
+  else if (b0.params.tile_size == 32)
+    e = launch_raster_edge<32>(views, n, s, rs, marks);
   else
-    e = launch_raster_edge<16>(b, sorted, s, rs, marks);
+    e = launch_raster_edge<16>(views, n, s, rs, marks);
   if (e != hipSuccess) return e;
   mark(kStageCount);
   return hipGetLastError();
+}
+
+hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStreams& rs,
+                        hipEvent_t* marks) {
+  return launch_group(&b, 1, s, rs, marks);
 }
 
 const uint2* sorted_depth_records(const FrameBuffers& b, uint32_t depth_in_place) {
@@ -2454,7 +2786,7 @@ const uint2* sorted_depth_records(const FrameBuffers& b, uint32_t depth_in_place
   return (filled % 2 == 0) ? b.depth_a : b.depth_b;
 }
 const uint2* sorted_tile_records(const FrameBuffers& b) {
-  return (b.tile_passes % 2 == 0) ? b.records_a : b.records_b;
+  return tile_records_after(b, b.tile_passes);
 }
 
 }  // namespace gs

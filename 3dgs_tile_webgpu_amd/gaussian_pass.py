@@ -57,7 +57,7 @@ ACCEPTED_OPTIONS = {"redrawStrategy", "radixBackend", "dataMode", "colorSpace", 
 KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | ACCEPTED_OPTIONS | {
     "fixedSize", "depthRange", "intersectionCapacity", "initialIntersectionCapacity",
     "maxIntersectionCapacity", "device", "batchLanes", "heavySplitUnits",
-    "heavyFineSplitUnits",
+    "heavyFineSplitUnits", "viewGroup",
 }
 MAX_CAPACITY = 1 << 28
 
@@ -171,6 +171,9 @@ def validate_options(options: dict) -> dict:
     lanes = o.get("batchLanes", 1)
     if lanes != 1:
         raise N.GsRangeError(N.GS_E_RANGE, "batch lanes above 1 are disabled")
+    group = o.get("viewGroup", 1)
+    if not isinstance(group, int) or isinstance(group, bool) or not 1 <= group <= 8:
+        raise N.GsRangeError(N.GS_E_RANGE, "viewGroup must be an integer in [1, 8]")
     for name in ("heavySplitUnits", "heavyFineSplitUnits"):
         split = o.get(name)
         if split is not None and (not isinstance(split, int) or isinstance(split, bool)
@@ -204,6 +207,7 @@ def config_for(o: dict, policy: IntersectionCapacityPolicy) -> N.GsConfig:
     cfg.heavy_split_units = 0xFFFFFFFF if split is None else int(split)
     fine = o.get("heavyFineSplitUnits")
     cfg.heavy_fine_split_units = 0xFFFFFFFF if fine is None else int(fine)
+    cfg.view_group = int(o.get("viewGroup", 1))
     return cfg
 
 

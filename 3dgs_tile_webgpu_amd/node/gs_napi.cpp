@@ -153,6 +153,7 @@ void read_config(napi_env env, napi_value o, gs_config* c) {
   c->heavy_split_units = (uint32_t)num(env, o, "heavySplitUnits", c->heavy_split_units);
   c->heavy_fine_split_units =
       (uint32_t)num(env, o, "heavyFineSplitUnits", c->heavy_fine_split_units);
+  c->view_group = (uint32_t)num(env, o, "viewGroup", c->view_group);
   napi_value bg;
   if (get_prop(env, o, "background", &bg)) {
     for (uint32_t i = 0; i < 4; i++) {

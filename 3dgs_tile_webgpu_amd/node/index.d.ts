@@ -45,6 +45,9 @@ export interface GaussianPassOptions {
   /** Heavy raster units above which those slabs have 4 rows instead of 8
    * (default: 16 per compute unit); never changes any output. */
   heavyFineSplitUnits?: number;
+  /** Views of a batch rendered by each launch, 1..8 (default 1): every
+   * stage is one kernel over the group's views; never changes any output. */
+  viewGroup?: number;
   device?: number;
 }
 

@@ -44,7 +44,7 @@ const ACCEPTED = ['redrawStrategy', 'radixBackend', 'dataMode', 'colorSpace', 'p
   'intersectionReadbackInterval'];
 const KNOWN = new Set([...Object.keys(DEFAULTS), ...ACCEPTED, 'fixedSize', 'depthRange',
   'intersectionCapacity', 'initialIntersectionCapacity', 'maxIntersectionCapacity', 'device',
-  'heavySplitUnits', 'heavyFineSplitUnits']);
+  'heavySplitUnits', 'heavyFineSplitUnits', 'viewGroup']);
 const BOUNDARY = { clip: 0, 'repeat-x': 1, 'repeat-y': 2 };
 const MAX_CAPACITY = 2 ** 28;
 const RESOURCE_NAMES = ['projected_mean', 'projected_conic', 'projected_color', 'visible_offsets',
@@ -145,6 +145,9 @@ function validateOptions(options) {
   if (![8, 16, 32].includes(o.tileSize) || ![4, 6, 8].includes(o.radixBits) || ![128, 256].includes(o.rasterBatch)) {
     throw new RangeError('tileSize must be 8, 16 or 32, radixBits 4, 6 or 8, rasterBatch 128 or 256');
   }
+  if (o.viewGroup !== undefined && !(Number.isInteger(o.viewGroup) && o.viewGroup >= 1 && o.viewGroup <= 8)) {
+    throw new RangeError('viewGroup must be an integer in [1, 8]');
+  }
   for (const name of ['heavySplitUnits', 'heavyFineSplitUnits']) {
     const v = o[name];
     if (v !== undefined && !(Number.isInteger(v) && v >= 0 && v < 0xffffffff)) {
@@ -174,6 +177,7 @@ function nativeConfig(o, policy) {
     rasterBatch: o.rasterBatch,
     heavySplitUnits: o.heavySplitUnits === undefined ? 0xffffffff : o.heavySplitUnits,
     heavyFineSplitUnits: o.heavyFineSplitUnits === undefined ? 0xffffffff : o.heavyFineSplitUnits,
+    viewGroup: dflt(o.viewGroup, 1),
   };
 }
 

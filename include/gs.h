@@ -92,6 +92,12 @@ typedef struct gs_config {
                                         slabs have 4 rows instead of 8;
                                         GS_SPLIT_AUTO (default) = 16 per compute
                                         unit; never changes any output */
+  uint32_t view_group;               /* views of a batch rendered by each
+                                        launch (1..8, default 1): every stage
+                                        is one kernel over the group's views,
+                                        largest views grouped first; each view
+                                        of a group has its own working set.
+                                        Never changes any output */
 } gs_config;
 
 #define GS_SPLIT_AUTO 0xffffffffu

@@ -85,10 +85,45 @@ def test_batch_matches_single_passes(gs, oracle, gpu_available, small, lanes):
     b.dispose()
 
 
-def test_batch_replay_follows_camera_motion(gs, gpu_available, small):
+GROUPS = [
+    (4, {}),
+    (3, {"tileSize": 8, "radixBits": 4, "rasterChunkSize": 256}),
+    (8, {"rasterChunkSize": 1024, "heavySplitUnits": 0}),
+]
+
+
+@pytest.mark.parametrize("group,options", GROUPS,
+                         ids=[f"group{g}-" + ",".join(f"{k}={v}" for k, v in o.items())
+                              for g, o in GROUPS])
+def test_view_groups_match_single_passes(gs, oracle, gpu_available, small, group, options):
+    """Views rendered in groups (each stage one launch over a group's views)
+    reproduce one gs_render per camera bit for bit: six views of five
+    resolutions, growing from a small capacity, with groups of 4 (+2), 3 and
+    all six; at tile 8 with 4-bit digits the views' tile sorts have 3 and 4
+    passes (a group runs the larger count); chunk tasks and row slabs."""
+    cams, sizes = batch_cameras(gs, 6, seed=21)
+    b = gs.BatchedPass(small, {"viewGroup": group, "initialIntersectionCapacity": 4096,
+                               **options})
+    b.render(cams, sizes)
+    states = b.read_states()
+    assert all(s.overflow == 0 and s.error_flags == 0 for s in states)
+    want = single_images(gs, small, cams, sizes, options)
+    for v in range(len(cams)):
+        np.testing.assert_array_equal(b.read_image(v).view(np.uint16), want[v].view(np.uint16),
+                                      err_msg=f"view {v}")
+    w, h = sizes[1]
+    p, ref = render_both(gs, oracle, small, cams[1], w, h, options)
+    np.testing.assert_array_equal(b.read_image(1).view(np.uint16), ref.get("image_f16"))
+    assert states[1].intersections == ref.counts.count
+    p.dispose()
+    b.dispose()
+
+
+@pytest.mark.parametrize("group", [1, 4])
+def test_batch_replay_follows_camera_motion(gs, gpu_available, small, group):
     """The captured batch graph is replayed with new camera uniforms."""
     cams, sizes = batch_cameras(gs, 5, seed=3)
-    b = gs.BatchedPass(small, {"initialIntersectionCapacity": 1 << 21})
+    b = gs.BatchedPass(small, {"initialIntersectionCapacity": 1 << 21, "viewGroup": group})
     b.render(cams, sizes)
     first = [b.read_image(v).copy() for v in range(5)]
     moved, _ = batch_cameras(gs, 5, seed=4)
@@ -122,10 +157,11 @@ def test_autotune_picks_a_shape_and_stays_exact(gs, oracle, gpu_available, small
 def test_c4_batch_at_configured_size(gs, oracle, gpu_available):
     """The batched workload at its configured size: 2M Gaussians, 64 seeded
     orbit cameras (radius 12-25) at resolutions drawn from 854x480,
-    1280x720, 1920x1080 and 2560x1440, in one gs_render_batch. Every view
-    ends without overflow or wait errors, publishes the same key count as a
-    single-camera pass of it and the same image bits; one view of each
-    resolution is checked bit for bit against the oracle."""
+    1280x720, 1920x1080 and 2560x1440, in one gs_render_batch, one view per
+    launch and in view groups of 4. Every view ends without overflow or
+    wait errors, publishes the same key count as a single-camera pass of it
+    and the same image bits; one view of each resolution is checked bit for
+    bit against the oracle."""
     w = gs.WORKLOADS["2m_batch"]
     scene = gs.workload_scene("2m_batch")
     views = gs.orbit_cameras(w.views, seed=777, radius=w.orbit, resolutions=gs.BATCH_RESOLUTIONS)
@@ -135,6 +171,10 @@ def test_c4_batch_at_configured_size(gs, oracle, gpu_available):
     b.render(cams, sizes)
     states = b.read_states()
     assert all(s.overflow == 0 and s.error_flags == 0 for s in states)
+    grouped = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22, "viewGroup": 4})
+    grouped.render(cams, sizes)
+    gstates = grouped.read_states()
+    assert all(s.overflow == 0 and s.error_flags == 0 for s in gstates)
     p = gs.gaussian_pass(cams[0], scene, {"intersectionCapacity": 1 << 23})
     checked = {}
     for v, (cam, (x, y)) in enumerate(views):
@@ -145,6 +185,9 @@ def test_c4_batch_at_configured_size(gs, oracle, gpu_available):
         assert (states[v].visible, states[v].intersections) == (st.visible, st.intersections), v
         np.testing.assert_array_equal(b.read_image(v).view(np.uint16),
                                       p.read_image().view(np.uint16), err_msg=f"view {v}")
+        assert (gstates[v].visible, gstates[v].intersections) == (st.visible, st.intersections)
+        np.testing.assert_array_equal(grouped.read_image(v).view(np.uint16),
+                                      p.read_image().view(np.uint16), err_msg=f"grouped view {v}")
         if (x, y) not in checked:
             checked[(x, y)] = v
             frame = gs.frame_for(cam, x, y, p.options)
@@ -158,3 +201,4 @@ def test_c4_batch_at_configured_size(gs, oracle, gpu_available):
     assert len(checked) == 4
     p.dispose()
     b.dispose()
+    grouped.dispose()

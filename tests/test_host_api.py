@@ -73,6 +73,9 @@ def test_capacity_rule_c_matches_python(gs):
     ({"radixBits": 5}, ValueError),
     ({"rasterBatch": 512}, ValueError),
     ({"batchLanes": 4}, ValueError),
+    ({"viewGroup": 0}, ValueError),
+    ({"viewGroup": 9}, ValueError),
+    ({"viewGroup": 2.5}, ValueError),
 ])
 def test_option_validation(gs, opts, err):
     with pytest.raises(err):

@@ -83,6 +83,11 @@ CASES = [
     {"heavyFineSplitUnits": 4096},
     {"heavyFineSplitUnits": -2},
     {"heavyFineSplitUnits": 0xFFFFFFFF},
+    {"viewGroup": 1},
+    {"viewGroup": 8},
+    {"viewGroup": 0},
+    {"viewGroup": 9},
+    {"viewGroup": 2.5},
 ]
 
 
