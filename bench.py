@@ -149,14 +149,19 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# Views per launch in the batched measurements (the largest view group).
+VIEW_GROUP = 8
+
+
 def batched(gs, reps=10):
     """64 orbit cameras (radius 12-25, resolutions drawn from 854x480 ..
-    2560x1440) over the 2M-Gaussian scene, one gs_render_batch per step."""
+    2560x1440) over the 2M-Gaussian scene, one gs_render_batch per step,
+    VIEW_GROUP views per launch; also timed with one view per launch."""
     w = gs.WORKLOADS["2m_batch"]
     scene = gs.workload_scene("2m_batch")
     views = gs.orbit_cameras(w.views, seed=777, radius=w.orbit, resolutions=gs.BATCH_RESOLUTIONS)
     cams, sizes = [c for c, _ in views], [s for _, s in views]
-    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22})
+    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22, "viewGroup": VIEW_GROUP})
     progress("batched: autotune")
     t0 = time.perf_counter()
     chosen, rows = b.autotune(cams, sizes, reps=2)
@@ -171,10 +176,25 @@ def batched(gs, reps=10):
     st = b.read_states()
     elapsed = time.perf_counter() - t0
     pixels = sum(x * y for x, y in sizes)
+    b.dispose()
+    # The same batch and launch shape, one view per launch.
+    one = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22, **chosen})
+    one.render(cams, sizes)
+    for _ in range(2):
+        one.launch(one._frames)
+    one.read_states()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one.launch(one._frames)
+    one.read_states()
+    one_elapsed = time.perf_counter() - t0
+    one.dispose()
     out = {
         "workload": w.name, "gaussians": w.count, "views": len(cams),
+        "view_group": VIEW_GROUP,
         "frames_per_s": round(len(cams) * reps / elapsed, 1),
         "ms_per_batch": round(elapsed * 1e3 / reps, 3),
+        "frames_per_s_one_view_per_launch": round(len(cams) * reps / one_elapsed, 1),
         "megapixels_per_s": round(pixels * reps / elapsed / 1e6, 1),
         "keys_per_batch": int(sum(x.intersections for x in st)),
         "tuning": chosen,
@@ -182,7 +202,6 @@ def batched(gs, reps=10):
                                   round(r["ms"], 3) for r in rows},
         "autotune_seconds": round(tune_s, 2),
     }
-    b.dispose()
     return out
 
 
@@ -197,7 +216,7 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     train = gs.orbit_cameras(train_views, seed=5001, radius=w.orbit, resolutions=res)
     held = gs.orbit_cameras(eval_views, seed=5002, radius=w.orbit, resolutions=res)
     valid = gs.orbit_cameras(calib_views, seed=5003, radius=w.orbit, resolutions=res)
-    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 24})
+    b = gs.BatchedPass(scene, {"initialIntersectionCapacity": 1 << 22, "viewGroup": VIEW_GROUP})
     cull = gs.LearnedCull(b, seed=3)
     cull.reserve(100_000_000)
     progress("learned cull: collecting training frames")
@@ -269,6 +288,7 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     return {
         "workload": w.name, "gaussians": w.count, "resolution": f"{w.width}x{w.height}",
         "train_views": train_views, "eval_views": eval_views, "calibration_views": calib_views, "threshold": 1e-4,
+        "view_group": VIEW_GROUP,
         "fps_plain_batched": round(len(cams) / t_plain, 1),
         "fps_cull_batched": round(len(cams) / t_cull, 1),
         "speedup": round(t_plain / t_cull, 3),
