@@ -123,8 +123,8 @@ def cpu_baseline(gs, scene, camera, workload, threads):
 # heavy raster instance that workload's frames run (whole tiles at 1M/1080p,
 # 4-row slabs at 6M/4K).
 TRAFFIC_PROFILES = {
-    "1m_1080p": ("profiles/frame_1m_1080p_hbm_traffic.md", "gs::k_raster_heavy<16, 0, false>"),
-    "6m_4k": ("profiles/frame_6m_4k_hbm_traffic.md", "gs::k_raster_heavy<16, 2, false>"),
+    "1m_1080p": ("profiles/frame_1m_1080p_hbm_traffic.md", "gs::k_raster_heavy<16, 0, false, 1>"),
+    "6m_4k": ("profiles/frame_6m_4k_hbm_traffic.md", "gs::k_raster_heavy<16, 2, false, 1>"),
 }
 
 
