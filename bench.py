@@ -228,9 +228,11 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     progress(f"learned cull: training on {n} samples")
     t0 = time.perf_counter()
     cull.fit_normalization()
-    cull.train(10000, batch=65536, lr=2e-3, under_weight=300.0, seed=1)
-    cull.train(5000, batch=65536, lr=5e-4, under_weight=300.0, seed=2)
-    loss = cull.train(5000, batch=65536, lr=1e-4, under_weight=300.0, seed=3)
+    # 40,000 Adam steps (3 s): against 20,000, 71-72 % of the keys removed at
+    # the calibrated PSNR instead of 68-71 %.
+    cull.train(20000, batch=65536, lr=2e-3, under_weight=300.0, seed=1)
+    cull.train(10000, batch=65536, lr=5e-4, under_weight=300.0, seed=2)
+    loss = cull.train(10000, batch=65536, lr=1e-4, under_weight=300.0, seed=3)
     train_s = time.perf_counter() - t0
     progress("learned cull: calibrating and evaluating")
     shift, rows = cull.calibrate([c for c, _ in valid], [s for _, s in valid],
