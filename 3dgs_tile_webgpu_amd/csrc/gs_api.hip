@@ -111,6 +111,7 @@ struct gs_ctx {
   uint32_t count = 0, active_count = 0, sh_degree = 0;
   bool has_data = false, has_active = false;
   DeviceBuffer means, scales_opacity, rotations, sh, active;
+  DeviceBuffer sh_top;  // per slot: the higher SH bands' largest exponent byte
   // Object frame table set by gs_set_objects (host copy).
   std::vector<float> objects_host;
   uint32_t object_count = 0;
@@ -461,7 +462,7 @@ extern "C" void gs_destroy(gs_ctx* ctx) {
   (void)hipSetDevice(ctx->cfg.device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DeviceBuffer* all[] = {&ctx->means, &ctx->scales_opacity, &ctx->rotations, &ctx->sh,
-                         &ctx->active};
+                         &ctx->active, &ctx->sh_top};
   for (DeviceBuffer* b : all) release(*b);
   set_release(ctx->single);
   set_release(ctx->batch);
@@ -508,8 +509,12 @@ extern "C" int gs_set_data(gs_ctx* ctx, const gs_data_desc* d) {
       (rc = copy_in(ctx, ctx->scales_opacity, d->scales_opacity, n * 8, d->memory,
                     "scalesOpacity")) ||
       (rc = copy_in(ctx, ctx->rotations, d->rotations, n * 4, d->memory, "rotations")) ||
-      (rc = copy_in(ctx, ctx->sh, d->sh, n * ncoef * 4, d->memory, "shCoefficients")))
+      (rc = copy_in(ctx, ctx->sh, d->sh, n * ncoef * 4, d->memory, "shCoefficients")) ||
+      (rc = ensure(ctx, ctx->sh_top, n, "sh-exponents")))
     return rc;
+  hipError_t se = gs::launch_sh_top((const uint32_t*)ctx->sh.ptr, (uint32_t)n, d->sh_degree,
+                                    (uint8_t*)ctx->sh_top.ptr, ctx->stream);
+  if (se != hipSuccess) return hip_fail(ctx, se, "sh exponents");
   if (d->active_slots && d->memory == GS_MEMORY_HOST)
     for (uint32_t i = 0; i < d->active_count; i++)
       if (d->active_slots[i] >= d->count)
@@ -832,6 +837,7 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.scene.scales_opacity = (const uint2*)ctx->scales_opacity.ptr;
   b.scene.rotations = (const uint32_t*)ctx->rotations.ptr;
   b.scene.sh = (const uint32_t*)ctx->sh.ptr;
+  b.scene.sh_top = (const uint8_t*)ctx->sh_top.ptr;
   b.scene.active_slots = (const uint32_t*)ctx->active.ptr;
   b.scene.objects = (const float*)(dyn + gs::kCameraBytes);
   b.proj.mean = (float4*)l.pmean.ptr;

@@ -100,6 +100,9 @@ struct SceneView {
   const uint32_t* sh;
   const uint32_t* active_slots;
   const float* objects;
+  // Per slot, the largest rgb8e8 exponent byte of the higher SH bands (a
+  // learned-cull feature; static with the scene), or null to read the SH.
+  const uint8_t* sh_top;
 };
 
 // A visible Gaussian's projection in one 64-byte line, so that the raster's
@@ -218,6 +221,9 @@ hipError_t launch_frame(const FrameBuffers& b, hipStream_t stream, const RasterS
 // the mode (plain, cull, collect), the scene and their sort pass counts
 // (hipErrorInvalidValue otherwise); each has its own working buffers,
 // state and outputs. launch_frame is the group of one.
+// The per-slot SH exponent bytes of SceneView::sh_top for n slots.
+hipError_t launch_sh_top(const uint32_t* sh, uint32_t n, uint32_t sh_degree, uint8_t* out,
+                         hipStream_t stream);
 hipError_t launch_group(const FrameBuffers* views, uint32_t n, hipStream_t stream,
                         const RasterStreams& rs, hipEvent_t* marks);
 // Location of the sorted depth / tile records after launch_frame

@@ -433,7 +433,34 @@ GS_DEV ProjOut project_finish_counted(const FrameParams& fp, const SceneView& sc
 // The largest rgb8e8 exponent of the higher SH bands, minus the bias: the
 // log2 scale of the strongest view-dependent colour term (a learned-cull
 // feature; -127 without higher bands).
+GS_DEV uint32_t sh_top_byte(const uint32_t* __restrict__ sh, uint32_t degree, uint32_t slot) {
+  uint32_t ncoef = (degree + 1) * (degree + 1);
+  const uint32_t* shc = sh + (size_t)slot * ncoef;
+  uint32_t top = 0u;
+  if (degree == 3) {
+    const uint4* sh4 = reinterpret_cast<const uint4*>(shc);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      uint4 q4 = sh4[v];
+      if (v > 0) top = max(top, q4.x >> 24);
+      top = max(max(top, q4.y >> 24), max(q4.z >> 24, q4.w >> 24));
+    }
+  } else {
+    for (uint32_t v = 1; v < ncoef; v++) top = max(top, shc[v] >> 24);
+  }
+  return top;
+}
+
+// The scene's exponent bytes, computed once per upload: the feature then
+// reads one byte per Gaussian instead of its 64-byte SH row.
+__global__ __launch_bounds__(kBlock) void k_sh_top(const uint32_t* __restrict__ sh, uint32_t n,
+                                                   uint32_t degree, uint8_t* __restrict__ out) {
+  uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+  if (slot < n) out[slot] = (uint8_t)sh_top_byte(sh, degree, slot);
+}
+
 GS_DEV float sh_top_exponent(const FrameParams& fp, const SceneView& sc, uint32_t slot) {
+  if (sc.sh_top) return (float)(int)sc.sh_top[slot] - 127.0f;
   uint32_t degree = fp.sh_degree;
   uint32_t ncoef = (degree + 1) * (degree + 1);
   const uint32_t* shc = sc.sh + (size_t)slot * ncoef;
@@ -2777,6 +2804,13 @@ hipError_t launch_group(const FrameBuffers* views, uint32_t n, hipStream_t s,
 hipError_t launch_frame(const FrameBuffers& b, hipStream_t s, const RasterStreams& rs,
                         hipEvent_t* marks) {
   return launch_group(&b, 1, s, rs, marks);
+}
+
+hipError_t launch_sh_top(const uint32_t* sh, uint32_t n, uint32_t sh_degree, uint8_t* out,
+                         hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_sh_top<<<div_up(n, kBlock), kBlock, 0, s>>>(sh, n, sh_degree, out);
+  return hipGetLastError();
 }
 
 const uint2* sorted_depth_records(const FrameBuffers& b, uint32_t depth_in_place) {
