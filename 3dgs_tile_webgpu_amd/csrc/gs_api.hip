@@ -1299,14 +1299,15 @@ extern "C" int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], doub
       sums[st] += ms;
     }
     // The heavy-unit kernel: of the three shapes' launches, all but the
-    // frame's exit at once; its time is the longest of their intervals.
+    // frame's exit at once; its time is the longest of their intervals
+    // (each taken between the events around it on its own stream).
     float whole = 0.0f, heavy = 0.0f;
     if (hipEventElapsedTime(&whole, m[0], m[gs::kStageCount]) != hipSuccess)
       return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
     for (int k = 0; k < gs::kMarkHeavyShapes; k++) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, m[gs::kMarkHeavyBegin + k], m[gs::kMarkHeavyBegin + k + 1]) !=
-          hipSuccess)
+      if (hipEventElapsedTime(&ms, m[gs::kMarkHeavyBegin + 2 * k],
+                              m[gs::kMarkHeavyBegin + 2 * k + 1]) != hipSuccess)
         return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
       heavy = ms > heavy ? ms : heavy;
     }

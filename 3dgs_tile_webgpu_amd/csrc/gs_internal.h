@@ -200,9 +200,11 @@ enum Stage : int {
   kStageRaster,
   kStageCount
 };
-constexpr int kMarkHeavyBegin = kStageCount + 1;  // then one after each heavy shape's launch
+// Then a pair (before, after) around each heavy shape's launch, on the
+// stream it runs on.
+constexpr int kMarkHeavyBegin = kStageCount + 1;
 constexpr int kMarkHeavyShapes = 3;
-constexpr int kMarks = kMarkHeavyBegin + kMarkHeavyShapes + 1;
+constexpr int kMarks = kMarkHeavyBegin + 2 * kMarkHeavyShapes;
 
 // The raster's second branch: the heavy-unit kernels run on aux, forked
 // from the frame's stream at `fork` and joined back into it at `join`,

@@ -2580,24 +2580,30 @@ hipError_t launch_raster(const FrameBuffers* views, uint32_t n, hipStream_t s,
     if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
     if (e != hipSuccess) return e;
   }
-  // The whole-tile kernel, then (edge 16) the two row-slab kernels: each
-  // takes the views of its shape, and one whose views all chose another
-  // shape exits at once, before the next can fill the GPU.
+  // The whole-tile kernel on the second branch; (edge 16) the two row-slab
+  // kernels on s, ahead of the light tiles. Each takes the views of its
+  // shape, and one whose views all chose another shape exits at once. So in
+  // whole-tile frames the slab launches' exits delay the light tiles (which
+  // finish before the heavy units) instead of the join, and in slab frames
+  // the whole-tile launch exits beside the slab kernel instead of ahead of
+  // it.
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
   launch_heavy<kEdge, kShapeWhole, kRecord>(ra, b0.cu_count, aux);
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 1], aux);
-  if constexpr (kEdge == 16) {
-    launch_heavy<kEdge, kShapeSlab8, kRecord>(ra, b0.cu_count, aux);
-  }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], aux);
-  if constexpr (kEdge == 16) {
-    launch_heavy<kEdge, kShapeSlab4, kRecord>(ra, b0.cu_count, aux);
-  }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], aux);
   if (rs.aux) {
     e = hipEventRecord(rs.join, rs.aux);
     if (e != hipSuccess) return e;
   }
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], s);
+  if constexpr (kEdge == 16) {
+    launch_heavy<kEdge, kShapeSlab8, kRecord>(ra, b0.cu_count, s);
+  }
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], s);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 4], s);
+  if constexpr (kEdge == 16) {
+    launch_heavy<kEdge, kShapeSlab4, kRecord>(ra, b0.cu_count, s);
+  }
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 5], s);
   uint32_t light_blocks = max_tiles < kLightBlocks ? max_tiles : kLightBlocks;
   if (n == 1)
     k_raster_light<kEdge, kBatch, kRecord, 1>
