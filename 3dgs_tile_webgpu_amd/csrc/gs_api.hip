@@ -212,8 +212,8 @@ int lane_slots(gs_ctx* ctx, Lane& l, uint32_t n) {
   if ((rc = ensure(ctx, l.pmean, c * 16, "projected-mean")) ||
       (rc = ensure(ctx, l.splat, c * sizeof(gs::Splat), "projected-splats")) ||
       (rc = ensure(ctx, l.tile_counts, c * 4, "tile-counts")) ||
-      (rc = ensure(ctx, l.hits, c * 16, "tile-hits")) ||
-      (rc = ensure(ctx, l.ordered_hits, c * 16, "ordered-tile-hits")) ||
+      (rc = ensure(ctx, l.hits, c * 8, "tile-hits")) ||
+      (rc = ensure(ctx, l.ordered_hits, c * 8, "ordered-tile-hits")) ||
       (rc = ensure(ctx, l.visible_offsets, c * 4, "visible-offsets")) ||
       (rc = ensure(ctx, l.depth_a, c * 8, "depth-records-a")) ||
       (rc = ensure(ctx, l.depth_b, c * 8, "depth-records-b")) ||
@@ -843,8 +843,8 @@ void fill_frame(gs_ctx* ctx, ViewSet& set, uint32_t v, const gs_frame* f, const 
   b.proj.mean = (float4*)l.pmean.ptr;
   b.proj.splat = (gs::Splat*)l.splat.ptr;
   b.proj.tile_counts = (uint32_t*)l.tile_counts.ptr;
-  b.proj.hits = (uint4*)l.hits.ptr;
-  b.ordered_hits = (uint4*)l.ordered_hits.ptr;
+  b.proj.hits = (uint2*)l.hits.ptr;
+  b.ordered_hits = (uint2*)l.ordered_hits.ptr;
   b.visible_offsets = (uint32_t*)l.visible_offsets.ptr;
   b.depth_a = (uint2*)l.depth_a.ptr;
   b.depth_b = (uint2*)l.depth_b.ptr;
@@ -1582,8 +1582,8 @@ NoAllocLayout noalloc_layout(const gs_config& c, uint32_t n, uint32_t capacity, 
   L.pmean = take((size_t)n * 16);
   L.splat = take((size_t)n * sizeof(gs::Splat));
   L.tile_counts = take((size_t)n * 4);
-  L.hits = take((size_t)n * 16);
-  L.ordered_hits = take((size_t)n * 16);
+  L.hits = take((size_t)n * 8);
+  L.ordered_hits = take((size_t)n * 8);
   L.visible_offsets = take((size_t)n * 4);
   L.depth_a = take((size_t)n * 8);
   L.depth_b = take((size_t)n * 8);
@@ -1672,8 +1672,8 @@ extern "C" int gs_render_noalloc(const gs_config* config, const gs_data_desc* da
   b.proj.mean = (float4*)(base + L.pmean);
   b.proj.splat = (gs::Splat*)(base + L.splat);
   b.proj.tile_counts = (uint32_t*)(base + L.tile_counts);
-  b.proj.hits = (uint4*)(base + L.hits);
-  b.ordered_hits = (uint4*)(base + L.ordered_hits);
+  b.proj.hits = (uint2*)(base + L.hits);
+  b.ordered_hits = (uint2*)(base + L.ordered_hits);
   b.visible_offsets = (uint32_t*)(base + L.visible_offsets);
   b.depth_a = (uint2*)(base + L.depth_a);
   b.depth_b = (uint2*)(base + L.depth_b);

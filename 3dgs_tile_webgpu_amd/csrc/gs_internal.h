@@ -122,7 +122,7 @@ struct ProjectedView {
   float4* mean;           // per ordinal, every active Gaussian (visibility, depth)
   Splat* splat;           // per ordinal, written for visible Gaussians only
   uint32_t* tile_counts;  // per ordinal
-  uint4* hits;            // per ordinal: tile-hit record (see kHitMask)
+  uint2* hits;            // per ordinal: 8-byte tile-hit record (see kHitMasked)
 };
 
 // Length of a device-sized sequence: (*ptr rounded up to a multiple of
@@ -149,7 +149,7 @@ struct FrameBuffers {
   uint2* depth_a;
   uint2* depth_b;
   uint32_t* ordered_counts;
-  uint4* ordered_hits;  // tile-hit records in depth order
+  uint2* ordered_hits;  // tile-hit records in depth order
   uint32_t* intersection_offsets;
   uint2* records_a;
   uint2* records_b;

@@ -84,16 +84,21 @@ struct ProjOut {
   float thr;  // ln(255 opacity)
   float3 support;  // the raster's support bound terms (support_terms)
   uint32_t count;
-  uint4 hits;  // HitRecord
+  uint2 hits;  // tile-hit record (kHitMasked)
 };
 
-// A visible Gaussian's tile hits, written by the projection and gathered
-// into depth order by the tile-count scan, so that emission reads it in
-// sequence: x = tx0 | ty0 << 16 (bounding-box origin in tiles), y = width |
-// height << 8 | kHitMask (when the box has at most 32 tiles), z = the
-// row-major hit mask over the box, w = the hit count. Without kHitMask the
-// emission tests the tiles again from the projected mean and conic.
-constexpr uint32_t kHitMask = 1u << 16;
+// A visible Gaussian's tile hits, 8 bytes, written by the projection and
+// gathered into depth order by the tile-count scan, so that emission reads
+// it in sequence. A box of at most 32 tiles (in a grid of at most 2^26
+// tiles): x = the row-major hit mask over the box, y = kHitMasked |
+// (width - 1) << 26 | the index of the box's first tile; the hit count is
+// the mask's popcount. Otherwise x = the hit count and y = 0, and emission
+// tests the tiles again from the projected mean and conic.
+constexpr uint32_t kHitMasked = 1u << 31;
+constexpr uint32_t kHitOriginBits = 26;
+GS_DEV uint32_t hit_count(uint2 h) {
+  return (h.y & kHitMasked) ? (uint32_t)__popc(h.x) : h.x;
+}
 
 // Projection modes: the plain path, the learned cull, and feature
 // collection for its training set.
@@ -344,7 +349,7 @@ GS_DEV ProjOut project_finish_counted(const FrameParams& fp, const SceneView& sc
   ProjOut o;
   o.kind = kProjNone;
   o.count = 0u;
-  o.hits = make_uint4(0u, 0u, 0u, 0u);
+  o.hits = make_uint2(0u, 0u);
   if (g.kind != kProjVisible) {
     o.kind = g.kind;
     if (g.kind == kProjSubpixel) o.mean = make_float4(g.cx, g.cy, g.depth, -1.0f);
@@ -355,9 +360,10 @@ GS_DEV ProjOut project_finish_counted(const FrameParams& fp, const SceneView& sc
   const float conic[3] = {g.conic[0], g.conic[1], g.conic[2]};
   if (count == 0) return o;
   const uint32_t bw = box.width(), bh = box.height();
-  const bool small = bw * bh <= kMaskTiles;
-  o.hits = make_uint4((uint32_t)box.tmin_x | ((uint32_t)box.tmin_y << 16),
-                      small ? (bw | (bh << 8) | kHitMask) : 0u, mask, count);
+  const bool masked = bw * bh <= kMaskTiles && fp.tile_count <= (1u << kHitOriginBits);
+  const uint32_t origin = (uint32_t)box.tmin_y * fp.tiles_x + (uint32_t)box.tmin_x;
+  o.hits = masked ? make_uint2(mask, kHitMasked | ((bw - 1u) << kHitOriginBits) | origin)
+                  : make_uint2(count, 0u);
   // SH colour (projectionHelpers.ts:118-176), direction = normalized
   // object-space offset from the camera.
   uint32_t degree = fp.sh_degree;
@@ -750,13 +756,13 @@ struct LoadOrdered {  // visibility.ts:63-76: tileCounts[sorted[rank].id]
   const uint2* sorted_a;   // the depth sort's last output ...
   const uint2* sorted_b;   // ... and its last input (when that pass was skipped)
   const uint32_t* in_place;
-  const uint4* hits;       // per ordinal; .w is the tile count
-  uint4* ordered_hits;     // the same records in depth order, for emission
+  const uint2* hits;       // per ordinal: tile-hit record (kHitMasked)
+  uint2* ordered_hits;     // the same records in depth order, for emission
   GS_DEV uint32_t operator()(uint32_t r) const {
     const uint2* sorted = *in_place ? sorted_b : sorted_a;
-    uint4 h = hits[sorted[r].y];
+    uint2 h = hits[sorted[r].y];
     ordered_hits[r] = h;
-    return h.w;
+    return hit_count(h);
   }
 };
 struct StoreCompact {  // visibility.ts:37-61 (+ visibleOffsets)
@@ -1281,7 +1287,7 @@ struct EmitArgs {
   const uint2* sorted_a;
   const uint2* sorted_b;
   const Splat* splat;
-  const uint4* ordered_hits;
+  const uint2* ordered_hits;
   const uint32_t* offsets;
   uint2* records;
 };
@@ -1295,7 +1301,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const ViewArgs<EmitArgs, kN> gr
   const uint2* __restrict__ sorted_a = a.sorted_a;
   const uint2* __restrict__ sorted_b = a.sorted_b;
   const Splat* __restrict__ splat = a.splat;
-  const uint4* __restrict__ ordered_hits = a.ordered_hits;
+  const uint2* __restrict__ ordered_hits = a.ordered_hits;
   const uint32_t* __restrict__ offsets = a.offsets;
   uint2* __restrict__ records = a.records;
   uint32_t rank = (blockIdx.x - view * grp.bpv) * kBlock + threadIdx.x;
@@ -1306,20 +1312,21 @@ __global__ __launch_bounds__(kBlock) void k_emit(const ViewArgs<EmitArgs, kN> gr
   // The depth sort's last output, or its last input if that pass was skipped.
   const uint2* sorted_depth = state->depth_in_place ? sorted_b : sorted_a;
   uint32_t gid = 0u, reserved = 0u, base = 0u, local = 0u;
-  uint4 h = make_uint4(0u, 0u, 0u, 0u);
+  uint2 h = make_uint2(0u, 0u);
   if (live) {
     gid = sorted_depth[rank].y;
     h = ordered_hits[rank];
-    reserved = h.w;
+    reserved = hit_count(h);
     base = offsets[rank];
   }
-  const bool masked = (h.y & kHitMask) != 0u;
+  const bool masked = (h.y & kHitMasked) != 0u;
   if (live && masked) {
     // The projection's hits, in the same row-major order.
-    uint32_t tx0 = h.x & 0xffffu, ty0 = h.x >> 16, bw = h.y & 0xffu;
-    for (uint32_t m = h.z; m != 0u; m &= m - 1u, local++) {
+    const uint32_t origin = h.y & ((1u << kHitOriginBits) - 1u);
+    const uint32_t bw = ((h.y >> kHitOriginBits) & 31u) + 1u;
+    for (uint32_t m = h.x; m != 0u; m &= m - 1u, local++) {
       uint32_t bit = (uint32_t)__builtin_ctz(m);
-      uint32_t tile = (ty0 + bit / bw) * fp.tiles_x + tx0 + bit % bw;
+      uint32_t tile = origin + (bit / bw) * fp.tiles_x + bit % bw;
       if (base + local < capacity) records[base + local] = make_uint2(tile, gid);
     }
   }

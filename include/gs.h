@@ -350,7 +350,7 @@ int gs_cull_gradient(gs_ctx* ctx, uint32_t n, const float* features, const float
  * per GPU, held from library load until the process ends. config supplies
  * the options, the device and the intersection capacity
  * (config->intersection_capacity); the frame's working set, about
- * 144 B per active Gaussian + 16 B per intersection of capacity + 20 B per
+ * 128 B per active Gaussian + 16 B per intersection of capacity + 20 B per
  * tile pixel per raster chunk task, must fit the array, else GS_E_RANGE.
  * Kernels are enqueued on `stream` (a hipStream_t; NULL is the null stream)
  * and nothing is allocated or freed. With state == NULL the call is
