@@ -205,7 +205,7 @@ def batched(gs, reps=10):
     return out
 
 
-def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
+def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3, target_psnr=46.0):
     """Train the cull on orbit views of the 3M-Gaussian scene, calibrate its
     output bias on validation views that are neither trained on nor timed
     (mean PSNR >= 46 dB against plain, 1 dB above the 45 dB bar for the
@@ -236,9 +236,9 @@ def learned_cull(gs, train_views=512, eval_views=64, calib_views=32, reps=3):
     train_s = time.perf_counter() - t0
     progress("learned cull: calibrating and evaluating")
     shift, rows = cull.calibrate([c for c, _ in valid], [s for _, s in valid],
-                                 target_psnr=46.0)
+                                 target_psnr=target_psnr)
     if shift is None:
-        raise RuntimeError("the learned cull reached no calibration shift at 46 dB")
+        raise RuntimeError(f"the learned cull reached no calibration shift at {target_psnr} dB")
     cams, sizes = [c for c, _ in held], [s for _, s in held]
 
     def timed():
