@@ -2154,8 +2154,12 @@ struct RasterArgs {
   uint32_t* record;
 };
 
+// Register budget: 5 waves per SIMD (96 VGPRs, a few spilled outside the
+// blend loop) instead of the compiler's 113 VGPRs at 4 waves: the 1M/1080p
+// frame 0.422 -> 0.416 ms, where this kernel runs beside the whole-tile one.
 template <int kEdge, int kBatch, bool kRecord, int kN>
-__global__ __launch_bounds__(light_threads(kEdge, kBatch)) void k_raster_light(
+__global__ __launch_bounds__(light_threads(kEdge, kBatch)) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_raster_light(
     const ViewArgs<RasterArgs, kN> grp) {
   __shared__ RasterLds<kEdge * kEdge, kBatch> lds;
   for (uint32_t view = 0; view < (kN == 1 ? 1u : grp.n); view++) {
@@ -2296,8 +2300,12 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const ViewArgs<HeavyUnitAr
 // tiles per item. A block starts with item b / 8 and then takes the XCD's
 // next unclaimed item (one atomic per item), so blocks that finish early
 // keep working.
+// Register budget: 6 waves per SIMD (at most 80 VGPRs, no spills; the row
+// slabs' 4- and 2-wave blocks took 81-85 at 5 waves); the whole-tile block
+// (16 waves) fills its CU's LDS and runs at 4 waves per SIMD either way.
 template <int kEdge, int kShape, bool kRecord, int kN>
-__global__ __launch_bounds__(heavy_threads(kEdge, kShape)) void k_raster_heavy(
+__global__ __launch_bounds__(heavy_threads(kEdge, kShape)) __attribute__((amdgpu_waves_per_eu(6, 8)))
+void k_raster_heavy(
     const ViewArgs<RasterArgs, kN> grp) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr bool kGroup = kEdge != 32;  // lane groups per pixel (edges 8, 16)
@@ -2402,14 +2410,14 @@ __global__ __launch_bounds__(heavy_threads(kEdge, kShape)) void k_raster_heavy(
 
 // Persistent grid of k_raster_heavy, a multiple of 8 blocks (at least 8):
 // whole-tile blocks (16 waves) one per CU, leaving room for the light-tile
-// kernel beside them; row-slab blocks (4 or 2 waves) 20 waves per CU, the
-// waves their registers allow (measured faster than leaving the light
-// kernel room: the slab units outnumber the CUs and gain most from
+// kernel beside them; row-slab blocks (4 or 2 waves) 24 waves per CU, the
+// waves their register budget allows (measured faster than leaving the
+// light kernel room: the slab units outnumber the CUs and gain most from
 // occupancy).
 template <int kEdge, int kShape>
 inline uint32_t heavy_grid(uint32_t cu_count) {
   uint32_t waves = (uint32_t)heavy_threads(kEdge, kShape) / 64u;
-  uint32_t per_cu = waves >= 16u ? 1u : 20u / waves;
+  uint32_t per_cu = waves >= 16u ? 1u : 24u / waves;
   uint32_t g = (cu_count * per_cu) & ~7u;
   return g < 8u ? 8u : g;
 }
