@@ -55,6 +55,7 @@ class GsConfig(C.Structure):
         ("heavy_split_units", u32),
         ("heavy_fine_split_units", u32),
         ("view_group", u32),
+        ("heavy_pair_units", u32),
     ]
 
 

@@ -352,6 +352,7 @@ extern "C" void gs_config_default(gs_config* c) {
   c->view_group = 1;
   c->heavy_split_units = GS_SPLIT_AUTO;
   c->heavy_fine_split_units = GS_SPLIT_AUTO;
+  c->heavy_pair_units = GS_SPLIT_AUTO;
 }
 
 // GaussianPass.ts:157-260 option validation.
@@ -815,6 +816,13 @@ void fill_params(const gs_config& c, uint32_t cu_count, const gs_frame* f,
   // frame (17,773 units) 25 % faster with 4.
   p.fine_split_units =
       c.heavy_fine_split_units == GS_SPLIT_AUTO ? cu_count * 16 : c.heavy_fine_split_units;
+  // Between the two, up to 5.5 units per CU, whole tiles with two lanes per
+  // pixel (three 512-thread blocks to a CU) instead of 8-row slabs, which
+  // stage every record twice: plain 3M/1080p views of 857-1,337 units
+  // rasterize 1-9 % faster, 1,546 units 1.5 % slower, and 2560x1440 views of
+  // 1,148-1,354 units about 1 % slower (the 64-view 2M batch: 1,667 ->
+  // 1,733 frames/s with the launch order of launch_raster).
+  p.pair_units = c.heavy_pair_units == GS_SPLIT_AUTO ? cu_count * 11 / 2 : c.heavy_pair_units;
 }
 
 // Launch description of view v of `set` on lane l. All lane buffers are
@@ -1298,7 +1306,7 @@ extern "C" int gs_profile_end(gs_ctx* ctx, double stage_ms[GS_STAGE_COUNT], doub
         return fail(ctx, GS_E_DEVICE, "hipEventElapsedTime");
       sums[st] += ms;
     }
-    // The heavy-unit kernel: of the three shapes' launches, all but the
+    // The heavy-unit kernel: of the four shapes' launches, all but the
     // frame's exit at once; its time is the longest of their intervals
     // (each taken between the events around it on its own stream).
     float whole = 0.0f, heavy = 0.0f;

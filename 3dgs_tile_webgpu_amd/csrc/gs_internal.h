@@ -69,8 +69,9 @@ struct FrameParams {
   uint32_t output_depth;
   uint32_t raster_stats;
   uint32_t tile_size;               // tile edge: 8, 16 or 32 (16 matches constants.ts:1)
-  uint32_t split_units;             // heavy units above which 16x16 tiles run as row slabs
-  uint32_t fine_split_units;        // ... and above which those slabs have 4 rows, not 8
+  uint32_t split_units;             // heavy units above which 16x16 tiles leave 4 lanes per pixel
+  uint32_t fine_split_units;        // ... above which row slabs have 4 rows, not 8
+  uint32_t pair_units;              // ... above which whole tiles with 2 lanes give way to slabs
 };
 
 // Device-side counters of one frame (dispatch state + outputState).
@@ -203,7 +204,7 @@ enum Stage : int {
 // Then a pair (before, after) around each heavy shape's launch, on the
 // stream it runs on.
 constexpr int kMarkHeavyBegin = kStageCount + 1;
-constexpr int kMarkHeavyShapes = 3;
+constexpr int kMarkHeavyShapes = 4;
 constexpr int kMarks = kMarkHeavyBegin + 2 * kMarkHeavyShapes;
 
 // The raster's second branch: the heavy-unit kernels run on aux, forked

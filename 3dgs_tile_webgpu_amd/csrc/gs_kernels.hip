@@ -1532,32 +1532,38 @@ struct PixelSum {
 // round:
 //  - whole tiles: one block per unit, four lanes per pixel (256 threads at
 //    edge 8, 1024 at 16);
-//  - row slabs (edge 16, frames with more than FrameParams::split_units
-//    units) with two lanes per pixel: two blocks of 256 threads per unit,
-//    each covering 8 rows, five to a CU; above fine_split_units units four
-//    blocks of 128 threads covering 4 rows, ten to a CU. Each slab stops as
-//    soon as its own pixels are saturated and a unit's work spreads over
-//    CUs, at the price of staging every record once per slab, which only
-//    pays when the units outnumber the CUs (and 4-row slabs only when they
-//    outnumber them many times over).
+//  - at edge 16, frames with more than FrameParams::split_units units use
+//    two lanes per pixel: first whole tiles (one block of 512 threads per
+//    unit, three to a CU), above pair_units row slabs (two blocks of 256
+//    threads per unit, each covering 8 rows, six to a CU), above
+//    fine_split_units four blocks of 128 threads covering 4 rows, twelve
+//    to a CU. A slab stops as soon as its own pixels are saturated and a
+//    unit's work spreads over CUs, at the price of staging every record
+//    once per slab, which pays only when the units outnumber the CUs many
+//    times over (or saturate early, as the 6M/4K frame's do).
 // Edge 32 uses one block of 1024 threads per unit (one lane per pixel, 512
 // records per round).
 constexpr uint32_t kLightMax = 1024;  // records up to which a tile is light
 constexpr int kHeavyThreads = 1024;  // edge 32
 constexpr int log2_edge(int e) { return e == 8 ? 3 : (e == 16 ? 4 : 5); }
 constexpr int light_threads(int e, int batch) { return e * e > batch ? e * e : batch; }
-// Heavy shapes (DeviceState::heavy_split; row slabs at edge 16 only).
-constexpr int kShapeWhole = 0, kShapeSlab8 = 1, kShapeSlab4 = 2;
+// Heavy shapes (DeviceState::heavy_split; all but kShapeWhole at edge 16
+// only): whole tiles with four lanes per pixel, 8-row and 4-row slabs, and
+// whole tiles with two lanes per pixel (kShapePair, a "slab" of all 16 rows).
+constexpr int kShapeWhole = 0, kShapeSlab8 = 1, kShapeSlab4 = 2, kShapePair = 3;
 constexpr int heavy_slab_rows(int e, int shape) {
-  return e == 16 && shape != kShapeWhole ? (shape == kShapeSlab8 ? 8 : 4) : e;
+  return e == 16 && shape != kShapeWhole
+             ? (shape == kShapeSlab8 ? 8 : (shape == kShapeSlab4 ? 4 : 16))
+             : e;
 }
 constexpr int heavy_slabs(int e, int shape) { return e / heavy_slab_rows(e, shape); }
 constexpr int heavy_slab_pixels(int e, int shape) { return e * heavy_slab_rows(e, shape); }
-// Lanes per pixel at edges 8 and 16: four for whole tiles, whose frames
-// have about as many units as CUs and wait on the longest unit (more lanes,
-// shorter units); two for row slabs, whose frames have many more units
-// than CUs and are bound by total VALU work (the lanes of a group repeat
-// every blend, so fewer lanes do less of it).
+// Lanes per pixel at edges 8 and 16: four for whole tiles in frames with
+// about as many units as CUs, which wait on the longest unit (more lanes,
+// shorter units); two for the shapes of frames with many more units than
+// CUs, which are bound by total VALU work and occupancy (two lanes split a
+// candidate step's blends two ways instead of four and make 512- or
+// smaller-thread blocks, several to a CU).
 constexpr int heavy_lanes(int e, int shape) { return e == 16 && shape != kShapeWhole ? 2 : 4; }
 constexpr int heavy_threads(int e, int shape) {
   return e == 32 ? kHeavyThreads : heavy_lanes(e, shape) * heavy_slab_pixels(e, shape);
@@ -2256,9 +2262,10 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const ViewArgs<HeavyUnitAr
       next += class_count[c];
     }
     state->heavy_units = min(next, unit_capacity);
-    state->heavy_split = next <= fp.split_units
-                             ? (uint32_t)kShapeWhole
-                             : (uint32_t)(next <= fp.fine_split_units ? kShapeSlab8 : kShapeSlab4);
+    state->heavy_split = next <= fp.split_units       ? (uint32_t)kShapeWhole
+                         : next > fp.fine_split_units ? (uint32_t)kShapeSlab4
+                         : next > fp.pair_units       ? (uint32_t)kShapeSlab8
+                                                      : (uint32_t)kShapePair;
     for (int x = 0; x < 8; x++) state->heavy_next[x] = 0u;
   }
   __syncthreads();
@@ -2309,7 +2316,7 @@ void k_raster_heavy(
     const ViewArgs<RasterArgs, kN> grp) {
   constexpr int kPixels = kEdge * kEdge;
   constexpr bool kGroup = kEdge != 32;  // lane groups per pixel (edges 8, 16)
-  constexpr bool kSplit = kShape != kShapeWhole;  // row slabs, which also take the light tiles
+  constexpr bool kSplit = kShape != kShapeWhole;  // two-lane shapes, which also take the light tiles
   constexpr uint32_t kSlabs = heavy_slabs(kEdge, kShape);
   constexpr int kBatch = heavy_batch(kEdge, kShape);
   __shared__ RasterLds<heavy_slab_pixels(kEdge, kShape), kBatch> lds;
@@ -2410,10 +2417,10 @@ void k_raster_heavy(
 
 // Persistent grid of k_raster_heavy, a multiple of 8 blocks (at least 8):
 // whole-tile blocks (16 waves) one per CU, leaving room for the light-tile
-// kernel beside them; row-slab blocks (4 or 2 waves) 24 waves per CU, the
-// waves their register budget allows (measured faster than leaving the
-// light kernel room: the slab units outnumber the CUs and gain most from
-// occupancy).
+// kernel beside them; the two-lane shapes' blocks (8, 4 or 2 waves) 24
+// waves per CU, the waves their register budget allows (measured faster
+// than leaving the light kernel room: their units outnumber the CUs and
+// gain most from occupancy).
 template <int kEdge, int kShape>
 inline uint32_t heavy_grid(uint32_t cu_count) {
   uint32_t waves = (uint32_t)heavy_threads(kEdge, kShape) / 64u;
@@ -2600,13 +2607,13 @@ hipError_t launch_raster(const FrameBuffers* views, uint32_t n, hipStream_t s,
     if (e == hipSuccess) e = hipStreamWaitEvent(rs.aux, rs.fork, 0);
     if (e != hipSuccess) return e;
   }
-  // The whole-tile kernel on the second branch; (edge 16) the two row-slab
-  // kernels on s, ahead of the light tiles. Each takes the views of its
+  // The whole-tile kernel on the second branch; on s the light tiles, then
+  // (edge 16) the three two-lane shapes. Each kernel takes the views of its
   // shape, and one whose views all chose another shape exits at once. So in
-  // whole-tile frames the slab launches' exits delay the light tiles (which
-  // finish before the heavy units) instead of the join, and in slab frames
-  // the whole-tile launch exits beside the slab kernel instead of ahead of
-  // it.
+  // whole-tile frames the other shapes' exits follow the light tiles (which
+  // finish before the heavy units) instead of delaying the join, and in
+  // two-lane frames the whole-tile and light launches exit beside or ahead
+  // of the working kernel.
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin], aux);
   launch_heavy<kEdge, kShapeWhole, kRecord>(ra, b0.cu_count, aux);
   if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 1], aux);
@@ -2614,16 +2621,6 @@ hipError_t launch_raster(const FrameBuffers* views, uint32_t n, hipStream_t s,
     e = hipEventRecord(rs.join, rs.aux);
     if (e != hipSuccess) return e;
   }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], s);
-  if constexpr (kEdge == 16) {
-    launch_heavy<kEdge, kShapeSlab8, kRecord>(ra, b0.cu_count, s);
-  }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], s);
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 4], s);
-  if constexpr (kEdge == 16) {
-    launch_heavy<kEdge, kShapeSlab4, kRecord>(ra, b0.cu_count, s);
-  }
-  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 5], s);
   uint32_t light_blocks = max_tiles < kLightBlocks ? max_tiles : kLightBlocks;
   if (n == 1)
     k_raster_light<kEdge, kBatch, kRecord, 1>
@@ -2631,6 +2628,18 @@ hipError_t launch_raster(const FrameBuffers* views, uint32_t n, hipStream_t s,
   else
     k_raster_light<kEdge, kBatch, kRecord, kGroupMax>
         <<<light_blocks, light_threads(kEdge, kBatch), 0, s>>>(ra);
+  // Mark pairs after the whole tiles' (every edge records them; other
+  // edges launch nothing between): the 8-row slabs, the 4-row slabs, the
+  // two-lane whole tiles.
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 6], s);
+  if constexpr (kEdge == 16) launch_heavy<kEdge, kShapePair, kRecord>(ra, b0.cu_count, s);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 7], s);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 2], s);
+  if constexpr (kEdge == 16) launch_heavy<kEdge, kShapeSlab8, kRecord>(ra, b0.cu_count, s);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 3], s);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 4], s);
+  if constexpr (kEdge == 16) launch_heavy<kEdge, kShapeSlab4, kRecord>(ra, b0.cu_count, s);
+  if (marks) (void)hipEventRecord(marks[kMarkHeavyBegin + 5], s);
   if (rs.aux) {
     e = hipStreamWaitEvent(s, rs.join, 0);
     if (e != hipSuccess) return e;

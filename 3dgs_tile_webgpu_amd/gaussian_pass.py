@@ -57,7 +57,7 @@ ACCEPTED_OPTIONS = {"redrawStrategy", "radixBackend", "dataMode", "colorSpace", 
 KNOWN_OPTIONS = set(DEFAULT_OPTIONS) | ACCEPTED_OPTIONS | {
     "fixedSize", "depthRange", "intersectionCapacity", "initialIntersectionCapacity",
     "maxIntersectionCapacity", "device", "batchLanes", "heavySplitUnits",
-    "heavyFineSplitUnits", "viewGroup",
+    "heavyFineSplitUnits", "heavyPairUnits", "viewGroup",
 }
 MAX_CAPACITY = 1 << 28
 
@@ -174,7 +174,7 @@ def validate_options(options: dict) -> dict:
     group = o.get("viewGroup", 1)
     if not isinstance(group, int) or isinstance(group, bool) or not 1 <= group <= 8:
         raise N.GsRangeError(N.GS_E_RANGE, "viewGroup must be an integer in [1, 8]")
-    for name in ("heavySplitUnits", "heavyFineSplitUnits"):
+    for name in ("heavySplitUnits", "heavyFineSplitUnits", "heavyPairUnits"):
         split = o.get(name)
         if split is not None and (not isinstance(split, int) or isinstance(split, bool)
                                   or not 0 <= split < 0xFFFFFFFF):
@@ -208,6 +208,8 @@ def config_for(o: dict, policy: IntersectionCapacityPolicy) -> N.GsConfig:
     fine = o.get("heavyFineSplitUnits")
     cfg.heavy_fine_split_units = 0xFFFFFFFF if fine is None else int(fine)
     cfg.view_group = int(o.get("viewGroup", 1))
+    pair = o.get("heavyPairUnits")
+    cfg.heavy_pair_units = 0xFFFFFFFF if pair is None else int(pair)
     return cfg
 
 

@@ -154,6 +154,7 @@ void read_config(napi_env env, napi_value o, gs_config* c) {
   c->heavy_fine_split_units =
       (uint32_t)num(env, o, "heavyFineSplitUnits", c->heavy_fine_split_units);
   c->view_group = (uint32_t)num(env, o, "viewGroup", c->view_group);
+  c->heavy_pair_units = (uint32_t)num(env, o, "heavyPairUnits", c->heavy_pair_units);
   napi_value bg;
   if (get_prop(env, o, "background", &bg)) {
     for (uint32_t i = 0; i < 4; i++) {

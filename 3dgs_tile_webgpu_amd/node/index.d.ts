@@ -39,12 +39,17 @@ export interface GaussianPassOptions {
   tileSize?: 8 | 16 | 32;
   radixBits?: 4 | 6 | 8;
   rasterBatch?: 128 | 256;
-  /** Heavy raster units above which 16-pixel tiles run as row slabs
-   * (default: 1.25 per compute unit); never changes any output. */
+  /** Heavy raster units above which 16-pixel tiles run with two lanes per
+   * pixel instead of four, as whole tiles and then row slabs (default: 1.25
+   * per compute unit); never changes any output. */
   heavySplitUnits?: number;
-  /** Heavy raster units above which those slabs have 4 rows instead of 8
+  /** Heavy raster units above which row slabs have 4 rows instead of 8
    * (default: 16 per compute unit); never changes any output. */
   heavyFineSplitUnits?: number;
+  /** Heavy raster units above which the two-lane shape is row slabs rather
+   * than whole tiles (above heavySplitUnits; default: 5.5 per compute unit);
+   * never changes any output. */
+  heavyPairUnits?: number;
   /** Views of a batch rendered by each launch, 1..8 (default 1): every
    * stage is one kernel over the group's views; never changes any output. */
   viewGroup?: number;

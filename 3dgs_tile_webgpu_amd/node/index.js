@@ -44,7 +44,7 @@ const ACCEPTED = ['redrawStrategy', 'radixBackend', 'dataMode', 'colorSpace', 'p
   'intersectionReadbackInterval'];
 const KNOWN = new Set([...Object.keys(DEFAULTS), ...ACCEPTED, 'fixedSize', 'depthRange',
   'intersectionCapacity', 'initialIntersectionCapacity', 'maxIntersectionCapacity', 'device',
-  'heavySplitUnits', 'heavyFineSplitUnits', 'viewGroup']);
+  'heavySplitUnits', 'heavyFineSplitUnits', 'heavyPairUnits', 'viewGroup']);
 const BOUNDARY = { clip: 0, 'repeat-x': 1, 'repeat-y': 2 };
 const MAX_CAPACITY = 2 ** 28;
 const RESOURCE_NAMES = ['projected_mean', 'projected_conic', 'projected_color', 'visible_offsets',
@@ -148,7 +148,7 @@ function validateOptions(options) {
   if (o.viewGroup !== undefined && !(Number.isInteger(o.viewGroup) && o.viewGroup >= 1 && o.viewGroup <= 8)) {
     throw new RangeError('viewGroup must be an integer in [1, 8]');
   }
-  for (const name of ['heavySplitUnits', 'heavyFineSplitUnits']) {
+  for (const name of ['heavySplitUnits', 'heavyFineSplitUnits', 'heavyPairUnits']) {
     const v = o[name];
     if (v !== undefined && !(Number.isInteger(v) && v >= 0 && v < 0xffffffff)) {
       throw new RangeError(name + ' must be a non-negative integer');
@@ -177,6 +177,7 @@ function nativeConfig(o, policy) {
     rasterBatch: o.rasterBatch,
     heavySplitUnits: o.heavySplitUnits === undefined ? 0xffffffff : o.heavySplitUnits,
     heavyFineSplitUnits: o.heavyFineSplitUnits === undefined ? 0xffffffff : o.heavyFineSplitUnits,
+    heavyPairUnits: o.heavyPairUnits === undefined ? 0xffffffff : o.heavyPairUnits,
     viewGroup: dflt(o.viewGroup, 1),
   };
 }

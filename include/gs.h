@@ -85,7 +85,9 @@ typedef struct gs_config {
                                         is accepted at present (default 1) */
   uint32_t heavy_split_units;        /* heavy raster units (tiles above 1024
                                         records, chunk tasks) above which 16-pixel
-                                        tiles are rasterized as row slabs;
+                                        tiles are rasterized with two lanes per
+                                        pixel instead of four (whole tiles, then
+                                        row slabs, see heavy_pair_units);
                                         GS_SPLIT_AUTO (default) = 1.25 per compute
                                         unit; never changes any output */
   uint32_t heavy_fine_split_units;   /* heavy raster units above which those
@@ -98,6 +100,12 @@ typedef struct gs_config {
                                         largest views grouped first; each view
                                         of a group has its own working set.
                                         Never changes any output */
+  uint32_t heavy_pair_units;         /* heavy raster units above which the
+                                        two-lane shape is row slabs rather than
+                                        whole tiles (applies above
+                                        heavy_split_units); GS_SPLIT_AUTO
+                                        (default) = 5.5 per compute unit; never
+                                        changes any output */
 } gs_config;
 
 #define GS_SPLIT_AUTO 0xffffffffu

@@ -89,6 +89,7 @@ GROUPS = [
     (4, {}),
     (3, {"tileSize": 8, "radixBits": 4, "rasterChunkSize": 256}),
     (8, {"rasterChunkSize": 1024, "heavySplitUnits": 0}),
+    (5, {"heavySplitUnits": 0, "heavyPairUnits": 0}),
 ]
 
 
@@ -98,9 +99,10 @@ GROUPS = [
 def test_view_groups_match_single_passes(gs, oracle, gpu_available, small, group, options):
     """Views rendered in groups (each stage one launch over a group's views)
     reproduce one gs_render per camera bit for bit: six views of five
-    resolutions, growing from a small capacity, with groups of 4 (+2), 3 and
-    all six; at tile 8 with 4-bit digits the views' tile sorts have 3 and 4
-    passes (a group runs the larger count); chunk tasks and row slabs."""
+    resolutions, growing from a small capacity, with groups of 4 (+2), 3,
+    all six and 5 (+1); at tile 8 with 4-bit digits the views' tile sorts
+    have 3 and 4 passes (a group runs the larger count); chunk tasks, whole
+    tiles with two lanes per pixel and 8-row slabs."""
     cams, sizes = batch_cameras(gs, 6, seed=21)
     b = gs.BatchedPass(small, {"viewGroup": group, "initialIntersectionCapacity": 4096,
                                **options})

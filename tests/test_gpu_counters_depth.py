@@ -48,6 +48,8 @@ def assert_counters_match(p, ref):
     ((640, 360), {"rasterChunkSize": None}),
     ((640, 360), {"heavySplitUnits": 0}),
     ((640, 360), {"heavySplitUnits": 0, "rasterChunkSize": 256}),
+    ((640, 360), {"heavySplitUnits": 0, "heavyPairUnits": 0}),
+    ((640, 360), {"heavySplitUnits": 0, "heavyPairUnits": 0, "rasterChunkSize": 256}),
     ((640, 360), {"heavySplitUnits": 0, "heavyFineSplitUnits": 0}),
     ((700, 420), {"tileSize": 8, "rasterChunkSize": 256}),
     ((700, 420), {"tileSize": 32, "rasterChunkSize": 2048}),

@@ -30,13 +30,15 @@ def small(gs):
     return gs.workload_scene("small", count=60_000)
 
 
-@pytest.mark.parametrize("split", [None, 0], ids=["auto", "slabs"])
+@pytest.mark.parametrize("split", [{}, {"heavySplitUnits": 0},
+                                   {"heavySplitUnits": 0, "heavyPairUnits": 0}],
+                         ids=["auto", "pairs", "slabs"])
 def test_collected_features_and_record_match_oracle(gs, oracle, gpu_available, small, split):
     w = gs.WORKLOADS["small"]
     cam = w.camera()
-    opts = {"intersectionCapacity": 1 << 21}
-    if split is not None:
-        opts["heavySplitUnits"] = split  # the instrumented raster as row slabs
+    # The instrumented raster as whole tiles, with two lanes per pixel, or
+    # as 8-row slabs.
+    opts = {"intersectionCapacity": 1 << 21, **split}
     p = gs.gaussian_pass(cam, small, opts)
     cull = gs.LearnedCull(p, seed=1)
     cull.reserve(1 << 17)

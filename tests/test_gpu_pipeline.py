@@ -95,6 +95,9 @@ def test_small_workload_bit_exact(gs, oracle, gpu_available, small):
     {"heavySplitUnits": 0},
     {"heavySplitUnits": 0, "rasterChunkSize": 256},
     {"heavySplitUnits": 0, "boundaryPolicy": "repeat-x"},
+    {"heavySplitUnits": 0, "heavyPairUnits": 0},
+    {"heavySplitUnits": 0, "heavyPairUnits": 0, "rasterChunkSize": 256},
+    {"heavySplitUnits": 0, "heavyPairUnits": 0, "boundaryPolicy": "repeat-y"},
     {"heavySplitUnits": 0, "heavyFineSplitUnits": 0},
     {"heavySplitUnits": 0, "heavyFineSplitUnits": 0, "rasterChunkSize": 256},
     {"heavySplitUnits": 0, "heavyFineSplitUnits": 0, "boundaryPolicy": "repeat-y"},
@@ -183,13 +186,14 @@ def test_graph_replay_and_recapture(gs, oracle, gpu_available, small):
 
 
 @pytest.mark.parametrize("opts", [{}, {"heavySplitUnits": 0},
+                                  {"heavySplitUnits": 0, "heavyPairUnits": 0},
                                   {"heavySplitUnits": 0, "heavyFineSplitUnits": 0},
                                   {"intersectionCapacity": 1 << 24}],
-                         ids=["auto", "slabs8", "slabs4", "large-sort-partitions"])
+                         ids=["auto", "pairs", "slabs8", "slabs4", "large-sort-partitions"])
 def test_1m_1080p_bit_exact(gs, oracle, gpu_available, opts):
     """Heavy units run as whole tiles here by default (275 units); forced
-    to row slabs, as 8-row slabs (275 units is below the 4-row threshold)
-    or as 4-row ones. A capacity of 2^24 records sizes the tile sort for
+    to two lanes per pixel, as whole tiles (275 units is below the row-slab
+    threshold), as 8-row slabs or as 4-row ones. A capacity of 2^24 records sizes the tile sort for
     its 16-keys-per-thread partitions. Every shape must give the oracle's
     bits."""
     w = gs.WORKLOADS["1m_1080p"]

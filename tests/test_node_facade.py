@@ -83,6 +83,10 @@ CASES = [
     {"heavyFineSplitUnits": 4096},
     {"heavyFineSplitUnits": -2},
     {"heavyFineSplitUnits": 0xFFFFFFFF},
+    {"heavyPairUnits": 0},
+    {"heavyPairUnits": 1408},
+    {"heavyPairUnits": -1},
+    {"heavyPairUnits": True},
     {"viewGroup": 1},
     {"viewGroup": 8},
     {"viewGroup": 0},
