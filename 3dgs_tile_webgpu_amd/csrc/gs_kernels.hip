@@ -2141,10 +2141,11 @@ GS_DEV void light_tile(const FrameParams& fp, DeviceState* __restrict__ state, u
 }
 
 // Light tiles, beside the whole-tile heavy kernel: the grid walks the tiles
-// in strides. In frames whose heavy units run as row slabs (edge 16) the
-// slab kernel takes the light tiles and the empty ones too, and this kernel
-// exits at once (while the whole-tile kernel it runs beside does the same),
-// before the slab kernel's persistent grid takes every CU.
+// in strides. In frames whose heavy units run with two lanes per pixel
+// (edge 16: two-lane whole tiles or row slabs) that shape's kernel takes the
+// light tiles and the empty ones too, and this kernel exits at once (as
+// does the four-lane whole-tile kernel), ahead of the two-lane kernel whose
+// persistent grid takes every CU.
 struct RasterArgs {
   FrameParams fp;
   DeviceState* state;
@@ -2269,8 +2270,8 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const ViewArgs<HeavyUnitAr
     for (int x = 0; x < 8; x++) state->heavy_next[x] = 0u;
   }
   __syncthreads();
-  // Row-slab frames (tile edge 16) also list the non-empty light tiles that
-  // are not chunked, after the heavy units, for the slab kernel.
+  // Two-lane frames (tile edge 16) also list the non-empty light tiles that
+  // are not chunked, after the heavy units, for the two-lane kernel.
   const bool list_light = fp.tile_size == 16 && state->heavy_split != 0u;
   const uint32_t heavy_total = state->heavy_units;
 #pragma unroll 4
@@ -2294,22 +2295,24 @@ __global__ __launch_bounds__(1024) void k_heavy_units(const ViewArgs<HeavyUnitAr
 }
 
 // Heavy units: per chunk task or heavy tile, S = heavy_slabs(E, kShape)
-// work items, one per row slab; the launches of every shape run and those
+// work items, one per row slab (S = 1 for whole tiles, with four or two
+// lanes per pixel); the launches of every shape run and those
 // the frame did not choose (DeviceState::heavy_split, set by
 // k_heavy_units) exit at once. The grid is persistent (as many blocks
 // as fit on the GPU at once, a multiple of 8). Workgroups go to the eight
 // XCDs in turn, so block b works for XCD x = b % 8: its items are
 // k = 0, 1, 2, ... of that XCD's sequence, where item k is unit
 // (k / S) * 8 + x, slab k % S. A unit's slabs therefore run on one XCD and
-// share its L2, and units start in list order (heaviest first). The row-slab
-// kernel then takes the light tiles listed after the heavy units, as slab
-// items of the same kind, and last the background of the empty tiles, 64
+// share its L2, and units start in list order (heaviest first). The
+// two-lane kernels then take the light tiles listed after the heavy units,
+// as items of the same kind, and last the background of the empty tiles, 64
 // tiles per item. A block starts with item b / 8 and then takes the XCD's
 // next unclaimed item (one atomic per item), so blocks that finish early
 // keep working.
 // Register budget: 6 waves per SIMD (at most 80 VGPRs, no spills; the row
-// slabs' 4- and 2-wave blocks took 81-85 at 5 waves); the whole-tile block
-// (16 waves) fills its CU's LDS and runs at 4 waves per SIMD either way.
+// slabs' 4- and 2-wave blocks took 81-85 at 5 waves); the four-lane
+// whole-tile block (16 waves) fills its CU's LDS and runs at 4 waves per
+// SIMD either way.
 template <int kEdge, int kShape, bool kRecord, int kN>
 __global__ __launch_bounds__(heavy_threads(kEdge, kShape)) __attribute__((amdgpu_waves_per_eu(6, 8)))
 void k_raster_heavy(
