@@ -191,6 +191,8 @@ SIGNATURES = {
     "gs_cull_train": (C.c_int, [P, u32, u32, f32, f32, C.c_uint64, C.POINTER(f32)]),
     "gs_cull_eval": (C.c_int, [P, u32, P, P]),
     "gs_cull_gradient": (C.c_int, [P, u32, P, P, f32, P, C.POINTER(f32)]),
+    "gs_cull_save": (C.c_int, [P, P, C.c_size_t]),
+    "gs_cull_load": (C.c_int, [P, C.c_char_p, C.c_size_t]),
     "gs_read_state": (C.c_int, [P, C.POINTER(GsState)]),
     "gs_resize_intersections": (C.c_int, [P, u32]),
     "gs_read_image": (C.c_int, [P, P, C.c_size_t, P, C.c_size_t]),

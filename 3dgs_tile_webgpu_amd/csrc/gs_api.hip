@@ -94,6 +94,7 @@ struct CullState {
   DeviceBuffer tmp_in, tmp_target, tmp_out;
   uint32_t ds_capacity = 0;
   uint32_t step = 0;
+  float threshold = 1e-4f;  // the last gs_cull_enable's (alpha * T)
   bool ready = false;
   bool enabled = false;
   bool collecting = false;
@@ -1840,6 +1841,7 @@ extern "C" int gs_cull_init(gs_ctx* ctx, uint64_t seed) {
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "cull state clear");
   c.step = 0;
+  c.threshold = 1e-4f;
   c.ready = true;
   return GS_OK;
 }
@@ -1890,6 +1892,99 @@ extern "C" int gs_cull_get_normalization(gs_ctx* ctx, float mu[16], float inv_si
   return GS_OK;
 }
 
+// Training checkpoint: a 32-byte header (magic, version, parameter and
+// feature counts, Adam steps taken, threshold) and the f32 arrays
+// params, Adam m, Adam v, mu, inv_sigma, little-endian as the host stores
+// them.
+namespace {
+constexpr char kCheckpointMagic[8] = {'G', 'S', 'C', 'U', 'L', 'L', '0', '1'};
+constexpr uint32_t kCheckpointVersion = 1;
+struct CheckpointHeader {
+  char magic[8];
+  uint32_t version, params, features, step;
+  float threshold;  // alpha * T, as gs_cull_enable takes it
+  uint32_t reserved;
+};
+static_assert(sizeof(CheckpointHeader) == 32, "checkpoint header layout");
+constexpr size_t kCheckpointBytes =
+    sizeof(CheckpointHeader) + (3 * (size_t)gs::kCullParams + 2 * gs::kCullIn) * sizeof(float);
+static_assert(kCheckpointBytes == GS_CULL_CHECKPOINT_BYTES, "checkpoint size");
+}  // namespace
+
+extern "C" int gs_cull_save(gs_ctx* ctx, void* dst, size_t bytes) {
+  if (!ctx || !dst) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  if (bytes != kCheckpointBytes) return fail(ctx, GS_E_RANGE, "checkpoint buffer size mismatch");
+  CullState& c = ctx->cull;
+  gs::CullNet n;
+  if ((rc = cull_sync_copy(ctx, &n, c.net.ptr, sizeof(n), hipMemcpyDeviceToHost, "cull network")))
+    return rc;
+  CheckpointHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, kCheckpointMagic, sizeof(h.magic));
+  h.version = kCheckpointVersion;
+  h.params = gs::kCullParams;
+  h.features = gs::kCullIn;
+  h.step = c.step;
+  h.threshold = c.threshold;
+  char* out = (char*)dst;
+  memcpy(out, &h, sizeof(h));
+  float* f = (float*)(out + sizeof(h));
+  const size_t pb = gs::kCullParams * sizeof(float);
+  if ((rc = cull_sync_copy(ctx, f, c.params.ptr, pb, hipMemcpyDeviceToHost, "cull parameters")) ||
+      (rc = cull_sync_copy(ctx, f + gs::kCullParams, c.m.ptr, pb, hipMemcpyDeviceToHost,
+                           "cull adam m")) ||
+      (rc = cull_sync_copy(ctx, f + 2 * gs::kCullParams, c.v.ptr, pb, hipMemcpyDeviceToHost,
+                           "cull adam v")))
+    return rc;
+  memcpy(f + 3 * gs::kCullParams, n.mu, sizeof(n.mu));
+  memcpy(f + 3 * gs::kCullParams + gs::kCullIn, n.inv_sigma, sizeof(n.inv_sigma));
+  return GS_OK;
+}
+
+extern "C" int gs_cull_load(gs_ctx* ctx, const void* src, size_t bytes) {
+  if (!ctx || !src) return GS_E_RANGE;
+  int rc = cull_require(ctx);
+  if (rc) return rc;
+  if (bytes != kCheckpointBytes) return fail(ctx, GS_E_RANGE, "checkpoint size mismatch");
+  CheckpointHeader h;
+  memcpy(&h, src, sizeof(h));
+  if (memcmp(h.magic, kCheckpointMagic, sizeof(h.magic)) != 0 || h.version != kCheckpointVersion ||
+      h.params != (uint32_t)gs::kCullParams || h.features != (uint32_t)gs::kCullIn)
+    return fail(ctx, GS_E_RANGE, "not a learned-cull checkpoint of this network");
+  // Everything is checked before anything is replaced.
+  std::vector<float> f(3 * gs::kCullParams + 2 * gs::kCullIn);
+  memcpy(f.data(), (const char*)src + sizeof(h), f.size() * sizeof(float));
+  if (!(h.threshold > 0.0f && h.threshold < 1.0f))
+    return fail(ctx, GS_E_RANGE, "checkpoint threshold out of range");
+  for (size_t k = 0; k < f.size(); k++)
+    if (!finite(f[k])) return fail(ctx, GS_E_RANGE, "checkpoint values must be finite");
+  for (int k = 0; k < gs::kCullParams; k++)
+    if (f[2 * gs::kCullParams + k] < 0.0f)
+      return fail(ctx, GS_E_RANGE, "checkpoint second moments must be non-negative");
+  CullState& c = ctx->cull;
+  gs::CullNet n;
+  if ((rc = cull_sync_copy(ctx, &n, c.net.ptr, sizeof(n), hipMemcpyDeviceToHost, "cull network")))
+    return rc;
+  pack_net(f.data(), n);
+  n.log_threshold = logf(h.threshold);  // as gs_cull_enable sets it
+  memcpy(n.mu, f.data() + 3 * gs::kCullParams, sizeof(n.mu));
+  memcpy(n.inv_sigma, f.data() + 3 * gs::kCullParams + gs::kCullIn, sizeof(n.inv_sigma));
+  const size_t pb = gs::kCullParams * sizeof(float);
+  if ((rc = cull_sync_copy(ctx, c.net.ptr, &n, sizeof(n), hipMemcpyHostToDevice, "cull network")) ||
+      (rc = cull_sync_copy(ctx, c.params.ptr, f.data(), pb, hipMemcpyHostToDevice,
+                           "cull parameters")) ||
+      (rc = cull_sync_copy(ctx, c.m.ptr, f.data() + gs::kCullParams, pb, hipMemcpyHostToDevice,
+                           "cull adam m")) ||
+      (rc = cull_sync_copy(ctx, c.v.ptr, f.data() + 2 * gs::kCullParams, pb,
+                           hipMemcpyHostToDevice, "cull adam v")))
+    return rc;
+  c.step = h.step;
+  c.threshold = h.threshold;
+  return GS_OK;
+}
+
 extern "C" int gs_cull_enable(gs_ctx* ctx, int enabled, float threshold) {
   if (!ctx) return GS_E_RANGE;
   int rc = cull_require(ctx);
@@ -1902,6 +1997,7 @@ extern "C" int gs_cull_enable(gs_ctx* ctx, int enabled, float threshold) {
                            hipMemcpyHostToDevice, "cull threshold")))
     return rc;
   ctx->cull.enabled = enabled != 0;
+  ctx->cull.threshold = threshold;
   return GS_OK;
 }
 

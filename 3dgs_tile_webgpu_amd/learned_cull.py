@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+from pathlib import Path
 
 import numpy as np
 
@@ -26,6 +27,8 @@ from .gaussian_pass import frame_for, object_frames_for
 
 PARAMS = 5313
 FEATURES = 16
+# gs_cull_save / gs_cull_load blob: 32-byte header, params, Adam m and v, mu, inv_sigma.
+CHECKPOINT_BYTES = 32 + (3 * PARAMS + 2 * FEATURES) * 4
 HIDDEN = 64
 # Feature indices (the order of include/gs.h).
 FEATURE_LOG_RADIUS = 0
@@ -115,6 +118,29 @@ class LearnedCull:
         inv = np.ascontiguousarray(inv_sigma, np.float32)
         N.check(N.lib().gs_cull_set_normalization(self._ctx, mu.ctypes.data, inv.ctypes.data),
                 self._ctx)
+
+    # -- checkpoint --------------------------------------------------------
+    def checkpoint(self) -> bytes:
+        """The training state (parameters, Adam moments and step count,
+        standardisation, threshold) as the library's checkpoint blob of
+        CHECKPOINT_BYTES bytes (gs_cull_save)."""
+        buf = (C.c_ubyte * CHECKPOINT_BYTES)()
+        N.check(N.lib().gs_cull_save(self._ctx, buf, CHECKPOINT_BYTES), self._ctx)
+        return bytes(buf)
+
+    def restore(self, blob: bytes) -> None:
+        """Resume from a checkpoint blob (gs_cull_load); the library rejects
+        a blob of another size, version or network, or with non-finite
+        values, with GsRangeError and keeps its state."""
+        blob = bytes(blob)
+        N.check(N.lib().gs_cull_load(self._ctx, blob, len(blob)), self._ctx)
+        self.threshold = float(np.frombuffer(blob, np.float32, 1, 24)[0])
+
+    def save(self, path) -> None:
+        Path(path).write_bytes(self.checkpoint())
+
+    def load(self, path) -> None:
+        self.restore(Path(path).read_bytes())
 
     # -- use ---------------------------------------------------------------
     def enable(self, threshold: float | None = None) -> None:

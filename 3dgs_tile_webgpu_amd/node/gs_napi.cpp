@@ -602,6 +602,33 @@ napi_value cull_get_params(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// Training checkpoint blob (gs_cull_save / gs_cull_load) as a Uint8Array.
+napi_value cull_save(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  args(env, info, 1, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  napi_value buffer, out;
+  void* p = nullptr;
+  napi_create_arraybuffer(env, GS_CULL_CHECKPOINT_BYTES, &p, &buffer);
+  napi_create_typedarray(env, napi_uint8_array, GS_CULL_CHECKPOINT_BYTES, buffer, 0, &out);
+  int rc = gs_cull_save(ctx, p, GS_CULL_CHECKPOINT_BYTES);
+  if (rc) return check(env, ctx, rc);
+  return out;
+}
+
+napi_value cull_load(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  args(env, info, 2, argv);
+  gs_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return nullptr;
+  void* p = nullptr;
+  size_t n = 0;
+  if (!typed(env, argv[1], napi_uint8_array, &p, &n))
+    return throw_status(env, GS_E_TYPE, "a checkpoint must be a Uint8Array");
+  return check(env, ctx, gs_cull_load(ctx, p, n));
+}
+
 napi_value cull_set_normalization(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   args(env, info, 3, argv);
@@ -821,6 +848,8 @@ NAPI_MODULE_INIT() {
       {"cullInit", nullptr, cull_init, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cullSetParams", nullptr, cull_set_params, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cullGetParams", nullptr, cull_get_params, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullSave", nullptr, cull_save, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+      {"cullLoad", nullptr, cull_load, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cullSetNormalization", nullptr, cull_set_normalization, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cullGetNormalization", nullptr, cull_get_normalization, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
       {"cullEnable", nullptr, cull_enable, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

@@ -195,6 +195,12 @@ export class LearnedCull {
   train(steps: number, opts?: { batch?: number; lr?: number; underWeight?: number; seed?: number }): number;
   /** W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3 (5313 floats). */
   params(): Float32Array;
+  /** Training state (parameters, Adam moments and step count,
+   * standardisation, threshold) as the library's checkpoint blob. */
+  checkpoint(): Uint8Array;
+  /** Resume from a checkpoint; throws RangeError for a blob of another
+   * size, version or network, or with non-finite values. */
+  restore(blob: Uint8Array): void;
   setParams(params: Float32Array): void;
   normalization(): { mu: Float32Array; invSigma: Float32Array };
   setNormalization(mu: Float32Array, invSigma: Float32Array): void;

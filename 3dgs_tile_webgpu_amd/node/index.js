@@ -699,6 +699,13 @@ class LearnedCull {
   }
   params() { return native.cullGetParams(this.owner._ctx); }
   setParams(p) { native.cullSetParams(this.owner._ctx, p); }
+  // Training checkpoint (parameters, Adam moments and steps, standardisation,
+  // threshold) as a Uint8Array, and resuming from one.
+  checkpoint() { return native.cullSave(this.owner._ctx); }
+  restore(blob) {
+    native.cullLoad(this.owner._ctx, blob);
+    this.threshold = new DataView(blob.buffer, blob.byteOffset, blob.byteLength).getFloat32(24, true);
+  }
   normalization() { return native.cullGetNormalization(this.owner._ctx); }
   setNormalization(mu, invSigma) { native.cullSetNormalization(this.owner._ctx, mu, invSigma); }
   predict(features) { return native.cullEval(this.owner._ctx, features); }

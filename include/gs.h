@@ -344,6 +344,19 @@ int gs_cull_train(gs_ctx* ctx, uint32_t steps, uint32_t batch, float lr, float u
 int gs_cull_eval(gs_ctx* ctx, uint32_t n, const float* features, float* out);
 int gs_cull_gradient(gs_ctx* ctx, uint32_t n, const float* features, const float* targets,
                      float under_weight, float* grad, float* loss);
+/* Training checkpoint (no counterpart in the reference; training state is
+ * the learned cull's own): the f32 parameters, Adam's first and second
+ * moments and step count, the input standardisation and the threshold, as
+ * one blob of GS_CULL_CHECKPOINT_BYTES bytes (a 32-byte header: magic
+ * "GSCULL01", u32 version 1, u32 GS_CULL_PARAMS, u32 GS_CULL_FEATURES, u32
+ * Adam steps taken, f32 threshold (gs_cull_enable's last), u32 0; then f32
+ * params, m, v, mu, inv_sigma; host byte order). Loading one resumes training where the
+ * saved context stopped; a blob of another size, version or network, or
+ * with non-finite values, is rejected with GS_E_RANGE and nothing changes.
+ * The dataset and the enabled flag are not part of it. */
+#define GS_CULL_CHECKPOINT_BYTES (32 + (3 * GS_CULL_PARAMS + 2 * GS_CULL_FEATURES) * 4)
+int gs_cull_save(gs_ctx* ctx, void* dst, size_t bytes);
+int gs_cull_load(gs_ctx* ctx, const void* src, size_t bytes);
 
 /* One frame without allocating (no counterpart in the reference): for
  * callers that cannot call hipMalloc -- code under HIP graph capture, tight

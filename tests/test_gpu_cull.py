@@ -267,3 +267,55 @@ def test_training_culls_on_held_out_1080p_views(gs, gpu_available):
     assert removed > opacity + 0.1 and removed > footprint + 0.1
     cull.set_params(learned)
     b.dispose()
+
+
+def test_checkpoint_resumes_training(gs, gpu_available, small):
+    """A checkpoint (gs_cull_save / gs_cull_load) holds the whole training
+    state. Resuming from one continues as the uninterrupted run does (to the
+    rounding of the gradient's float atomics), unlike a restart from the same
+    parameters with fresh Adam moments; the blob round-trips bit for bit and
+    restores the threshold; blobs of another size, magic or with non-finite
+    values are rejected and leave the state as it was."""
+    w = gs.WORKLOADS["small"]
+    cam = w.camera()
+    p = gs.gaussian_pass(cam, small, {"intersectionCapacity": 1 << 21})
+    cull = gs.LearnedCull(p, seed=1)
+    cull.reserve(1 << 18)
+    cull.collect(cam, 640, 360, rate=1.0, seed=3)
+    cull.fit_normalization()
+    cull.train(200, batch=4096, lr=2e-3, under_weight=300.0, seed=1)
+    cull.set_threshold(3e-4)
+    blob = cull.checkpoint()
+    assert len(blob) == 32 + (3 * 5313 + 2 * 16) * 4
+    assert blob[:8] == b"GSCULL01"
+    assert np.frombuffer(blob, np.uint32, 1, 20)[0] == 200  # Adam steps taken
+
+    cull.train(100, batch=4096, lr=2e-3, under_weight=300.0, seed=2)
+    p_continued = cull.params()
+    cull.set_threshold(1e-4)
+    cull.restore(blob)
+    assert cull.threshold == float(np.float32(3e-4))
+    assert cull.checkpoint() == blob
+    cull.train(100, batch=4096, lr=2e-3, under_weight=300.0, seed=2)
+    p_resumed = cull.params()
+
+    fresh = bytearray(blob)  # same parameters, Adam moments and step count cleared
+    fresh[20:24] = bytes(4)
+    start = 32 + 5313 * 4
+    fresh[start:start + 2 * 5313 * 4] = bytes(2 * 5313 * 4)
+    cull.restore(bytes(fresh))
+    cull.train(100, batch=4096, lr=2e-3, under_weight=300.0, seed=2)
+    p_restarted = cull.params()
+    d_resumed = np.abs(p_resumed - p_continued).max()
+    d_restarted = np.abs(p_restarted - p_continued).max()
+    print(f"max |resumed - continued| {d_resumed:.3g}, |restarted - continued| {d_restarted:.3g}")
+    assert d_resumed < 1e-3 and d_resumed < 0.05 * d_restarted
+
+    cull.restore(blob)
+    bad_nan = bytearray(blob)
+    bad_nan[32:36] = np.array([np.nan], np.float32).tobytes()
+    for bad in (blob[:-4], b"X" + blob[1:], bytes(bad_nan)):
+        with pytest.raises(gs.GsRangeError):
+            cull.restore(bad)
+    assert cull.checkpoint() == blob
+    p.dispose()

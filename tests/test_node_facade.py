@@ -351,3 +351,46 @@ def test_node_trains_calibrates_and_enables_the_cull(gs, gpu_available, tmp_path
     assert got["shift"] is not None
     assert got["keysCull"] < 0.7 * got["keysPlain"]
     assert np.mean(got["ps"]) >= 44.0
+
+
+@pytest.mark.gpu
+def test_node_cull_checkpoint_round_trip(gs, gpu_available, tmp_path):
+    """The learned cull's training checkpoint from JavaScript: checkpoint()
+    returns the library's blob, restore() resumes from it (threshold
+    included) and a truncated blob throws RangeError with the state kept."""
+    scene = gs.workload_scene("small", count=20_000)
+    cam = gs.WORKLOADS["small"].camera()
+    payload = {"scene": write_scene(tmp_path, scene), "camera": camera_json(gs, cam)}
+    got = run_node(LOAD_SCENE + """
+      const pass = facade.gaussianPass(camera(input.camera), data, {initialIntersectionCapacity: 1 << 20});
+      const cull = new facade.LearnedCull(pass, 2);
+      cull.reserve(1 << 16);
+      cull.collect(camera(input.camera), 320, 240, 1.0, 1);
+      cull.fitNormalization();
+      cull.train(50, {batch: 4096, underWeight: 300, seed: 1});
+      cull.threshold = 2e-4;
+      cull.disable();
+      const blob = cull.checkpoint();
+      cull.train(20, {batch: 4096, underWeight: 300, seed: 2});
+      const moved = cull.params();
+      cull.threshold = 1e-4;
+      cull.disable();
+      cull.restore(blob);
+      const again = cull.checkpoint();
+      let same = again.length === blob.length;
+      for (let i = 0; same && i < blob.length; i++) same = again[i] === blob[i];
+      let error = null;
+      try { cull.restore(blob.subarray(0, blob.length - 4)); } catch (e) { error = e.name; }
+      const kept = cull.checkpoint();
+      let keptSame = true;
+      for (let i = 0; keptSame && i < blob.length; i++) keptSame = kept[i] === blob[i];
+      const p = cull.params();
+      let diff = 0;
+      for (let i = 0; i < p.length; i++) diff = Math.max(diff, Math.abs(p[i] - moved[i]));
+      pass.dispose();
+      console.log(JSON.stringify({bytes: blob.length, same, error, keptSame,
+                                  threshold: cull.threshold, diff}));""", payload)
+    assert got["bytes"] == 32 + (3 * 5313 + 2 * 16) * 4
+    assert got["same"] and got["keptSame"] and got["error"] == "RangeError"
+    assert abs(got["threshold"] - 2e-4) < 1e-9
+    assert got["diff"] > 0.0  # the restore took the parameters back before the 20 steps
